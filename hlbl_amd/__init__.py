@@ -1,0 +1,361 @@
+"""hlbl_amd: the per-pair connected HLbL kernel sum on MI355X (gfx950).
+
+Python mirror of the reference's host interface for this path (ctypes over the
+C ABI of include/hlbl_amd.h):
+
+  Lattice / xunit                 include/hlbl/lattice.hpp, include/hlbl/physics.hpp
+  fill_random_block, RandomProvider  src/provider.cpp:10-69
+  Contractor.kernel_sum           src/contract.cpp:104-114
+  cu_dzu_dzsu / cu_4pt_contraction   kernels/cuda_lattice.h:49-57
+  build_schedule, check_complete, simulate, max_steps_without_wrap
+                                  src/schedule.cpp:34-214
+
+Device buffers are passed as integer addresses (e.g. torch.Tensor.data_ptr()).
+The library must be built (hlbl_amd.build); there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhlbl_amd.so")
+
+
+class HlblError(RuntimeError):
+    pass
+
+
+class Geom(C.Structure):
+    _fields_ = [("T", C.c_int), ("LX", C.c_int), ("LY", C.c_int), ("LZ", C.c_int)]
+
+
+class Coord(C.Structure):
+    _fields_ = [("t", C.c_int), ("x", C.c_int), ("y", C.c_int), ("z", C.c_int)]
+
+
+class Pair(C.Structure):
+    _fields_ = [("a", C.c_double), ("b", C.c_double)]
+
+
+class PairDesc(C.Structure):
+    _fields_ = [("d_x", C.c_void_p), ("d_y", C.c_void_p), ("gsx", Coord), ("yv", Coord)]
+
+
+def _geom(v):
+    return Geom(*[int(a) for a in v])
+
+
+def _coord(v):
+    return Coord(*[int(a) for a in v])
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "hlbl_last_error": (C.c_char_p, []),
+    "hlbl_xunit": (C.c_double, [C.c_double, C.c_double]),
+    "hlbl_source_seed": (C.c_uint64, [C.c_uint64, Coord]),
+    "hlbl_block_length": (C.c_size_t, [Geom]),
+    "hlbl_fill_random_block_host": (C.c_int, [C.c_void_p, Geom, Coord, C.c_uint64, C.c_int]),
+    "hlbl_fill_random_block_device": (C.c_int, [C.c_void_p, Geom, Coord, C.c_uint64, C.c_void_p]),
+    "hlbl_qed_init": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    "hlbl_qed_free": (C.c_int, [C.c_void_p]),
+    "hlbl_qed_table_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_double)]),
+    "hlbl_qed_table_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "hlbl_qed_kernel": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                  C.c_void_p, C.c_void_p]),
+    "hlbl_qed_channels": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                    C.c_void_p]),
+    "hlbl_mlp_param_count": (C.c_int, [C.c_void_p]),
+    "hlbl_mlp_init": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "hlbl_mlp_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64,
+                                 C.c_void_p, C.c_void_p]),
+    "hlbl_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "hlbl_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "hlbl_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                   C.c_void_p]),
+    "hlbl_mlp_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                     C.POINTER(C.c_float), C.c_void_p]),
+    "hlbl_contractor_create": (C.c_int, [C.POINTER(C.c_void_p), Geom, Geom, Coord, C.c_void_p]),
+    "hlbl_contractor_free": (C.c_int, [C.c_void_p]),
+    "hlbl_kernel_sum": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, Coord, Coord, C.c_double,
+                                  C.POINTER(C.c_double), C.c_void_p]),
+    "hlbl_dzu_dzsu": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_int, Coord, Coord, Geom, Geom, C.c_void_p]),
+    "hlbl_insert_g5_gamma": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p]),
+    "hlbl_4pt_contraction": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_int, Coord, Coord, Pair,
+                                       Coord, Geom, Geom, C.c_void_p]),
+    "hlbl_window_kernel_sum": (C.c_int, [C.c_void_p, C.POINTER(PairDesc), C.c_int, C.c_double,
+                                         C.c_int, C.POINTER(C.c_double), C.c_void_p]),
+    "hlbl_window_judge": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "hlbl_schedule_build": (C.c_void_p, [Coord, C.c_int, C.c_int, Coord]),
+    "hlbl_schedule_free": (None, [C.c_void_p]),
+    "hlbl_schedule_n_windows": (C.c_int, [C.c_void_p]),
+    "hlbl_schedule_n_sources": (C.c_int, [C.c_void_p]),
+    "hlbl_schedule_source": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                       C.POINTER(Coord), C.POINTER(C.c_int)]),
+    "hlbl_schedule_partner": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(Coord)]),
+    "hlbl_schedule_check_complete": (C.c_int, [C.c_void_p, C.c_int]),
+    "hlbl_schedule_simulate": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_longlong)]),
+    "hlbl_max_steps_without_wrap": (C.c_int, [Coord, C.c_int, Coord, Coord, C.c_int]),
+    "hlbl_run_chain": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_double, C.c_int,
+                                 C.c_int, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+}
+
+
+def lib():
+    """The loaded C library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HlblError(f"{LIB_PATH} is not built: run python -m hlbl_amd.build")
+        L = C.CDLL(LIB_PATH)
+        missing = [n for n in _SIGS if not hasattr(L, n)]
+        if missing:
+            raise HlblError(f"{LIB_PATH} lacks {missing}: rebuild with python -m hlbl_amd.build")
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise HlblError(lib().hlbl_last_error().decode())
+
+
+def xunit(muon_mass_mev, lattice_spacing_fm):
+    """m_mu a / (hbar c) (include/hlbl/physics.hpp:9-14)."""
+    return lib().hlbl_xunit(muon_mass_mev, lattice_spacing_fm)
+
+
+class Lattice:
+    """Local/global extents and process coordinates; one process here."""
+
+    def __init__(self, local, procs=(1, 1, 1, 1), proc_coords=(0, 0, 0, 0)):
+        self.local = tuple(int(v) for v in local)
+        self.procs = tuple(int(v) for v in procs)
+        self.proc_coords = tuple(int(v) for v in proc_coords)
+        self.global_ = tuple(l * p for l, p in zip(self.local, self.procs))
+
+    def volume(self):
+        v = 1
+        for a in self.local:
+            v *= a
+        return v
+
+    def local_index(self, c):
+        t, x, y, z = c
+        return ((t * self.local[1] + x) * self.local[2] + y) * self.local[3] + z
+
+    def site_map_diff(self, a, b):
+        """Minimum-image a - b in [-L/2, L/2) (include/hlbl/vec4.hpp:40-44)."""
+        out = []
+        for ai, bi, L in zip(a, b, self.global_):
+            r = (ai - bi) % L
+            out.append(r - L if r >= L // 2 else r)
+        return tuple(out)
+
+
+def block_length(lattice):
+    return int(lib().hlbl_block_length(_geom(lattice.local)))
+
+
+def source_seed(seed, gsx):
+    return int(lib().hlbl_source_seed(C.c_uint64(seed), _coord(gsx)))
+
+
+def fill_random_block_host(lattice, seed, out_ptr, n_threads=0):
+    """Host fill of one block at out_ptr (block_length doubles)."""
+    check(lib().hlbl_fill_random_block_host(C.c_void_p(out_ptr), _geom(lattice.local),
+                                            _coord(lattice.proc_coords), C.c_uint64(seed),
+                                            n_threads))
+
+
+def fill_random_block_device(lattice, seed, d_out, stream=None):
+    check(lib().hlbl_fill_random_block_device(C.c_void_p(d_out), _geom(lattice.local),
+                                              _coord(lattice.proc_coords), C.c_uint64(seed),
+                                              stream))
+
+
+class RandomProvider:
+    """A different seeded random field per source position (src/provider.cpp:59-69)."""
+
+    def __init__(self, lattice, seed=0):
+        self.lattice = lattice
+        self.seed = seed
+
+    def compute(self, gsx, d_out, stream=None):
+        fill_random_block_device(self.lattice, source_seed(self.seed, gsx), d_out, stream)
+        return 0
+
+
+class QedKernel:
+    """L^(Lambda), Lambda = 0.4 (KQED's initialise / QED_Mkernel_L2 replacement)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().hlbl_qed_init(C.byref(h), device))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            check(lib().hlbl_qed_free(self.handle))
+            self.handle = None
+
+    def table_info(self):
+        vals = [C.c_int() for _ in range(4)]
+        rad = C.c_double()
+        check(lib().hlbl_qed_table_info(self.handle, *[C.byref(v) for v in vals], C.byref(rad)))
+        return tuple(v.value for v in vals) + (rad.value,)
+
+    def kernel(self, d_x, d_y, n, d_kerv, mode=0, stream=None):
+        check(lib().hlbl_qed_kernel(self.handle, mode, C.c_void_p(d_x), C.c_void_p(d_y), n,
+                                    C.c_void_p(d_kerv), stream))
+
+    def channels(self, d_acb, n, d_out, mode=0, stream=None):
+        check(lib().hlbl_qed_channels(self.handle, mode, C.c_void_p(d_acb), n,
+                                      C.c_void_p(d_out), stream))
+
+
+class Contractor:
+    """The connected kernel sum of a pair (include/hlbl/contract.hpp:42-94)."""
+
+    def __init__(self, lattice, qed):
+        h = C.c_void_p()
+        check(lib().hlbl_contractor_create(C.byref(h), _geom(lattice.global_),
+                                           _geom(lattice.local), _coord(lattice.proc_coords),
+                                           qed.handle))
+        self.handle = h
+        self.lattice = lattice
+        self.qed = qed
+
+    def close(self):
+        if self.handle:
+            check(lib().hlbl_contractor_free(self.handle))
+            self.handle = None
+
+    def kernel_sum(self, d_fwd_src, d_fwd_y, gsx, yv, xunit_, stream=None):
+        """[flavour 0, flavour 1] connected kernel sum of x = gsx, y = gsx + yv (FP64)."""
+        out = (C.c_double * 2)()
+        check(lib().hlbl_kernel_sum(self.handle, C.c_void_p(d_fwd_src), C.c_void_p(d_fwd_y),
+                                    _coord(gsx), _coord(yv), xunit_, out, stream))
+        return [out[0], out[1]]
+
+    def window_kernel_sum(self, pairs, xunit_, form=1, stream=None):
+        """pairs: list of (d_x, d_y, gsx, yv). Returns [[f0, f1], ...]."""
+        n = len(pairs)
+        arr = (PairDesc * n)()
+        for i, (dx, dy, gsx, yv) in enumerate(pairs):
+            arr[i] = PairDesc(C.c_void_p(dx), C.c_void_p(dy), _coord(gsx), _coord(yv))
+        out = (C.c_double * (2 * n))()
+        check(lib().hlbl_window_kernel_sum(self.handle, arr, n, xunit_, form, out, stream))
+        return [[out[2 * i], out[2 * i + 1]] for i in range(n)]
+
+    def judge(self):
+        e = C.c_double()
+        n = C.c_int()
+        check(lib().hlbl_window_judge(self.handle, C.byref(e), C.byref(n)))
+        return e.value, n.value
+
+    # seam 2: the launchers of kernels/cuda_lattice.h
+    def dzu_dzsu(self, d_dzu, d_dzsu, fwd_src, fwd_y, iflavor, gsx, stream=None):
+        L = self.lattice
+        check(lib().hlbl_dzu_dzsu(self.handle, C.c_void_p(d_dzu), C.c_void_p(d_dzsu),
+                                  C.c_void_p(fwd_src), C.c_void_p(fwd_y), iflavor,
+                                  _coord(L.proc_coords), _coord(gsx), _geom(L.global_),
+                                  _geom(L.local), stream))
+
+    def four_pt(self, d_kernel_sum, d_g_dzu, d_g_dzsu, fwd_src, fwd_y, iflavor, gsx, xunit_, yv,
+                stream=None):
+        L = self.lattice
+        check(lib().hlbl_4pt_contraction(self.handle, C.c_void_p(d_kernel_sum),
+                                         C.c_void_p(d_g_dzu), C.c_void_p(d_g_dzsu),
+                                         C.c_void_p(fwd_src), C.c_void_p(fwd_y), iflavor,
+                                         _coord(L.proc_coords), _coord(gsx), Pair(xunit_, 0.0),
+                                         _coord(yv), _geom(L.global_), _geom(L.local), stream))
+
+
+def insert_g5_gamma(d_dzu, d_dzsu, d_g_dzu, d_g_dzsu, stream=None):
+    check(lib().hlbl_insert_g5_gamma(C.c_void_p(d_dzu), C.c_void_p(d_dzsu), C.c_void_p(d_g_dzu),
+                                     C.c_void_p(d_g_dzsu), stream))
+
+
+class Schedule:
+    """A built sliding-hypercube schedule (src/schedule.cpp:61-91)."""
+
+    def __init__(self, h, stride, n_steps, origin=(0, 0, 0, 0)):
+        p = lib().hlbl_schedule_build(_coord(h), stride, n_steps, _coord(origin))
+        if not p:
+            raise HlblError(lib().hlbl_last_error().decode())
+        self.handle = C.c_void_p(p)
+        self.stride = stride
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.hlbl_schedule_free(self.handle)
+            self.handle = None
+
+    def n_windows(self):
+        return lib().hlbl_schedule_n_windows(self.handle)
+
+    def sources(self):
+        """[(window, position, [partners...]), ...] in run order."""
+        out = []
+        L = lib()
+        for i in range(L.hlbl_schedule_n_sources(self.handle)):
+            w = C.c_int()
+            pos = Coord()
+            npart = C.c_int()
+            check(L.hlbl_schedule_source(self.handle, i, C.byref(w), C.byref(pos),
+                                         C.byref(npart)))
+            parts = []
+            for j in range(npart.value):
+                q = Coord()
+                check(L.hlbl_schedule_partner(self.handle, i, j, C.byref(q)))
+                parts.append((q.t, q.x, q.y, q.z))
+            out.append((w.value, (pos.t, pos.x, pos.y, pos.z), parts))
+        return out
+
+    def pairs(self):
+        """(older, newer) in run order (all_pairs, schedule.cpp:101-109)."""
+        return [(q, p) for _w, p, parts in self.sources() for q in parts]
+
+    def check_complete(self):
+        check(lib().hlbl_schedule_check_complete(self.handle, self.stride))
+
+    def simulate(self, interleaved=True):
+        st = (C.c_longlong * 5)()
+        check(lib().hlbl_schedule_simulate(self.handle, 1 if interleaved else 0, st))
+        keys = ("n_propagators", "n_solves", "n_pairs", "n_kernel_sums", "peak_cache")
+        return dict(zip(keys, [int(v) for v in st]))
+
+
+def max_steps_without_wrap(h, stride, lattice, origin=(0, 0, 0, 0), cap=10000):
+    return lib().hlbl_max_steps_without_wrap(_coord(h), stride, _coord(lattice), _coord(origin),
+                                             cap)
+
+
+def run_chain(contractor, schedule, seed, xunit_, residency=0, form=0, batch=0):
+    """The schedule loop on random fields (src/driver.cpp:23-165).
+
+    Returns (rows, stats); rows are dicts {x, y, yv, window, chain, kernel_sum}."""
+    n_pairs = len(schedule.pairs())
+    buf = (C.c_double * (16 * max(1, n_pairs)))()
+    st = (C.c_double * 5)()
+    check(lib().hlbl_run_chain(contractor.handle, schedule.handle, C.c_uint64(seed), xunit_,
+                               residency, form, batch, buf, n_pairs, st))
+    rows = []
+    for i in range(n_pairs):
+        r = buf[16 * i:16 * i + 16]
+        rows.append({"x": tuple(int(v) for v in r[0:4]), "y": tuple(int(v) for v in r[4:8]),
+                     "yv": tuple(int(v) for v in r[8:12]), "window": int(r[12]),
+                     "chain": int(r[13]), "kernel_sum": [r[14], r[15]]})
+    keys = ("pairs", "positions_solved", "cache_peak", "seconds_contract", "seconds_solve")
+    return rows, dict(zip(keys, list(st)))

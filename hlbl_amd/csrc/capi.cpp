@@ -1,0 +1,586 @@
+// The C ABI: argument checks, status codes, and the host orchestration of
+// the per-pair contraction (Contractor::kernel_sum, src/contract.cpp:104-114).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/hlbl_amd.h"
+#include "context.hpp"
+#include "contract_kernels.hpp"
+#include "provider.hpp"
+#include "qed_host.hpp"
+#include "schedule.hpp"
+
+namespace hlbl {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+
+hipStream_t as_stream(hlbl_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+void check_geom(const hlbl_geom& g, const char* what) {
+  if (g.T <= 0 || g.LX <= 0 || g.LY <= 0 || g.LZ <= 0)
+    throw Error(std::string(what) + ": extents must be positive");
+}
+
+void check_layout(const hlbl_geom& global, const hlbl_geom& local, const hlbl_coord& proc) {
+  check_geom(global, "global geometry");
+  check_geom(local, "local geometry");
+  for (int r = 0; r < 4; ++r) {
+    const int Lg = geom_dim(global, r), Ll = geom_dim(local, r), p = coord_dim(proc, r);
+    if (Lg % Ll != 0) throw Error("local extents must divide the global extents");
+    if (p < 0 || p * Ll >= Lg) throw Error("process coordinates out of range");
+    if (Lg % 2 != 0) throw Error("global extents must be even");
+  }
+}
+
+void check_ptr(const void* p, const char* what) {
+  if (!p) throw Error(std::string(what) + " is null");
+}
+
+int device_cu_count(int device) {
+  int n = 0;
+  check_hip(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device),
+            "hipDeviceGetAttribute");
+  return n;
+}
+
+}  // namespace
+
+}  // namespace hlbl
+
+using namespace hlbl;
+
+extern "C" {
+
+const char* hlbl_last_error(void) { return g_last_error.c_str(); }
+
+double hlbl_xunit(double muon_mass_mev, double lattice_spacing_fm) {
+  return muon_mass_mev / qed::kHbarC * lattice_spacing_fm;
+}
+
+// ---- provider ------------------------------------------------------------------
+
+uint64_t hlbl_source_seed(uint64_t seed, hlbl_coord gsx) {
+  const int c[4] = {gsx.t, gsx.x, gsx.y, gsx.z};
+  for (int i = 0; i < 4; ++i) seed = hash64(seed ^ (uint64_t)(int64_t)(c[i] + 1));
+  return seed;
+}
+
+size_t hlbl_block_length(hlbl_geom local) { return (size_t)2 * 12 * 24 * geom_volume(local); }
+
+int hlbl_fill_random_block_host(double* out, hlbl_geom local, hlbl_coord proc, uint64_t seed,
+                                int n_threads) {
+  return guarded([&] {
+    check_ptr(out, "out");
+    check_geom(local, "local geometry");
+    const size_t V = geom_volume(local);
+    int nt = n_threads > 0 ? n_threads : (int)std::thread::hardware_concurrency();
+    if (nt < 1) nt = 1;
+    auto work = [&](size_t s0, size_t s1) {
+      for (size_t site = s0; site < s1; ++site) {
+        int c[4];
+        site_coords(site, local, c);
+        const uint64_t g = site_key(proc.t * local.T + c[0], proc.x * local.LX + c[1],
+                                    proc.y * local.LY + c[2], proc.z * local.LZ + c[3]);
+        for (int f = 0; f < 2; ++f)
+          for (int a = 0; a < 12; ++a) {
+            double* dst = out + (((size_t)(f * 12 + a)) * V + site) * 24;
+            for (int cc = 0; cc < 24; ++cc) dst[cc] = hash_to_unit(element_key(g, f, a, cc, seed));
+          }
+      }
+    };
+    std::vector<std::thread> pool;
+    const size_t chunk = (V + nt - 1) / nt;
+    for (int t = 0; t < nt; ++t) {
+      const size_t s0 = t * chunk, s1 = std::min(V, s0 + chunk);
+      if (s0 < s1) pool.emplace_back(work, s0, s1);
+    }
+    for (auto& th : pool) th.join();
+  });
+}
+
+int hlbl_fill_random_block_device(double* d_out, hlbl_geom local, hlbl_coord proc, uint64_t seed,
+                                  hlbl_stream stream) {
+  return guarded([&] {
+    check_ptr(d_out, "d_out");
+    check_geom(local, "local geometry");
+    launch_fill_random(d_out, local, proc, seed, as_stream(stream));
+  });
+}
+
+// ---- QED kernel -------------------------------------------------------------------
+
+int hlbl_qed_init(hlbl_qed** out, int device) {
+  return guarded([&] {
+    check_ptr(out, "out");
+    *out = nullptr;
+    check_hip(hipSetDevice(device), "hipSetDevice");
+    auto q = new hlbl_qed();
+    try {
+      q->device = device;
+      qed::TermTable tt;
+      qed::build_term_table(&tt);
+      upload_terms_fp64(tt);
+      upload_terms_fp32(tt);
+      upload_terms_qed(tt);
+      qed::build_weight_table(q->host_table);
+      const size_t n = q->host_table.size();
+      q->table64.ensure(n);
+      check_hip(hipMemcpy(q->table64.p, q->host_table.data(), n * sizeof(double),
+                          hipMemcpyHostToDevice), "hipMemcpy table");
+      std::vector<float> t32(q->host_table.begin(), q->host_table.end());
+      q->table32.ensure(n);
+      check_hip(hipMemcpy(q->table32.p, t32.data(), n * sizeof(float), hipMemcpyHostToDevice),
+                "hipMemcpy table");
+    } catch (...) {
+      delete q;
+      throw;
+    }
+    *out = q;
+  });
+}
+
+int hlbl_qed_free(hlbl_qed* q) {
+  return guarded([&] { delete q; });
+}
+
+int hlbl_qed_table_info(const hlbl_qed* q, int* n_channels, int* n_a, int* n_c, int* n_b,
+                        double* radius) {
+  return guarded([&] {
+    check_ptr(q, "qed");
+    if (n_channels) *n_channels = qed::kNCh;
+    if (n_a) *n_a = qed::kNA;
+    if (n_c) *n_c = qed::kNC;
+    if (n_b) *n_b = qed::kNB;
+    if (radius) *radius = qed::kRadius;
+  });
+}
+
+int hlbl_qed_table_copy(const hlbl_qed* q, double* host_out, size_t n_doubles) {
+  return guarded([&] {
+    check_ptr(q, "qed");
+    check_ptr(host_out, "host_out");
+    if (n_doubles < q->host_table.size()) throw Error("hlbl_qed_table_copy: output too small");
+    std::memcpy(host_out, q->host_table.data(), q->host_table.size() * sizeof(double));
+  });
+}
+
+int hlbl_qed_kernel(hlbl_qed* q, int mode, const double* d_x, const double* d_y, int n,
+                    double* d_kerv, hlbl_stream stream) {
+  return guarded([&] {
+    check_ptr(q, "qed");
+    if (n < 0) throw Error("n < 0");
+    if (n > 0) {
+      check_ptr(d_x, "d_x");
+      check_ptr(d_y, "d_y");
+      check_ptr(d_kerv, "d_kerv");
+    }
+    launch_qed_kernel(q, mode, d_x, d_y, n, d_kerv, as_stream(stream));
+  });
+}
+
+int hlbl_qed_channels(hlbl_qed* q, int mode, const double* d_acb, int n, double* d_out,
+                      hlbl_stream stream) {
+  return guarded([&] {
+    check_ptr(q, "qed");
+    if (n < 0) throw Error("n < 0");
+    if (n > 0) {
+      check_ptr(d_acb, "d_acb");
+      check_ptr(d_out, "d_out");
+    }
+    launch_qed_channels(q, mode, d_acb, n, d_out, as_stream(stream));
+  });
+}
+
+// ---- contraction ------------------------------------------------------------------
+
+int hlbl_contractor_create(hlbl_contractor** out, hlbl_geom global, hlbl_geom local,
+                           hlbl_coord proc_coords, hlbl_qed* qed) {
+  return guarded([&] {
+    check_ptr(out, "out");
+    check_ptr(qed, "qed");
+    *out = nullptr;
+    check_layout(global, local, proc_coords);
+    auto c = new hlbl_contractor();
+    try {
+      c->global = global;
+      c->local = local;
+      c->proc = proc_coords;
+      c->qed = qed;
+      c->volume = geom_volume(local);
+      check_hip(hipSetDevice(qed->device), "hipSetDevice");
+      c->n_cu = device_cu_count(qed->device);
+      c->nwg1 = 2 * c->n_cu;
+      c->nwg2 = 2 * c->n_cu;
+      c->result.ensure(2);
+      c->host_result.ensure(2);
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+int hlbl_contractor_free(hlbl_contractor* c) {
+  return guarded([&] { delete c; });
+}
+
+int hlbl_kernel_sum(hlbl_contractor* c, const double* d_fwd_src, const double* d_fwd_y,
+                    hlbl_coord gsx, hlbl_coord yv, double xunit, double out[2],
+                    hlbl_stream stream) {
+  return guarded([&] {
+    check_ptr(c, "contractor");
+    check_ptr(d_fwd_src, "d_fwd_src");
+    check_ptr(d_fwd_y, "d_fwd_y");
+    check_ptr(out, "out");
+    hipStream_t st = as_stream(stream);
+    launch_pass1(c, d_fwd_src, d_fwd_y, c->proc, gsx, c->global, c->local, 3, st);
+    c->gdz.ensure(2 * 40 * 144 * 2);
+    c->bmat.ensure(2 * 288 * 48);
+    launch_gamma(c->dz.p, c->gdz.p, st);
+    launch_bmat(c->gdz.p, c->bmat.p, st);
+    launch_pass2(c, d_fwd_src, d_fwd_y, c->bmat.p, c->proc, gsx, yv, xunit, c->global, c->local,
+                 3, c->result.p, 0, 0, st);
+    check_hip(hipMemcpyAsync(c->host_result.p, c->result.p, 2 * sizeof(double),
+                             hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    check_hip(hipStreamSynchronize(st), "kernel_sum");
+    out[0] = c->host_result.p[0];
+    out[1] = c->host_result.p[1];
+  });
+}
+
+int hlbl_dzu_dzsu(hlbl_contractor* c, double* d_dzu, double* d_dzsu, const double* fwd_src,
+                  const double* fwd_y, int iflavor, hlbl_coord proc_coords, hlbl_coord gsx,
+                  hlbl_geom global, hlbl_geom local, hlbl_stream stream) {
+  return guarded([&] {
+    check_ptr(c, "contractor");
+    check_ptr(d_dzu, "d_dzu");
+    check_ptr(d_dzsu, "d_dzsu");
+    check_ptr(fwd_src, "fwd_src");
+    check_ptr(fwd_y, "fwd_y");
+    if (iflavor != 0 && iflavor != 1) throw Error("iflavor must be 0 or 1");
+    check_layout(global, local, proc_coords);
+    hipStream_t st = as_stream(stream);
+    launch_pass1(c, fwd_src, fwd_y, proc_coords, gsx, global, local, 1 << iflavor, st);
+    // add this flavour's dzu / dzsu to the caller's buffers
+    const double* dzf = c->dz.p + (size_t)iflavor * 10 * 144 * 2;
+    launch_add(d_dzu, dzf, 6 * 144 * 2, st);
+    launch_add(d_dzsu, dzf + 6 * 144 * 2, 4 * 144 * 2, st);
+    check_hip(hipStreamSynchronize(st), "dzu_dzsu");
+  });
+}
+
+int hlbl_insert_g5_gamma(const double* d_dzu, const double* d_dzsu, double* d_g_dzu,
+                         double* d_g_dzsu, hlbl_stream stream) {
+  return guarded([&] {
+    check_ptr(d_dzu, "d_dzu");
+    check_ptr(d_dzsu, "d_dzsu");
+    check_ptr(d_g_dzu, "d_g_dzu");
+    check_ptr(d_g_dzsu, "d_g_dzsu");
+    hipStream_t st = as_stream(stream);
+    // stage one flavour into the two-flavour layout of form_gdz
+    double *dz = nullptr, *gdz = nullptr;
+    check_hip(hipMallocAsync((void**)&dz, 2 * 10 * 144 * 2 * sizeof(double), st), "hipMallocAsync");
+    check_hip(hipMallocAsync((void**)&gdz, 2 * 40 * 144 * 2 * sizeof(double), st), "hipMallocAsync");
+    check_hip(hipMemsetAsync(dz, 0, 2 * 10 * 144 * 2 * sizeof(double), st), "memset");
+    check_hip(hipMemcpyAsync(dz, d_dzu, 6 * 144 * 2 * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+    check_hip(hipMemcpyAsync(dz + 6 * 144 * 2, d_dzsu, 4 * 144 * 2 * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+    launch_gamma(dz, gdz, st);
+    check_hip(hipMemcpyAsync(d_g_dzu, gdz, 6 * 4 * 144 * 2 * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+    check_hip(hipMemcpyAsync(d_g_dzsu, gdz + 6 * 4 * 144 * 2, 4 * 4 * 144 * 2 * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+    (void)hipFreeAsync(dz, st);
+    (void)hipFreeAsync(gdz, st);
+    check_hip(hipStreamSynchronize(st), "insert_g5_gamma");
+  });
+}
+
+int hlbl_4pt_contraction(hlbl_contractor* c, double* d_kernel_sum, const double* d_g_dzu,
+                         const double* d_g_dzsu, const double* fwd_src, const double* fwd_y,
+                         int iflavor, hlbl_coord proc_coords, hlbl_coord gsx, hlbl_pair xunit,
+                         hlbl_coord yv, hlbl_geom global, hlbl_geom local,
+                         hlbl_stream stream) {
+  return guarded([&] {
+    check_ptr(c, "contractor");
+    check_ptr(d_kernel_sum, "d_kernel_sum");
+    check_ptr(d_g_dzu, "d_g_dzu");
+    check_ptr(d_g_dzsu, "d_g_dzsu");
+    check_ptr(fwd_src, "fwd_src");
+    check_ptr(fwd_y, "fwd_y");
+    if (iflavor != 0 && iflavor != 1) throw Error("iflavor must be 0 or 1");
+    check_layout(global, local, proc_coords);
+    hipStream_t st = as_stream(stream);
+    c->gdz.ensure(2 * 40 * 144 * 2);
+    c->bmat.ensure(2 * 288 * 48);
+    double* gf = c->gdz.p + (size_t)iflavor * 40 * 144 * 2;
+    check_hip(hipMemcpyAsync(gf, d_g_dzu, 6 * 4 * 144 * 2 * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+    check_hip(hipMemcpyAsync(gf + 6 * 4 * 144 * 2, d_g_dzsu, 4 * 4 * 144 * 2 * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+    launch_bmat(c->gdz.p, c->bmat.p, st);
+    launch_pass2(c, fwd_src, fwd_y, c->bmat.p, proc_coords, gsx, yv, xunit.a, global, local,
+                 1 << iflavor, d_kernel_sum, 1, 1, st);
+    check_hip(hipStreamSynchronize(st), "4pt_contraction");
+  });
+}
+
+// ---- schedule -----------------------------------------------------------------------
+
+}  // extern "C"
+
+struct hlbl_schedule {
+  hlbl::Schedule steps;
+  // flattened sources in run order
+  std::vector<std::pair<int, const hlbl::Source*>> flat;
+};
+
+namespace {
+hlbl::Vec4 v4(const hlbl_coord& c) { return {c.t, c.x, c.y, c.z}; }
+hlbl_coord c4(const hlbl::Vec4& v) { return {v[0], v[1], v[2], v[3]}; }
+}  // namespace
+
+extern "C" {
+
+hlbl_schedule* hlbl_schedule_build(hlbl_coord h, int stride, int n_steps, hlbl_coord origin) {
+  hlbl_schedule* s = nullptr;
+  int rc = guarded([&] {
+    for (int v : {h.t, h.x, h.y, h.z})
+      if (v != 1 && v != -1) throw Error("hlbl_schedule_build: h must be a sign vector");
+    if (stride <= 0) throw Error("hlbl_schedule_build: stride must be positive");
+    if (n_steps < 0) throw Error("hlbl_schedule_build: n_steps must be >= 0");
+    s = new hlbl_schedule();
+    s->steps = build_schedule(v4(h), stride, n_steps, v4(origin));
+    for (const auto& st : s->steps)
+      for (const auto& src : st.sources) s->flat.emplace_back(st.index, &src);
+  });
+  return rc == 0 ? s : nullptr;
+}
+
+void hlbl_schedule_free(hlbl_schedule* s) { delete s; }
+
+int hlbl_schedule_n_windows(const hlbl_schedule* s) { return s ? (int)s->steps.size() : -1; }
+int hlbl_schedule_n_sources(const hlbl_schedule* s) { return s ? (int)s->flat.size() : -1; }
+
+int hlbl_schedule_source(const hlbl_schedule* s, int i, int* window, hlbl_coord* pos,
+                         int* n_partners) {
+  return guarded([&] {
+    check_ptr(s, "schedule");
+    if (i < 0 || i >= (int)s->flat.size()) throw Error("source index out of range");
+    if (window) *window = s->flat[i].first;
+    if (pos) *pos = c4(s->flat[i].second->position);
+    if (n_partners) *n_partners = (int)s->flat[i].second->partners.size();
+  });
+}
+
+int hlbl_schedule_partner(const hlbl_schedule* s, int i, int j, hlbl_coord* pos) {
+  return guarded([&] {
+    check_ptr(s, "schedule");
+    if (i < 0 || i >= (int)s->flat.size()) throw Error("source index out of range");
+    const auto& p = s->flat[i].second->partners;
+    if (j < 0 || j >= (int)p.size()) throw Error("partner index out of range");
+    check_ptr(pos, "pos");
+    *pos = c4(p[j]);
+  });
+}
+
+int hlbl_schedule_check_complete(const hlbl_schedule* s, int stride) {
+  return guarded([&] {
+    check_ptr(s, "schedule");
+    check_complete(s->steps, stride);
+  });
+}
+
+int hlbl_schedule_simulate(const hlbl_schedule* s, int mode, long long stats[5]) {
+  return guarded([&] {
+    check_ptr(s, "schedule");
+    check_ptr(stats, "stats");
+    const ScheduleStats r = simulate(s->steps, mode == 1);
+    stats[0] = r.n_propagators;
+    stats[1] = r.n_solves;
+    stats[2] = r.n_pairs;
+    stats[3] = r.n_kernel_sums;
+    stats[4] = r.peak_cache;
+  });
+}
+
+int hlbl_max_steps_without_wrap(hlbl_coord h, int stride, hlbl_coord lattice, hlbl_coord origin,
+                                int cap) {
+  return max_steps_without_wrap(v4(h), stride, v4(lattice), v4(origin), cap);
+}
+
+}  // extern "C"
+
+// ---- window entry, surrogate, driver ------------------------------------------------
+
+namespace hlbl {
+
+void window_run(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit, int form,
+                double* out, hipStream_t st);
+void window_judge_result(const hlbl_contractor* c, double* err, int* n);
+
+struct PairRow {
+  Vec4 x, y, yv;
+  int window;
+  double ks[2];
+};
+void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xunit, bool host_tier,
+               int form, bool batch, std::vector<PairRow>& rows, double stats[5]);
+
+// the window entry for every form; form 0 runs the FP64 pair path per pair
+void window_entry(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit, int form,
+                  double* out, hipStream_t st) {
+  if (form < 0 || form > 2) throw Error("form must be 0, 1 or 2");
+  for (int i = 0; i < n; ++i)
+    if (!pairs[i].d_x || !pairs[i].d_y) throw Error("a pair has a null block");
+  if (n == 0) return;
+  if (form == 0) {
+    for (int i = 0; i < n; ++i)
+      if (hlbl_kernel_sum(c, pairs[i].d_x, pairs[i].d_y, pairs[i].gsx, pairs[i].yv, xunit, out + 2 * i, st) != 0)
+        throw Error(hlbl_last_error());
+    return;
+  }
+  window_run(c, pairs, n, xunit, form, out, st);
+}
+
+}  // namespace hlbl
+
+#include "mlp.hpp"
+
+extern "C" {
+
+int hlbl_window_kernel_sum(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n_pairs,
+                           double xunit, int form, double* out, hlbl_stream stream) {
+  return guarded([&] {
+    check_ptr(c, "contractor");
+    if (n_pairs < 0) throw Error("n_pairs < 0");
+    if (n_pairs > 0) {
+      check_ptr(pairs, "pairs");
+      check_ptr(out, "out");
+    }
+    window_entry(c, pairs, n_pairs, xunit, form, out, as_stream(stream));
+  });
+}
+
+int hlbl_window_judge(const hlbl_contractor* c, double* max_rel_err, int* n_samples) {
+  return guarded([&] {
+    check_ptr(c, "contractor");
+    check_ptr(max_rel_err, "max_rel_err");
+    check_ptr(n_samples, "n_samples");
+    window_judge_result(c, max_rel_err, n_samples);
+  });
+}
+
+int hlbl_mlp_param_count(const hlbl_qed* q) { return q ? mlp_layout().total : -1; }
+
+int hlbl_mlp_init(hlbl_qed* q, uint64_t seed) {
+  return guarded([&] {
+    check_ptr(q, "qed");
+    check_hip(hipSetDevice(q->device), "hipSetDevice");
+    mlp_configure_kernels();
+    mlp_init_params(q, seed);
+    q->mlp_ready = false;
+  });
+}
+
+int hlbl_mlp_train(hlbl_qed* q, int n_samples, int epochs, int batch, float lr, uint64_t seed,
+                   float* loss, hlbl_stream stream) {
+  return guarded([&] {
+    check_ptr(q, "qed");
+    if (n_samples <= 0 || epochs <= 0 || batch <= 0 || batch > n_samples)
+      throw Error("hlbl_mlp_train: need 0 < batch <= n_samples and epochs > 0");
+    check_hip(hipSetDevice(q->device), "hipSetDevice");
+    mlp_configure_kernels();
+    mlp_train(q, n_samples, epochs, batch, lr, seed, loss, as_stream(stream));
+  });
+}
+
+int hlbl_mlp_get_params(const hlbl_qed* q, float* host, int n) {
+  return guarded([&] {
+    check_ptr(q, "qed");
+    check_ptr(host, "host");
+    const int total = mlp_layout().total;
+    if (!q->mlp_w.p) throw Error("hlbl_mlp_get_params: the network is not initialised");
+    if (n < total) throw Error("hlbl_mlp_get_params: buffer too small");
+    check_hip(hipMemcpy(host, q->mlp_w.p, total * sizeof(float), hipMemcpyDeviceToHost), "copy params");
+  });
+}
+
+int hlbl_mlp_set_params(hlbl_qed* q, const float* host, int n) {
+  return guarded([&] {
+    check_ptr(q, "qed");
+    check_ptr(host, "host");
+    const int total = mlp_layout().total;
+    if (n != total) throw Error("hlbl_mlp_set_params: wrong parameter count");
+    if (!q->mlp_w.p) mlp_init_params(q, 0);
+    check_hip(hipMemcpy(q->mlp_w.p, host, total * sizeof(float), hipMemcpyHostToDevice), "copy params");
+    mlp_requantise(q, nullptr);
+    check_hip(hipDeviceSynchronize(), "requantise");
+  });
+}
+
+int hlbl_mlp_forward(hlbl_qed* q, const float* d_in, int n, float* d_out, int precision,
+                     hlbl_stream stream) {
+  return guarded([&] {
+    check_ptr(q, "qed");
+    if (!q->mlp_w.p) throw Error("hlbl_mlp_forward: the network is not initialised");
+    if (n < 0) throw Error("n < 0");
+    if (n == 0) return;
+    check_ptr(d_in, "d_in");
+    check_ptr(d_out, "d_out");
+    mlp_configure_kernels();
+    if (precision == 0) mlp_forward_fp32(q, d_in, n, d_out, as_stream(stream));
+    else mlp_infer(q, d_in, n, d_out, as_stream(stream));
+    check_hip(hipStreamSynchronize(as_stream(stream)), "mlp forward");
+  });
+}
+
+int hlbl_mlp_loss_grad(hlbl_qed* q, const float* d_in, const float* d_target, int n,
+                       float* d_grad, float* loss_host, hlbl_stream stream) {
+  return guarded([&] {
+    check_ptr(q, "qed");
+    if (!q->mlp_w.p) throw Error("hlbl_mlp_loss_grad: the network is not initialised");
+    if (n <= 0) throw Error("n must be positive");
+    check_ptr(d_in, "d_in");
+    check_ptr(d_target, "d_target");
+    check_ptr(d_grad, "d_grad");
+    const double l = mlp_loss_grad(q, d_in, d_target, n, d_grad, as_stream(stream));
+    if (loss_host) *loss_host = (float)l;
+  });
+}
+
+int hlbl_run_chain(hlbl_contractor* c, const hlbl_schedule* s, uint64_t seed, double xunit,
+                   int residency, int form, int batch, double* rows, int max_rows,
+                   double stats[5]) {
+  return guarded([&] {
+    check_ptr(c, "contractor");
+    check_ptr(s, "schedule");
+    check_ptr(stats, "stats");
+    if (residency != 0 && residency != 1) throw Error("residency must be 0 (device) or 1 (host)");
+    std::vector<PairRow> out;
+    run_chain(c, s->steps, seed, xunit, residency == 1, form, batch != 0, out, stats);
+    if ((int)out.size() > max_rows) throw Error("hlbl_run_chain: rows buffer too small");
+    if (!out.empty()) check_ptr(rows, "rows");
+    for (size_t i = 0; i < out.size(); ++i) {
+      double* r = rows + 16 * i;
+      for (int k = 0; k < 4; ++k) {
+        r[k] = out[i].x[k];
+        r[4 + k] = out[i].y[k];
+        r[8 + k] = out[i].yv[k];
+      }
+      r[12] = out[i].window;
+      r[13] = 0;
+      r[14] = out[i].ks[0];
+      r[15] = out[i].ks[1];
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,563 @@
+// MLP surrogate of the QED kernel's weight functions: 3 inputs (|x|/R, |y|/R,
+// cos beta), 4 hidden layers of 128 GELU, 31 outputs (the tabulated channels,
+// divided by an envelope and standardised per channel). Weights are stored in
+// BF16 and used as BF16 values in every forward pass (the FP32 master copy is
+// what the optimiser updates: quantisation-aware training); products are
+// accumulated in FP32. Training (forward, backward, Adam) and inference run on
+// this library's own kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "context.hpp"
+#include "contract_kernels.hpp"
+#include "kpart.hpp"
+#include "mlp.hpp"
+#include "provider.hpp"
+#include "qed.hpp"
+#include "qed_host.hpp"
+#include "window.hpp"
+
+namespace hlbl {
+
+namespace {
+size_t div_up(size_t a, size_t b) { return (a + b - 1) / b; }
+}  // namespace
+
+// round to the nearest BF16 value (ties to even), returned as float
+__host__ __device__ inline float bf16_round(float x) {
+  unsigned u;
+  memcpy(&u, &x, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return x;  // inf / nan unchanged
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  float r;
+  memcpy(&r, &u, 4);
+  return r;
+}
+
+__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ inline float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// ---------------------------------------------------------- generic GEMM --
+// C[i][j] (ldc) = sum_k A(i,k) B(k,j) (+ bias[j]) (+ C if accumulate),
+// A(i,k) = A[i*sai + k*sak], B(k,j) = B[k*sbk + j*sbj]. 64x64 tiles, 4x4 per thread.
+__global__ __launch_bounds__(256) void gemm_f32(int M, int N, int K, const float* __restrict__ A,
+                                                long sai, long sak, const float* __restrict__ B,
+                                                long sbk, long sbj, float* __restrict__ Cm, int ldc,
+                                                const float* __restrict__ bias, int accumulate) {
+  __shared__ float As[16][68];
+  __shared__ float Bs[16][68];
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+  float acc[4][4] = {};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int e = tid; e < 64 * 16; e += 256) {
+      const int ii = e >> 4, kk = e & 15;
+      const int gi = i0 + ii, gk = k0 + kk;
+      As[kk][ii] = (gi < M && gk < K) ? A[gi * sai + gk * sak] : 0.f;
+      const int jj = e >> 4, kb = e & 15;
+      const int gj = j0 + jj, gk2 = k0 + kb;
+      Bs[kb][jj] = (gj < N && gk2 < K) ? B[gk2 * sbk + gj * sbj] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      float a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        a[q] = As[kk][ty * 4 + q];
+        b[q] = Bs[kk][tx * 4 + q];
+      }
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * b[q];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = i0 + ty * 4 + p, j = j0 + tx * 4 + q;
+      if (i < M && j < N) {
+        float v = acc[p][q] + (bias ? bias[j] : 0.f);
+        if (accumulate) v += Cm[(size_t)i * ldc + j];
+        Cm[(size_t)i * ldc + j] = v;
+      }
+    }
+}
+
+void gemm(int M, int N, int K, const float* A, long sai, long sak, const float* B, long sbk,
+          long sbj, float* Cm, int ldc, const float* bias, int accumulate, hipStream_t st) {
+  hipLaunchKernelGGL(gemm_f32, dim3((unsigned)div_up(N, 64), (unsigned)div_up(M, 64)), dim3(256), 0,
+                     st, M, N, K, A, sai, sak, B, sbk, sbj, Cm, ldc, bias, accumulate);
+}
+
+__global__ void k_round_bf16(const float* __restrict__ w, float* __restrict__ wq,
+                             unsigned short* __restrict__ w16, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float r = bf16_round(w[i]);
+  wq[i] = r;
+  unsigned u;
+  memcpy(&u, &r, 4);
+  if (w16) w16[i] = (unsigned short)(u >> 16);
+}
+
+__global__ void k_gelu(const float* __restrict__ z, float* __restrict__ a, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = gelu(z[i]);
+}
+// dz = da * gelu'(z)
+__global__ void k_gelu_back(const float* __restrict__ z, const float* __restrict__ da,
+                            float* __restrict__ dz, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dz[i] = da[i] * gelu_grad(z[i]);
+}
+// db[j] = sum_b dz[b][j], fixed order
+__global__ void k_colsum(const float* __restrict__ dz, int B, int N, float* __restrict__ db) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  float s = 0.f;
+  for (int b = 0; b < B; ++b) s += dz[(size_t)b * N + j];
+  db[j] = s;
+}
+// dout = 2 (out - t) / (B*C); per-row loss partials (FP64)
+__global__ void k_mse(const float* __restrict__ out, const float* __restrict__ t, int B, int Cn,
+                      float* __restrict__ dout, double* __restrict__ row_loss) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  double l = 0.0;
+  const float inv = 2.0f / (float)(B * Cn);
+  for (int c = 0; c < Cn; ++c) {
+    const float d = out[(size_t)b * Cn + c] - t[(size_t)b * Cn + c];
+    l += (double)d * d;
+    dout[(size_t)b * Cn + c] = inv * d;
+  }
+  row_loss[b] = l;
+}
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+__global__ void k_gather_rows(const float* __restrict__ src, const int* __restrict__ idx, int B,
+                              int width, float* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B * width) return;
+  const int b = i / width, c = i % width;
+  dst[i] = src[(size_t)idx[b] * width + c];
+}
+
+// ------------------------------------------------------------ the network --
+
+struct MlpWork {
+  int B = 0;
+  DevBuf<float> z[5], a[5], dz, da, dtmp;  // pre-activations and activations per layer
+  DevBuf<double> row_loss;
+};
+
+// forward of B rows (x: [B][3]) with weights wq (BF16-rounded floats); keeps z/a
+void mlp_forward_train(const float* wq, const float* x, int B, MlpWork& wk, hipStream_t st) {
+  const MlpLayout L = mlp_layout();
+  const float* in = x;
+  for (int l = 0; l < 5; ++l) {
+    wk.z[l].ensure((size_t)B * L.out[l]);
+    // z = in W^T + b: A(i,k) = in[i][k], B(k,j) = W[j][k]
+    gemm(B, L.out[l], L.in[l], in, L.in[l], 1, wq + L.w[l], 1, L.in[l], wk.z[l].p, L.out[l],
+         wq + L.b[l], 0, st);
+    if (l < 4) {
+      wk.a[l].ensure((size_t)B * L.out[l]);
+      const size_t n = (size_t)B * L.out[l];
+      hipLaunchKernelGGL(k_gelu, dim3((unsigned)div_up(n, 256)), dim3(256), 0, st, wk.z[l].p, wk.a[l].p, n);
+      in = wk.a[l].p;
+    }
+  }
+}
+
+// loss and gradient (into grad, all parameters) of a batch
+double mlp_backward(const float* wq, const float* x, const float* target, int B, MlpWork& wk,
+                    float* grad, hipStream_t st, bool want_loss) {
+  const MlpLayout L = mlp_layout();
+  mlp_forward_train(wq, x, B, wk, st);
+  wk.da.ensure((size_t)B * kMlpHidden);
+  wk.dz.ensure((size_t)B * kMlpHidden);
+  wk.dtmp.ensure((size_t)B * kMlpHidden);
+  wk.row_loss.ensure(B);
+  // output layer: dz4 = dL/dz4 (no activation)
+  hipLaunchKernelGGL(k_mse, dim3((unsigned)div_up(B, 256)), dim3(256), 0, st, wk.z[4].p, target, B,
+                     kMlpOut, wk.dz.p, wk.row_loss.p);
+  for (int l = 4; l >= 0; --l) {
+    const float* a_in = l == 0 ? x : wk.a[l - 1].p;
+    // dW[j][k] = sum_b dz[b][j] a_in[b][k]
+    gemm(L.out[l], L.in[l], B, wk.dz.p, 1, L.out[l], a_in, L.in[l], 1, grad + L.w[l], L.in[l],
+         nullptr, 0, st);
+    hipLaunchKernelGGL(k_colsum, dim3((unsigned)div_up(L.out[l], 128)), dim3(128), 0, st, wk.dz.p, B,
+                       L.out[l], grad + L.b[l]);
+    if (l > 0) {
+      // da_in[b][k] = sum_j dz[b][j] W[j][k]
+      gemm(B, L.in[l], L.out[l], wk.dz.p, L.out[l], 1, wq + L.w[l], L.in[l], 1, wk.dtmp.p, L.in[l],
+           nullptr, 0, st);
+      const size_t n = (size_t)B * L.in[l];
+      hipLaunchKernelGGL(k_gelu_back, dim3((unsigned)div_up(n, 256)), dim3(256), 0, st,
+                         wk.z[l - 1].p, wk.dtmp.p, wk.dz.p, n);
+    }
+  }
+  check_hip(hipGetLastError(), "mlp backward launch");
+  if (!want_loss) return 0.0;
+  std::vector<double> rl(B);
+  check_hip(hipMemcpyAsync(rl.data(), wk.row_loss.p, B * sizeof(double), hipMemcpyDeviceToHost, st), "copy loss");
+  check_hip(hipStreamSynchronize(st), "mlp loss");
+  double s = 0.0;
+  for (double v : rl) s += v;
+  return s / ((double)B * kMlpOut);
+}
+
+void mlp_forward_fp32(hlbl_qed* q, const float* x, int n, float* y, hipStream_t st) {
+  MlpWork wk;
+  mlp_forward_train(q->mlp_w.p, x, n, wk, st);
+  check_hip(hipMemcpyAsync(y, wk.z[4].p, (size_t)n * kMlpOut * sizeof(float), hipMemcpyDeviceToDevice, st), "copy");
+  check_hip(hipStreamSynchronize(st), "mlp forward");
+}
+
+double mlp_loss_grad(hlbl_qed* q, const float* x, const float* t, int n, float* grad, hipStream_t st) {
+  MlpWork wk;
+  mlp_requantise(q, st);
+  return mlp_backward(q->mlp_wq.p, x, t, n, wk, grad, st, true);
+}
+
+// ---------------------------------------------------- training-set and stats --
+
+// envelope dividing the channels: exp(1 - (sqrt(1+a^2) + sqrt(1+b^2)) / 2)
+__host__ __device__ inline double mlp_envelope(double a, double b) {
+  return exp(1.0 - 0.5 * (sqrt(1.0 + a * a) + sqrt(1.0 + b * b)));
+}
+
+// counter-based uniform in [0, 1)
+__host__ __device__ inline double mlp_uniform(uint64_t seed, uint64_t i, int k) {
+  return (double)(hash64(seed * 0x9E3779B97F4A7C15ULL + i * 3 + k) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+void mlp_sample_inputs(uint64_t seed, size_t i, double& a, double& c, double& b) {
+  a = qed::kRadius * mlp_uniform(seed, i, 0);
+  b = qed::kRadius * mlp_uniform(seed, i, 1);
+  c = 2.0 * mlp_uniform(seed, i, 2) - 1.0;
+}
+
+// exact channels of n samples (host threads): x [n][3] normalised inputs, t [n][31] raw/envelope
+void mlp_build_set(uint64_t seed, int n, std::vector<float>& x, std::vector<double>& t) {
+  x.resize((size_t)n * 3);
+  t.resize((size_t)n * kMlpOut);
+  int nt = (int)std::thread::hardware_concurrency();
+  if (nt < 1) nt = 1;
+  if (nt > 16) nt = 16;
+  std::vector<std::thread> pool;
+  for (int th = 0; th < nt; ++th)
+    pool.emplace_back([&, th] {
+      double ch[qed::kNCh];
+      for (int i = th; i < n; i += nt) {
+        double a, c, b;
+        mlp_sample_inputs(seed, (size_t)i, a, c, b);
+        qed::weight_channels(a * a, b * b, a * b * c, ch);
+        const double env = mlp_envelope(a, b);
+        x[(size_t)i * 3 + 0] = (float)(a / qed::kRadius);
+        x[(size_t)i * 3 + 1] = (float)(b / qed::kRadius);
+        x[(size_t)i * 3 + 2] = (float)c;
+        for (int k = 0; k < kMlpOut; ++k) t[(size_t)i * kMlpOut + k] = ch[k] / env;
+      }
+    });
+  for (auto& p : pool) p.join();
+}
+
+// ---------------------------------------------------------------- inference --
+// Persistent FP32-MFMA forward: all BF16 weights (as floats) in LDS would not fit,
+// so the weights stay in BF16 in LDS (108 KB) and the 64-row activation tile in
+// FP32 (32 KB). Rows are generated by a functor of the row index.
+typedef float f4m __attribute__((ext_vector_type(4)));
+
+constexpr int kInfThreads = 256;
+constexpr int kInfRows = 64;
+
+__device__ inline float bf16_to_f(unsigned short h) {
+  unsigned u = (unsigned)h << 16;
+  float r;
+  memcpy(&r, &u, 4);
+  return r;
+}
+
+// One 64-row tile through the network. act holds the inputs [64][128] (first 3
+// columns used), out receives the 31 outputs of row r via out_fn(r, c, v).
+template <class OutFn>
+__device__ void mlp_tile(const unsigned short* __restrict__ w16, float* act, OutFn out_fn) {
+  const MlpLayout L = mlp_layout();
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int row0 = wave * 16;  // this wave's 16 rows
+  for (int l = 0; l < 5; ++l) {
+    const int K = L.in[l], N = L.out[l];
+    const int ksteps = (K + 3) / 4, nblk = (N + 15) / 16;
+    f4m c[8];
+#pragma unroll
+    for (int cb = 0; cb < 8; ++cb) {
+      const int col = 16 * cb + (lane & 15);
+      const float bias = (cb < nblk && col < N) ? bf16_to_f(w16[L.b[l] + col]) : 0.f;
+      c[cb] = {bias, bias, bias, bias};
+    }
+    for (int s = 0; s < ksteps; ++s) {
+      const int k = 4 * s + (lane >> 4);
+      const float a = k < K ? act[(row0 + (lane & 15)) * kMlpHidden + k] : 0.f;
+#pragma unroll
+      for (int cb = 0; cb < 8; ++cb) {
+        if (cb >= nblk) break;
+        const int col = 16 * cb + (lane & 15);
+        const float b = (k < K && col < N) ? bf16_to_f(w16[L.w[l] + col * K + k]) : 0.f;
+        c[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c[cb], 0, 0, 0);
+      }
+    }
+    __syncthreads();  // every wave has read act
+#pragma unroll
+    for (int cb = 0; cb < 8; ++cb) {
+      if (cb >= nblk) break;
+      const int col = 16 * cb + (lane & 15);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = row0 + 4 * (lane >> 4) + i;
+        if (l < 4) {
+          act[row * kMlpHidden + col] = gelu(c[cb][i]);
+        } else if (col < N) {
+          out_fn(row, col, c[cb][i]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// plain forward of n rows x [n][3] -> y [n][31] (normalised), BF16 weights
+__global__ __launch_bounds__(kInfThreads) void mlp_infer_rows(const unsigned short* __restrict__ w16g,
+                                                              const float* __restrict__ x, int n,
+                                                              float* __restrict__ y) {
+  extern __shared__ unsigned char smem[];
+  unsigned short* w16 = reinterpret_cast<unsigned short*>(smem);
+  float* act = reinterpret_cast<float*>(smem + kMlpParamsPadded * 2);
+  for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) w16[i] = w16g[i];
+  __syncthreads();
+  for (int t0 = blockIdx.x * kInfRows; t0 < n; t0 += gridDim.x * kInfRows) {
+    for (int e = threadIdx.x; e < kInfRows * 3; e += blockDim.x) {
+      const int r = e / 3, k = e % 3;
+      act[r * kMlpHidden + k] = (t0 + r < n) ? x[(size_t)(t0 + r) * 3 + k] : 0.f;
+    }
+    __syncthreads();
+    mlp_tile(w16, act, [&](int r, int c, float v) {
+      if (t0 + r < n) y[(size_t)(t0 + r) * kMlpOut + c] = v;
+    });
+  }
+}
+
+// Channels of the six site evaluations of every site for every distinct yv
+// (gsx = 0): chbuf[u][site][e][32] = envelope * (mean + sd * net).
+__global__ __launch_bounds__(kInfThreads) void mlp_surrogate_channels(
+    const unsigned short* __restrict__ w16g, const float* __restrict__ scale,
+    const hlbl_coord* __restrict__ yvs, int n_unique, hlbl_geom global, double xunit,
+    float* __restrict__ chbuf) {
+  extern __shared__ unsigned char smem[];
+  unsigned short* w16 = reinterpret_cast<unsigned short*>(smem);
+  float* act = reinterpret_cast<float*>(smem + kMlpParamsPadded * 2);
+  __shared__ float s_env[kInfRows];
+  __shared__ float s_mean[kMlpOut], s_sd[kMlpOut];
+  for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) w16[i] = w16g[i];
+  for (int i = threadIdx.x; i < kMlpOut; i += blockDim.x) {
+    s_mean[i] = scale[i];
+    s_sd[i] = scale[kMlpOut + i];
+  }
+  __syncthreads();
+  const size_t V = geom_volume(global);
+  const size_t rows = (size_t)n_unique * V * 6;
+  const hlbl_coord zero = {0, 0, 0, 0};
+  const float inv_r = (float)(1.0 / qed::kRadius);
+  for (size_t t0 = (size_t)blockIdx.x * kInfRows; t0 < rows; t0 += (size_t)gridDim.x * kInfRows) {
+    if (threadIdx.x < kInfRows) {
+      const int r = threadIdx.x;
+      const size_t row = t0 + r;
+      float a = 0.f, c = 0.f, b = 0.f, env = 0.f;
+      if (row < rows) {
+        const int e = (int)(row % 6);
+        const size_t site = (row / 6) % V;
+        const int u = (int)(row / (6 * V));
+        SiteGeom<float> g;
+        site_geom(site, global, global, zero, zero, yvs[u], xunit, g);
+        float pu[4], pv[4];
+        eval_args(e, g, pu, pv);
+        qed::acb(pu, pv, a, c, b);
+        env = (float)mlp_envelope(a, b);
+      }
+      act[r * kMlpHidden + 0] = a * inv_r;
+      act[r * kMlpHidden + 1] = b * inv_r;
+      act[r * kMlpHidden + 2] = c;
+      s_env[r] = env;
+    }
+    __syncthreads();
+    mlp_tile(w16, act, [&](int r, int c, float v) {
+      const size_t row = t0 + r;
+      if (row < rows) chbuf[row * qed::kNCh + c] = s_env[r] * (s_mean[c] + s_sd[c] * v);
+    });
+  }
+}
+
+size_t mlp_infer_smem() { return (size_t)kMlpParamsPadded * 2 + (size_t)kInfRows * kMlpHidden * 4; }
+
+void launch_surrogate_channels(hlbl_contractor* c, WindowPlan& w, hipStream_t st) {
+  hlbl_qed* q = c->qed;
+  const int nwg = 2 * c->n_cu;
+  hipLaunchKernelGGL(mlp_surrogate_channels, dim3(nwg), dim3(kInfThreads), mlp_infer_smem(), st,
+                     q->mlp_w_bf16.p, q->mlp_scale.p, w.d_yvs.p, w.n_unique, c->global, w.xunit,
+                     w.chbuf.p);
+  check_hip(hipGetLastError(), "mlp_surrogate_channels launch");
+}
+
+void mlp_infer(hlbl_qed* q, const float* x, int n, float* y, hipStream_t st) {
+  int dev = 0, ncu = 0;
+  check_hip(hipGetDevice(&dev), "hipGetDevice");
+  check_hip(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "attr");
+  const int nwg = (int)std::min<size_t>(div_up(n, kInfRows), (size_t)2 * ncu);
+  if (nwg <= 0) return;
+  hipLaunchKernelGGL(mlp_infer_rows, dim3(nwg), dim3(kInfThreads), mlp_infer_smem(), st,
+                     q->mlp_w_bf16.p, x, n, y);
+  check_hip(hipGetLastError(), "mlp_infer_rows launch");
+}
+
+void mlp_configure_kernels() {
+  check_hip(hipFuncSetAttribute((const void*)mlp_infer_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)mlp_infer_smem()), "hipFuncSetAttribute");
+  check_hip(hipFuncSetAttribute((const void*)mlp_surrogate_channels,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)mlp_infer_smem()),
+            "hipFuncSetAttribute");
+}
+
+// ------------------------------------------------------------------ training --
+
+void mlp_init_params(hlbl_qed* q, uint64_t seed) {
+  const MlpLayout L = mlp_layout();
+  std::vector<float> w(L.total, 0.f);
+  for (int l = 0; l < 5; ++l) {
+    // uniform(-s, s), s = sqrt(6 / (fan_in + fan_out)) (Glorot), biases 0
+    const double s = std::sqrt(6.0 / (L.in[l] + L.out[l]));
+    for (int i = 0; i < L.in[l] * L.out[l]; ++i)
+      w[L.w[l] + i] = (float)(s * (2.0 * mlp_uniform(seed + 17 * l, (uint64_t)i, 0) - 1.0));
+  }
+  q->mlp_w.ensure(L.total);
+  check_hip(hipMemcpy(q->mlp_w.p, w.data(), L.total * sizeof(float), hipMemcpyHostToDevice), "copy params");
+  q->mlp_wq.ensure(L.total);
+  q->mlp_w_bf16.ensure(L.total);
+  hipLaunchKernelGGL(k_round_bf16, dim3((unsigned)div_up(L.total, 256)), dim3(256), 0, 0, q->mlp_w.p,
+                     q->mlp_wq.p, q->mlp_w_bf16.p, L.total);
+  check_hip(hipDeviceSynchronize(), "mlp init");
+  if (!q->mlp_scale.p) {
+    std::vector<float> sc(2 * kMlpOut);
+    for (int k = 0; k < kMlpOut; ++k) {
+      sc[k] = 0.f;
+      sc[kMlpOut + k] = 1.f;
+    }
+    q->mlp_scale.ensure(2 * kMlpOut);
+    check_hip(hipMemcpy(q->mlp_scale.p, sc.data(), sc.size() * sizeof(float), hipMemcpyHostToDevice), "copy scale");
+  }
+}
+
+void mlp_requantise(hlbl_qed* q, hipStream_t st) {
+  const int n = mlp_layout().total;
+  hipLaunchKernelGGL(k_round_bf16, dim3((unsigned)div_up(n, 256)), dim3(256), 0, st, q->mlp_w.p,
+                     q->mlp_wq.p, q->mlp_w_bf16.p, n);
+}
+
+void mlp_train(hlbl_qed* q, int n_samples, int epochs, int batch, float lr, uint64_t seed,
+               float* loss_hist, hipStream_t st) {
+  const MlpLayout L = mlp_layout();
+  if (!q->mlp_w.p) mlp_init_params(q, seed ^ 0x5eedULL);
+  std::vector<float> x;
+  std::vector<double> t;
+  mlp_build_set(seed, n_samples, x, t);
+  // per-channel standardisation of the enveloped targets
+  std::vector<float> sc(2 * kMlpOut);
+  std::vector<float> tn((size_t)n_samples * kMlpOut);
+  for (int k = 0; k < kMlpOut; ++k) {
+    double m = 0, v = 0;
+    for (int i = 0; i < n_samples; ++i) m += t[(size_t)i * kMlpOut + k];
+    m /= n_samples;
+    for (int i = 0; i < n_samples; ++i) {
+      const double d = t[(size_t)i * kMlpOut + k] - m;
+      v += d * d;
+    }
+    const double sd = std::sqrt(v / n_samples) + 1e-30;
+    sc[k] = (float)m;
+    sc[kMlpOut + k] = (float)sd;
+    for (int i = 0; i < n_samples; ++i) tn[(size_t)i * kMlpOut + k] = (float)((t[(size_t)i * kMlpOut + k] - m) / sd);
+  }
+  q->mlp_scale.ensure(2 * kMlpOut);
+  check_hip(hipMemcpy(q->mlp_scale.p, sc.data(), sc.size() * sizeof(float), hipMemcpyHostToDevice), "copy scale");
+  DevBuf<float> dx, dt, bx, bt, grad, m, v;
+  dx.ensure(x.size());
+  dt.ensure(tn.size());
+  check_hip(hipMemcpy(dx.p, x.data(), x.size() * 4, hipMemcpyHostToDevice), "copy x");
+  check_hip(hipMemcpy(dt.p, tn.data(), tn.size() * 4, hipMemcpyHostToDevice), "copy t");
+  bx.ensure((size_t)batch * 3);
+  bt.ensure((size_t)batch * kMlpOut);
+  grad.ensure(L.total);
+  m.ensure(L.total);
+  v.ensure(L.total);
+  check_hip(hipMemsetAsync(m.p, 0, L.total * 4, st), "memset");
+  check_hip(hipMemsetAsync(v.p, 0, L.total * 4, st), "memset");
+  DevBuf<int> didx;
+  didx.ensure(batch);
+  std::vector<int> idx(n_samples), bidx(batch);
+  for (int i = 0; i < n_samples; ++i) idx[i] = i;
+  MlpWork wk;
+  long step = 0;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const int steps_per_epoch = n_samples / batch;
+  for (int ep = 0; ep < epochs; ++ep) {
+    // deterministic shuffle
+    for (int i = n_samples - 1; i > 0; --i) {
+      const int j = (int)(hash64(seed + 1000003ULL * ep + i) % (uint64_t)(i + 1));
+      std::swap(idx[i], idx[j]);
+    }
+    // cosine learning-rate decay over the run
+    double eloss = 0.0;
+    for (int s = 0; s < steps_per_epoch; ++s) {
+      for (int b = 0; b < batch; ++b) bidx[b] = idx[(size_t)s * batch + b];
+      check_hip(hipMemcpyAsync(didx.p, bidx.data(), batch * 4, hipMemcpyHostToDevice, st), "copy idx");
+      hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)div_up(batch * 3, 256)), dim3(256), 0, st, dx.p, didx.p, batch, 3, bx.p);
+      hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)div_up(batch * kMlpOut, 256)), dim3(256), 0, st, dt.p, didx.p, batch, kMlpOut, bt.p);
+      mlp_requantise(q, st);
+      const bool want = (s == steps_per_epoch - 1);
+      const double l = mlp_backward(q->mlp_wq.p, bx.p, bt.p, batch, wk, grad.p, st, want);
+      if (want) eloss = l;
+      ++step;
+      const double frac = (double)step / ((double)epochs * steps_per_epoch);
+      const float lr_t = (float)(lr * 0.5 * (1.0 + std::cos(3.14159265358979 * frac)));
+      const float c1 = 1.f - std::pow(b1, (float)step), c2 = 1.f - std::pow(b2, (float)step);
+      hipLaunchKernelGGL(k_adam, dim3((unsigned)div_up(L.total, 256)), dim3(256), 0, st, q->mlp_w.p,
+                         grad.p, m.p, v.p, L.total, lr_t, b1, b2, 1e-8f, c1, c2);
+      // the staging copy of bidx is synchronous with respect to the host vector reuse
+      check_hip(hipStreamSynchronize(st), "mlp step");
+    }
+    if (loss_hist) loss_hist[ep] = (float)eloss;
+  }
+  mlp_requantise(q, st);
+  check_hip(hipStreamSynchronize(st), "mlp train");
+  q->mlp_ready = true;
+}
+
+}  // namespace hlbl

@@ -1,0 +1,66 @@
+// The window-batched path: per-window plan, buffers and the captured graph.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "context.hpp"
+
+namespace hlbl {
+
+// below this distance (muon units) from the origin, |x| or |x - y|, the
+// mixed-precision forms evaluate the QED kernel in FP64 (Lambda subtraction
+// cancels there)
+constexpr double kNearRadius = 0.25;
+
+// one pair of a window as the kernels read it
+struct WinPair {
+  const double* S;   // fwd_src: block of x
+  const double* Y;   // fwd_y: block of y
+  hlbl_coord gsx;
+  hlbl_coord yv;
+  int yv_slot;       // index of yv among the window's distinct yv
+  int pad;
+};
+
+struct WindowPlan {
+  int n_pairs = 0, n_unique = 0, form = 0, n_samples = 0;
+  double xunit = 0.0;
+  DevBuf<WinPair> d_pairs;
+  DevBuf<hlbl_coord> d_yvs;
+  PinBuf<WinPair> h_pairs;
+  PinBuf<hlbl_coord> h_yvs;
+  DevBuf<double> part1, xi, dz, gdz;
+  DevBuf<float> bmat;
+  DevBuf<float> coef;      // [n_unique][V][640]
+  DevBuf<float> chbuf;     // surrogate channels [n_unique][V][6][32]
+  DevBuf<double> part2, result;
+  PinBuf<double> h_result;
+  DevBuf<size_t> d_samples;
+  DevBuf<double> judge;    // [n_unique][n_samples]
+  PinBuf<double> h_judge;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~WindowPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+// kernels of a window on `st` (recorded into the graph)
+void window_record(hlbl_contractor* c, WindowPlan& w, hipStream_t st);
+size_t window_part1_len(const hlbl_contractor* c, int n_pairs);
+size_t window_part2_len(const hlbl_contractor* c, int n_pairs);
+
+// dz, gdz (FP64) and the B operands (FP64 into bmat64 or FP32 into bmat32) of
+// n_pairs reduced pass-1 sums xi[pair][...]
+void launch_forms_batched(const double* xi, double* dz, double* gdz, double* bmat64,
+                          float* bmat32, int n_pairs, hipStream_t st);
+
+// surrogate channels of the six site evaluations, for every site and yv
+void launch_surrogate_channels(hlbl_contractor* c, WindowPlan& w, hipStream_t st);
+
+}  // namespace hlbl

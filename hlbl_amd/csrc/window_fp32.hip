@@ -1,0 +1,394 @@
+// Window-batched mixed-precision contraction: all pairs of a window and both
+// flavours per call. Pass 1 and pass 2 are each one persistent launch over the
+// pairs. Propagator products run in FP32 (pass-2 traces on FP32 MFMA), the
+// per-workgroup lattice sums and the per-pair sums are FP64, combined with
+// Kahan compensation. The coefficient field of the QED kernel depends only on
+// (x - gsx, yv), so it is built once per distinct yv of the window (at gsx = 0)
+// and read at the shifted site; sites near the origin (|x| or |x - y| below
+// kNearRadius) evaluate the kernel in FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "context.hpp"
+#include "contract_kernels.hpp"
+#include "kpart.hpp"
+#include "pass_common.hpp"
+#include "qed.hpp"
+#include "window.hpp"
+
+namespace hlbl {
+
+__constant__ qed::TermTable c_terms32;
+
+void upload_terms_fp32(const qed::TermTable& t) {
+  check_hip(hipMemcpyToSymbol(HIP_SYMBOL(c_terms32), &t, sizeof(t)), "upload terms");
+}
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+namespace {
+size_t div_up(size_t a, size_t b) { return (a + b - 1) / b; }
+}  // namespace
+
+// ---------------------------------------------------------------- pass 1 --
+// As pass1_fp64 (thread = flavour, spin pair, 3x3 block of (a, b)), with the
+// spinors staged in LDS as FP32 and FP32 colour products; the weighted sums
+// over sites are FP64. One launch runs every pair of the window.
+constexpr int kM1Threads = 256;
+constexpr int kM1Sites = 8;
+constexpr int kXiLenM = 2 * 8 * 5 * 144 * 2;
+
+__global__ __launch_bounds__(kM1Threads) void pass1_mixed(
+    const WinPair* __restrict__ pairs, int n_pairs, hlbl_geom global, hlbl_geom local,
+    hlbl_coord proc, size_t sites_per_wg, int nwg, double* __restrict__ part) {
+  __shared__ float sh[kM1Sites][2][2][12][24];
+  const int tid = threadIdx.x;
+  const int f = tid >> 7;
+  const int p = (tid >> 4) & 7;
+  const int blk = tid & 15;
+  const int a0 = 3 * (blk >> 2), b0 = 3 * (blk & 3);
+  const int s = p & 3;
+  const int sp = (p < 4) ? (s ^ 2) : (3 - s);
+  const size_t V = geom_volume(local);
+  const size_t begin = (size_t)blockIdx.x * sites_per_wg;
+  const size_t end = begin + sites_per_wg < V ? begin + sites_per_wg : V;
+
+  for (int ip = 0; ip < n_pairs; ++ip) {
+    const WinPair wp = pairs[ip];
+    double acc[5][9][2];
+#pragma unroll
+    for (int w = 0; w < 5; ++w)
+#pragma unroll
+      for (int i = 0; i < 9; ++i) acc[w][i][0] = acc[w][i][1] = 0.0;
+    for (size_t z0 = begin; z0 < end; z0 += kM1Sites) {
+      const int ns = (end - z0) < (size_t)kM1Sites ? (int)(end - z0) : kM1Sites;
+      __syncthreads();
+      for (int i = tid; i < 48 * kM1Sites * 24; i += kM1Threads) {
+        const int q = i / (kM1Sites * 24), r = i % (kM1Sites * 24);
+        const int zi = r / 24, c = r % 24;
+        const int bl = q / 24, ff = (q / 12) & 1, aa = q % 12;
+        float v = 0.f;
+        if (zi < ns) v = (float)(bl ? wp.Y : wp.S)[(((size_t)(ff * 12 + aa)) * V + z0 + zi) * 24 + c];
+        sh[zi][bl][ff][aa][c] = v;
+      }
+      __syncthreads();
+      for (int zi = 0; zi < ns; ++zi) {
+        int crd[4];
+        site_coords(z0 + zi, local, crd);
+        double w[5];
+        w[0] = 1.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int L = geom_dim(global, r);
+          const int xr = mod_pos(crd[r] + coord_dim(proc, r) * geom_dim(local, r) - coord_dim(wp.gsx, r), L);
+          w[1 + r] = (double)coord_map_zerohalf(xr, L);
+        }
+        float sr[3][3], si[3][3], yr[3][3], yi[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int col = 0; col < 3; ++col) {
+            const float* ps = &sh[zi][0][f][a0 + i][2 * (3 * sp + col)];
+            const float* py = &sh[zi][1][1 - f][b0 + i][2 * (3 * s + col)];
+            sr[i][col] = ps[0];
+            si[i][col] = ps[1];
+            yr[i][col] = py[0];
+            yi[i][col] = py[1];
+          }
+#pragma unroll
+        for (int ia = 0; ia < 3; ++ia)
+#pragma unroll
+          for (int ib = 0; ib < 3; ++ib) {
+            float re = 0.f, im = 0.f;
+#pragma unroll
+            for (int col = 0; col < 3; ++col) {
+              re += sr[ia][col] * yr[ib][col] + si[ia][col] * yi[ib][col];
+              im += si[ia][col] * yr[ib][col] - sr[ia][col] * yi[ib][col];
+            }
+#pragma unroll
+            for (int ww = 0; ww < 5; ++ww) {
+              acc[ww][ia * 3 + ib][0] += w[ww] * (double)re;
+              acc[ww][ia * 3 + ib][1] += w[ww] * (double)im;
+            }
+          }
+      }
+    }
+    double* out = part + ((size_t)ip * nwg + blockIdx.x) * kXiLenM;
+#pragma unroll
+    for (int ww = 0; ww < 5; ++ww)
+#pragma unroll
+      for (int ia = 0; ia < 3; ++ia)
+#pragma unroll
+        for (int ib = 0; ib < 3; ++ib) {
+          const size_t o = ((((size_t)f * 8 + p) * 5 + ww) * 144 + (a0 + ia) * 12 + (b0 + ib)) * 2;
+          out[o] = acc[ww][ia * 3 + ib][0];
+          out[o + 1] = acc[ww][ia * 3 + ib][1];
+        }
+  }
+}
+
+// xi[pair][i] = Kahan sum over workgroups (fixed order) of part[pair][wg][i]
+__global__ void reduce_window1(const double* __restrict__ part, int nwg, int len,
+                               double* __restrict__ xi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ip = blockIdx.y;
+  if (i >= len) return;
+  Kahan k;
+  const double* p = part + (size_t)ip * nwg * len + i;
+  for (int w = 0; w < nwg; ++w) k.add(p[(size_t)w * len]);
+  xi[(size_t)ip * len + i] = k.s;
+}
+
+// ------------------------------------------------------------ coefficients --
+// The field F_yv[rel site][16][40] (FP32) for each distinct yv, gsx = 0. A site
+// with |x| or |x - y| below kNearRadius is evaluated in FP64 with the FP64
+// table; the others in FP32 from the FP32 table, or from the surrogate's
+// channels chbuf[u][site][6][32] when given.
+constexpr int kKmThreads = 256;
+
+__global__ __launch_bounds__(kKmThreads) void kfield_mixed(
+    const float* __restrict__ table32, const double* __restrict__ table64,
+    const hlbl_coord* __restrict__ yvs, int n_unique, hlbl_geom global, double xunit,
+    size_t sites_per_wg, const float* __restrict__ chbuf, float* __restrict__ coef) {
+  __shared__ double lds64[kKfieldLds];
+  __shared__ float lds32[kKfieldLds];
+  const size_t V = geom_volume(global);
+  const size_t begin = (size_t)blockIdx.x * sites_per_wg;
+  const size_t end = begin + sites_per_wg < V ? begin + sites_per_wg : V;
+  if (begin >= end) return;
+  const hlbl_coord zero = {0, 0, 0, 0};
+  const double near2 = kNearRadius * kNearRadius;
+  for (int u = 0; u < n_unique; ++u) {
+    const hlbl_coord yv = yvs[u];
+    SiteGeom<double> g64;
+    SiteGeom<float> g32;
+    site_geom(0, global, global, zero, zero, yv, xunit, g64);
+    site_geom(0, global, global, zero, zero, yv, xunit, g32);
+    kfield_constants(table64, c_terms32, g64, lds64);
+    // FP32 copies of the pair constants
+    for (int t = threadIdx.x; t < 2 * qed::kNL; t += blockDim.x)
+      lds32[8 * qed::kNCh + 8 * qed::kNT + 6 * qed::kNL + t] = (float)lds64[8 * qed::kNCh + 8 * qed::kNT + 6 * qed::kNL + t];
+    __syncthreads();
+    float* cu = coef + (size_t)u * V * kCoefField;
+    for (size_t site = begin; site < end; ++site) {
+      site_geom(site, global, global, zero, zero, yv, xunit, g64);
+      const bool near = norm2(g64.xm) < near2 || norm2(g64.dm) < near2;
+      if (near) {
+        kfield_site<double, double, float>(table64, c_terms32, g64, nullptr, lds64, cu + site * kCoefField);
+      } else {
+        site_geom(site, global, global, zero, zero, yv, xunit, g32);
+        const float* cb = chbuf ? chbuf + ((size_t)u * V + site) * 6 * qed::kNCh : nullptr;
+        kfield_site<float, float, float>(table32, c_terms32, g32, cb, lds32, cu + site * kCoefField);
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+}
+
+// The surrogate's judge: at sample sites, the FP64 plain coefficients against
+// the field in use; err[u][k] = max |c - c_plain| / max |c_plain| for sample k.
+__global__ __launch_bounds__(kKmThreads) void judge_coef(
+    const double* __restrict__ table64, const hlbl_coord* __restrict__ yvs,
+    hlbl_geom global, double xunit, const size_t* __restrict__ samples, int n_samples,
+    const float* __restrict__ coef, double* __restrict__ err) {
+  __shared__ double lds64[kKfieldLds];
+  __shared__ double ref[kCoefField];
+  __shared__ double red[2][kKmThreads];
+  const int u = blockIdx.y, k = blockIdx.x;
+  const size_t V = geom_volume(global);
+  const hlbl_coord zero = {0, 0, 0, 0};
+  SiteGeom<double> g;
+  site_geom(0, global, global, zero, zero, yvs[u], xunit, g);
+  kfield_constants(table64, c_terms32, g, lds64);
+  const size_t site = samples[k];
+  site_geom(site, global, global, zero, zero, yvs[u], xunit, g);
+  kfield_site<double, double, double>(table64, c_terms32, g, nullptr, lds64, ref);
+  __syncthreads();
+  const float* c = coef + ((size_t)u * V + site) * kCoefField;
+  double dmax = 0.0, rmax = 0.0;
+  for (int t = threadIdx.x; t < kCoefField; t += blockDim.x) {
+    dmax = fmax(dmax, fabs((double)c[t] - ref[t]));
+    rmax = fmax(rmax, fabs(ref[t]));
+  }
+  red[0][threadIdx.x] = dmax;
+  red[1][threadIdx.x] = rmax;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + o]);
+      red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) err[(size_t)u * n_samples + k] = red[1][0] > 0 ? red[0][0] / red[1][0] : red[0][0];
+}
+
+// ---------------------------------------------------------------- pass 2 --
+// As pass2_fp64 on FP32 MFMA: 8 waves = (flavour, quarter of K), three
+// 16-column blocks each. Per lane and site the 12 weighted products are summed
+// in FP32 and added to an FP64 Kahan sum. Persistent over the pairs.
+constexpr int kM2Threads = 512;
+constexpr int kM2Waves = 8;
+constexpr int kM2KSteps = 18;
+constexpr int kM2Sites = 4;
+
+__global__ __launch_bounds__(kM2Threads) void pass2_mixed(
+    const WinPair* __restrict__ pairs, int n_pairs, const float* __restrict__ bmat,
+    const float* __restrict__ coef, hlbl_geom global, size_t sites_per_wg, int nwg,
+    double* __restrict__ part) {
+  __shared__ float sSY[kM2Sites][2][2][12][24];
+  __shared__ float sDxu[kM2Sites][2][4][12][24];
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int f = wave / 4, kq = wave & 3;
+  const size_t V = geom_volume(global);
+  const int arow = lane & 15, amu = arow >> 2, alam = arow & 3;
+  int aoff[6];
+  float asg[6];
+#pragma unroll
+  for (int jq = 0; jq < 6; ++jq) {
+    const int j = 4 * jq + (lane >> 4);
+    const int pj = gamma_perm24(alam, j);
+    aoff[jq] = pj;
+    asg[jq] = (float)(gamma_sign24(4, pj) * gamma_sign24(alam, j));
+  }
+  const size_t begin = (size_t)blockIdx.x * sites_per_wg;
+  const size_t end = begin + sites_per_wg < V ? begin + sites_per_wg : V;
+
+  for (int ip = 0; ip < n_pairs; ++ip) {
+    const WinPair wp = pairs[ip];
+    const float* bm = bmat + (size_t)ip * 2 * 288 * kCoefCols;
+    float breg[3][kM2KSteps];
+#pragma unroll
+    for (int cb = 0; cb < 3; ++cb)
+#pragma unroll
+      for (int i = 0; i < kM2KSteps; ++i)
+        breg[cb][i] = bm[((size_t)f * 288 + 4 * (kq * kM2KSteps + i) + (lane >> 4)) * kCoefCols +
+                         16 * cb + (lane & 15)];
+    const float* cu = coef + (size_t)wp.yv_slot * V * kCoefField;
+    Kahan acc;
+    for (size_t z0 = begin; z0 < end; z0 += kM2Sites) {
+      const int ns = (end - z0) < (size_t)kM2Sites ? (int)(end - z0) : kM2Sites;
+      __syncthreads();
+      for (int i = tid; i < kM2Sites * 1152; i += kM2Threads) {
+        const int zi = i / 1152, rr = i % 1152;
+        const int bl = rr / 576, rem = rr % 576, ff = rem / 288, a = (rem / 24) % 12, c = rem % 24;
+        float v = 0.f;
+        if (zi < ns) v = (float)(bl ? wp.Y : wp.S)[(((size_t)(ff * 12 + a)) * V + z0 + zi) * 24 + c];
+        sSY[zi][bl][ff][a][c] = v;
+      }
+      __syncthreads();
+      for (int t = tid; t < kM2Sites * 288; t += kM2Threads) {
+        const int zi = t / 288, f2 = (t / 144) % 2, r = (t / 12) % 12, a = t % 12;
+        dxu_site<float, float>(sSY[zi][0][1 - f2][a], sSY[zi][1][f2][r], &sDxu[zi][f2][0][r][2 * a], 12 * 24);
+      }
+      __syncthreads();
+      for (int zi = 0; zi < ns; ++zi) {
+        const float* dx = &sDxu[zi][f][amu][0][0];
+        f4v c0 = {0.f, 0.f, 0.f, 0.f}, c1 = c0, c2 = c0;
+#pragma unroll
+        for (int i = 0; i < kM2KSteps; ++i) {
+          const int st = kq * kM2KSteps + i;
+          const int r = st / 6, jq = i % 6;
+          const float a = asg[jq] * dx[r * 24 + aoff[jq]];
+          c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[0][i], c0, 0, 0, 0);
+          c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[1][i], c1, 0, 0, 0);
+          c2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[2][i], c2, 0, 0, 0);
+        }
+        // the coefficient field at the site relative to gsx
+        int crd[4], rel[4];
+        site_coords(z0 + zi, global, crd);
+        for (int r = 0; r < 4; ++r) rel[r] = mod_pos(crd[r] - coord_dim(wp.gsx, r), geom_dim(global, r));
+        const size_t rs = (((size_t)rel[0] * global.LX + rel[1]) * global.LY + rel[2]) * global.LZ + rel[3];
+        const float* cs = cu + rs * kCoefField + (4 * (lane >> 4)) * kCoefUsed + (lane & 15);
+        float v = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          v += c0[i] * cs[i * kCoefUsed] + c1[i] * cs[i * kCoefUsed + 16];
+          if ((lane & 15) < kCoefUsed - 32) v += c2[i] * cs[i * kCoefUsed + 32];
+        }
+        acc.add((double)v);
+      }
+    }
+    // wave sum (pairwise, FP64), one partial per wave
+    double s = acc.s - acc.c;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) part[((size_t)ip * nwg + blockIdx.x) * kM2Waves + wave] = s;
+  }
+}
+
+// out[pair][f] = Kahan sum over workgroups and waves of the pair's partials
+__global__ void reduce_window2(const double* __restrict__ part, int nwg, int n_pairs,
+                               double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (pair, f)
+  if (i >= 2 * n_pairs) return;
+  const int ip = i / 2, f = i % 2;
+  Kahan k;
+  for (int w = 0; w < nwg; ++w)
+    for (int j = 0; j < kM2Waves / 2; ++j) k.add(part[((size_t)ip * nwg + w) * kM2Waves + f * (kM2Waves / 2) + j]);
+  out[i] = k.s;
+}
+
+// ------------------------------------------------------------- launchers --
+
+void window_record(hlbl_contractor* c, WindowPlan& w, hipStream_t st) {
+  const hlbl_geom g = c->global;
+  const size_t V = c->volume;
+  const int np = w.n_pairs, nu = w.n_unique;
+  // pass 1
+  {
+    const int nwg = c->n_cu;
+    const size_t spw = div_up(div_up(V, (size_t)nwg), kM1Sites) * kM1Sites;
+    const int n = (int)div_up(V, spw);
+    hipLaunchKernelGGL(pass1_mixed, dim3(n), dim3(kM1Threads), 0, st, w.d_pairs.p, np, g, g,
+                       c->proc, spw, n, w.part1.p);
+    check_hip(hipGetLastError(), "pass1_mixed launch");
+    hipLaunchKernelGGL(reduce_window1, dim3((unsigned)div_up(kXiLenM, 256), np), dim3(256), 0, st,
+                       w.part1.p, n, kXiLenM, w.xi.p);
+    check_hip(hipGetLastError(), "reduce_window1 launch");
+    launch_forms_batched(w.xi.p, w.dz.p, w.gdz.p, nullptr, w.bmat.p, np, st);
+  }
+  // coefficient fields, one per distinct yv
+  {
+    const int nwg = 4 * c->n_cu;
+    const size_t spw = div_up(V, (size_t)nwg);
+    const int n = (int)div_up(V, spw);
+    hipLaunchKernelGGL(kfield_mixed, dim3(n), dim3(kKmThreads), 0, st, c->qed->table32.p,
+                       c->qed->table64.p, w.d_yvs.p, nu, g, w.xunit, spw,
+                       w.form == 2 ? w.chbuf.p : nullptr, w.coef.p);
+    check_hip(hipGetLastError(), "kfield_mixed launch");
+    if (w.form == 2) {
+      hipLaunchKernelGGL(judge_coef, dim3(w.n_samples, nu), dim3(kKmThreads), 0, st,
+                         c->qed->table64.p, w.d_yvs.p, g, w.xunit, w.d_samples.p, w.n_samples,
+                         w.coef.p, w.judge.p);
+      check_hip(hipGetLastError(), "judge_coef launch");
+    }
+  }
+  // pass 2
+  {
+    const int nwg = 2 * c->n_cu;
+    const size_t spw = div_up(div_up(V, (size_t)nwg), kM2Sites) * kM2Sites;
+    const int n = (int)div_up(V, spw);
+    hipLaunchKernelGGL(pass2_mixed, dim3(n), dim3(kM2Threads), 0, st, w.d_pairs.p, np, w.bmat.p,
+                       w.coef.p, g, spw, n, w.part2.p);
+    check_hip(hipGetLastError(), "pass2_mixed launch");
+    hipLaunchKernelGGL(reduce_window2, dim3((unsigned)div_up(2 * np, 64)), dim3(64), 0, st,
+                       w.part2.p, n, np, w.result.p);
+    check_hip(hipGetLastError(), "reduce_window2 launch");
+  }
+}
+
+size_t window_part1_len(const hlbl_contractor* c, int n_pairs) {
+  return (size_t)n_pairs * c->n_cu * kXiLenM;
+}
+size_t window_part2_len(const hlbl_contractor* c, int n_pairs) {
+  return (size_t)n_pairs * 2 * c->n_cu * kM2Waves;
+}
+
+}  // namespace hlbl

@@ -1,0 +1,168 @@
+// Host side of the window entry: distinct-yv slots, buffers, and one hipGraph
+// per window shape (pairs, distinct yv, form, xunit), replayed for later windows.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "../../include/hlbl_amd.h"
+#include "context.hpp"
+#include "contract_kernels.hpp"
+#include "window.hpp"
+
+namespace hlbl {
+
+namespace {
+
+// canonical (min-image) yv, the key of a coefficient field
+hlbl_coord canonical_yv(const hlbl_geom& g, const hlbl_coord& yv) {
+  hlbl_coord r;
+  r.t = coord_map(mod_pos(yv.t, g.T), g.T);
+  r.x = coord_map(mod_pos(yv.x, g.LX), g.LX);
+  r.y = coord_map(mod_pos(yv.y, g.LY), g.LY);
+  r.z = coord_map(mod_pos(yv.z, g.LZ), g.LZ);
+  return r;
+}
+
+bool same(const hlbl_coord& a, const hlbl_coord& b) {
+  return a.t == b.t && a.x == b.x && a.y == b.y && a.z == b.z;
+}
+
+constexpr int kJudgeSamples = 64;
+
+}  // namespace
+
+struct WindowCache {
+  std::vector<std::unique_ptr<WindowPlan>> plans;
+  hipStream_t stream = nullptr;
+  double last_judge = -1.0;
+  int last_judge_n = 0;
+  ~WindowCache() {
+    plans.clear();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+WindowCache* window_cache(hlbl_contractor* c) {
+  if (!c->window) c->window = new WindowCache();
+  return c->window;
+}
+
+void window_cache_free(WindowCache* w) { delete w; }
+
+void window_judge_result(const hlbl_contractor* c, double* err, int* n) {
+  if (!c->window || c->window->last_judge < 0) throw Error("hlbl_window_judge: no surrogate window has run");
+  *err = c->window->last_judge;
+  *n = c->window->last_judge_n;
+}
+
+void window_run(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit, int form,
+                double* out, hipStream_t user_stream) {
+  const hlbl_geom g = c->global;
+  if (g.T != c->local.T || g.LX != c->local.LX || g.LY != c->local.LY || g.LZ != c->local.LZ)
+    throw Error("hlbl_window_kernel_sum: the mixed-precision forms need the whole lattice on this process");
+  if (form == 2 && !c->qed->mlp_ready)
+    throw Error("hlbl_window_kernel_sum: form 2 needs a trained surrogate (hlbl_mlp_train)");
+  WindowCache* cache = window_cache(c);
+
+  // distinct yv
+  std::vector<hlbl_coord> yvs;
+  std::vector<int> slot(n);
+  for (int i = 0; i < n; ++i) {
+    const hlbl_coord cy = canonical_yv(g, pairs[i].yv);
+    int s = -1;
+    for (size_t u = 0; u < yvs.size(); ++u)
+      if (same(yvs[u], cy)) s = (int)u;
+    if (s < 0) {
+      s = (int)yvs.size();
+      yvs.push_back(cy);
+    }
+    slot[i] = s;
+  }
+  const int nu = (int)yvs.size();
+
+  // find or build the plan
+  WindowPlan* w = nullptr;
+  for (auto& p : cache->plans)
+    if (p->n_pairs == n && p->n_unique == nu && p->form == form && p->xunit == xunit) w = p.get();
+  if (!w) {
+    // one plan at a time keeps the footprint to one window
+    cache->plans.clear();
+    auto p = std::make_unique<WindowPlan>();
+    p->n_pairs = n;
+    p->n_unique = nu;
+    p->form = form;
+    p->xunit = xunit;
+    p->n_samples = kJudgeSamples;
+    const size_t V = c->volume;
+    p->d_pairs.ensure(n);
+    p->h_pairs.ensure(n);
+    p->d_yvs.ensure(nu);
+    p->h_yvs.ensure(nu);
+    p->part1.ensure(window_part1_len(c, n));
+    p->xi.ensure((size_t)n * 2 * 8 * 5 * 144 * 2);
+    p->dz.ensure((size_t)n * 2 * 10 * 144 * 2);
+    p->gdz.ensure((size_t)n * 2 * 40 * 144 * 2);
+    p->bmat.ensure((size_t)n * 2 * 288 * 48);
+    p->coef.ensure((size_t)nu * V * 640);
+    if (form == 2) p->chbuf.ensure((size_t)nu * V * 6 * qed::kNCh);
+    p->part2.ensure(window_part2_len(c, n));
+    p->result.ensure((size_t)2 * n);
+    p->h_result.ensure((size_t)2 * n);
+    // judge samples: a fixed spread of sites
+    std::vector<size_t> smp(kJudgeSamples);
+    for (int k = 0; k < kJudgeSamples; ++k) smp[k] = (size_t)(((unsigned long long)k * 2654435761ULL) % V);
+    p->d_samples.ensure(kJudgeSamples);
+    check_hip(hipMemcpy(p->d_samples.p, smp.data(), smp.size() * sizeof(size_t), hipMemcpyHostToDevice),
+              "hipMemcpy samples");
+    p->judge.ensure((size_t)nu * kJudgeSamples);
+    p->h_judge.ensure((size_t)nu * kJudgeSamples);
+    cache->plans.push_back(std::move(p));
+    w = cache->plans.back().get();
+  }
+
+  for (int i = 0; i < n; ++i) {
+    w->h_pairs.p[i] = WinPair{pairs[i].d_x, pairs[i].d_y, pairs[i].gsx, pairs[i].yv, slot[i], 0};
+  }
+  for (int u = 0; u < nu; ++u) w->h_yvs.p[u] = yvs[u];
+
+  // capture once per plan; graphs need a stream that is not the null stream
+  if (!cache->stream) check_hip(hipStreamCreateWithFlags(&cache->stream, hipStreamNonBlocking), "hipStreamCreate");
+  hipStream_t st = cache->stream;
+  if (!w->exec) {
+    check_hip(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+      check_hip(hipMemcpyAsync(w->d_pairs.p, w->h_pairs.p, n * sizeof(WinPair), hipMemcpyHostToDevice, st), "copy pairs");
+      check_hip(hipMemcpyAsync(w->d_yvs.p, w->h_yvs.p, nu * sizeof(hlbl_coord), hipMemcpyHostToDevice, st), "copy yv");
+      if (form == 2) launch_surrogate_channels(c, *w, st);
+      window_record(c, *w, st);
+      check_hip(hipMemcpyAsync(w->h_result.p, w->result.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, st), "copy result");
+      if (form == 2)
+        check_hip(hipMemcpyAsync(w->h_judge.p, w->judge.p, (size_t)nu * w->n_samples * sizeof(double),
+                                 hipMemcpyDeviceToHost, st), "copy judge");
+    } catch (...) {
+      hipGraph_t gr = nullptr;
+      (void)hipStreamEndCapture(st, &gr);
+      if (gr) (void)hipGraphDestroy(gr);
+      throw;
+    }
+    check_hip(hipStreamEndCapture(st, &w->graph), "hipStreamEndCapture");
+    check_hip(hipGraphInstantiate(&w->exec, w->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  }
+  // order after the caller's earlier work on its stream
+  if (user_stream) check_hip(hipStreamSynchronize(user_stream), "user stream");
+  else check_hip(hipDeviceSynchronize(), "device");
+  check_hip(hipGraphLaunch(w->exec, st), "hipGraphLaunch");
+  check_hip(hipStreamSynchronize(st), "window");
+  for (int i = 0; i < 2 * n; ++i) out[i] = w->h_result.p[i];
+  if (form == 2) {
+    double e = 0.0;
+    for (int k = 0; k < nu * w->n_samples; ++k) e = std::max(e, w->h_judge.p[k]);
+    cache->last_judge = e;
+    cache->last_judge_n = nu * w->n_samples;
+  }
+}
+
+}  // namespace hlbl

@@ -1,0 +1,187 @@
+/*
+ * hlbl_amd: the per-pair connected HLbL kernel sum on MI355X (gfx950).
+ *
+ * A thin C ABI over hand-written HIP kernels. Plain pointers, sizes and POD
+ * structs only; every entry point returns an int status (0 = success) and
+ * never throws or exits. hlbl_last_error() gives the message of the last
+ * failure on the calling thread.
+ *
+ * Propagator blocks keep the reference layout (include/hlbl/provider.hpp:18-20,
+ * src/provider.cpp:45): doubles at ((f*12 + a)*V + s)*24 + 2*c + {re,im} with
+ * flavour f, source spin-colour a, local site s = ((t*LX+x)*LY+y)*LZ+z and sink
+ * spin-colour c = 3*spin + colour, in the cvc gamma basis.
+ */
+#ifndef HLBL_AMD_H
+#define HLBL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- POD types ------------------------------------------------------------
+ * Same members and meaning as Geom, Coord and Pair of kernels/cuda_lattice.h:22-32. */
+typedef struct { int T, LX, LY, LZ; } hlbl_geom;
+typedef struct { int t, x, y, z; } hlbl_coord;
+typedef struct { double a, b; } hlbl_pair; /* {xunit, unused}, src/contract.cpp:85-87 */
+
+/* stream handle: a hipStream_t (NULL = the default stream) */
+typedef void* hlbl_stream;
+
+typedef struct hlbl_qed hlbl_qed;               /* QED kernel state (tables, MLP) */
+typedef struct hlbl_contractor hlbl_contractor; /* work buffers of one lattice  */
+typedef struct hlbl_schedule hlbl_schedule;     /* a built sliding-hypercube schedule */
+
+/* message of the last failed call on this thread ("" if none) */
+const char* hlbl_last_error(void);
+
+/* ---- units ----------------------------------------------------------------
+ * xunit = m_mu a / (hbar c), hbar c = 197.3269804 MeV fm.
+ * Replaces hlbl::xunit, include/hlbl/physics.hpp:9-14. */
+double hlbl_xunit(double muon_mass_mev, double lattice_spacing_fm);
+
+/* ---- propagator provider ----------------------------------------------------
+ * Replaces fill_random_block / RandomProvider::compute, src/provider.cpp:10-69.
+ * hlbl_source_seed folds a source position into the seed (provider.cpp:60-64);
+ * the fills write block_length = 2*12*24*V doubles, bit-identical to the
+ * reference's host generator, for any process grid. */
+uint64_t hlbl_source_seed(uint64_t seed, hlbl_coord gsx);
+size_t hlbl_block_length(hlbl_geom local);
+int hlbl_fill_random_block_host(double* out, hlbl_geom local, hlbl_coord proc_coords,
+                                uint64_t seed, int n_threads);
+int hlbl_fill_random_block_device(double* d_out, hlbl_geom local, hlbl_coord proc_coords,
+                                  uint64_t seed, hlbl_stream stream);
+
+/* ---- QED kernel L^(Lambda), Lambda = 0.4 -----------------------------------
+ * Replaces KQED's initialise(&kqed_t) / free_QED_temps (bin/hlbl_hypercube.cpp:175,226)
+ * and the device call QED_Mkernel_L2(0.4, x, y, kqed_t, kerv) (kernels/cuda_lattice.cu:266-275).
+ * hlbl_qed_init builds the weight-function tables on `device`. */
+int hlbl_qed_init(hlbl_qed** out, int device);
+int hlbl_qed_free(hlbl_qed* q);
+/* weight-function table geometry: channels, points along |x|, cos(beta), |y|, and the
+ * outer radius in muon units (the kernel is 0 beyond it) */
+int hlbl_qed_table_info(const hlbl_qed* q, int* n_channels, int* n_a, int* n_c, int* n_b,
+                        double* radius);
+/* copy the FP64 table [n_a][n_c][n_b][n_channels] to host memory */
+int hlbl_qed_table_copy(const hlbl_qed* q, double* host_out, size_t n_doubles);
+
+/* Kernel values at n argument pairs (muon units): x, y are [n][4] device arrays,
+ * kerv is the [n][6][4][4][4] device output, index [k][mu][nu][lambda] with k over
+ * (rho,sigma) in (0,1),(0,2),(0,3),(1,2),(1,3),(2,3) (cuda_lattice.cu:52-60).
+ * mode 0 = plain (table interpolation, FP64), 1 = surrogate (MLP, FP32/BF16;
+ * plain outside the trained domain). */
+int hlbl_qed_kernel(hlbl_qed* q, int mode, const double* d_x, const double* d_y, int n,
+                    double* d_kerv, hlbl_stream stream);
+/* The interpolated weight-function channels at n points (a=|x|, c=cos beta, b=|y|):
+ * d_acb is [n][3], d_out [n][n_channels]. mode as above. */
+int hlbl_qed_channels(hlbl_qed* q, int mode, const double* d_acb, int n, double* d_out,
+                      hlbl_stream stream);
+
+/* ---- MLP surrogate of the weight functions --------------------------------
+ * 3 inputs (|x|, |y|, cos beta in muon units), 4 hidden layers of 128 GELU,
+ * n_channels outputs. BF16 weights, FP32 accumulation, FP32 master weights.
+ * Trained on the GPU by this library's own forward/backward kernels on samples of
+ * the plain table evaluation. No counterpart in the reference (north-star part). */
+int hlbl_mlp_param_count(const hlbl_qed* q);
+int hlbl_mlp_init(hlbl_qed* q, uint64_t seed);
+/* one full training run; loss history (one value per epoch) goes to host `loss` */
+int hlbl_mlp_train(hlbl_qed* q, int n_samples, int epochs, int batch, float lr,
+                   uint64_t seed, float* loss, hlbl_stream stream);
+/* parameters (FP32 master copy, packed layer by layer: W then b) to/from host */
+int hlbl_mlp_get_params(const hlbl_qed* q, float* host, int n);
+int hlbl_mlp_set_params(hlbl_qed* q, const float* host, int n);
+/* forward on n inputs ([n][3] device, normalised inputs) -> [n][n_channels] outputs
+ * (normalised); precision 0 = FP32 weights, 1 = BF16 weights */
+int hlbl_mlp_forward(hlbl_qed* q, const float* d_in, int n, float* d_out, int precision,
+                     hlbl_stream stream);
+/* mean-squared-error loss and its gradient w.r.t. all parameters (FP32 weights) for a
+ * batch: d_in [n][3], d_target [n][n_channels]; gradient goes to device d_grad */
+int hlbl_mlp_loss_grad(hlbl_qed* q, const float* d_in, const float* d_target, int n,
+                       float* d_grad, float* loss_host, hlbl_stream stream);
+
+/* ---- contraction ------------------------------------------------------------
+ * A contractor owns the work buffers for one lattice geometry; replaces the
+ * Contractor of include/hlbl/contract.hpp:42-94 (buffers :69-79). */
+int hlbl_contractor_create(hlbl_contractor** out, hlbl_geom global, hlbl_geom local,
+                           hlbl_coord proc_coords, hlbl_qed* qed);
+int hlbl_contractor_free(hlbl_contractor* c);
+
+/* Seam 1, Contractor::kernel_sum (src/contract.cpp:104-114): the connected kernel
+ * sum of both flavours of the pair x = gsx, y = gsx + yv, plain FP64. Blocks are
+ * borrowed device pointers. Synchronous on `stream`; out[2] is host memory. The sum
+ * over ranks is the identity on one process. */
+int hlbl_kernel_sum(hlbl_contractor* c, const double* d_fwd_src, const double* d_fwd_y,
+                    hlbl_coord gsx, hlbl_coord yv, double xunit, double out[2],
+                    hlbl_stream stream);
+
+/* Seam 2, the launchers of kernels/cuda_lattice.h:49-57 (cu_dzu_dzsu,
+ * cu_4pt_contraction), with the contractor standing in for QED_kernel_temps and
+ * the work space. Both ADD their result to the output buffers, which the caller
+ * zeroes, as in the reference. d_dzu is [6][12][12] complex, d_dzsu [4][12][12],
+ * d_g_dzu [6][4][12][12], d_g_dzsu [4][4][12][12] (contract.hpp:18-23). */
+int hlbl_dzu_dzsu(hlbl_contractor* c, double* d_dzu, double* d_dzsu, const double* fwd_src,
+                  const double* fwd_y, int iflavor, hlbl_coord proc_coords, hlbl_coord gsx,
+                  hlbl_geom global, hlbl_geom local, hlbl_stream stream);
+/* g = gamma_mu gamma_5 applied to dzu/dzsu (src/contract.cpp:58-77), on the device */
+int hlbl_insert_g5_gamma(const double* d_dzu, const double* d_dzsu, double* d_g_dzu,
+                         double* d_g_dzsu, hlbl_stream stream);
+int hlbl_4pt_contraction(hlbl_contractor* c, double* d_kernel_sum, const double* d_g_dzu,
+                         const double* d_g_dzsu, const double* fwd_src, const double* fwd_y,
+                         int iflavor, hlbl_coord proc_coords, hlbl_coord gsx, hlbl_pair xunit,
+                         hlbl_coord yv, hlbl_geom global, hlbl_geom local,
+                         hlbl_stream stream);
+
+/* Window entry: n pairs {x block, y block, gsx, yv} and both flavours per call.
+ * form 0 = plain FP64; 1 = FP32 products + FP64 Kahan sums + plain kernel;
+ * 2 = FP32 + surrogate kernel (plain FP64 near the origin and outside the trained
+ * domain). Pass 1 and pass 2 of all pairs each run as one persistent launch; the
+ * sequence is captured once as a hipGraph and replayed for later windows with the
+ * same n. out is host [n][2]. No counterpart in the reference (north-star
+ * window batching); it replaces the pair loop of src/driver.cpp:110-132. */
+typedef struct {
+  const double* d_x;
+  const double* d_y;
+  hlbl_coord gsx;
+  hlbl_coord yv;
+} hlbl_pair_desc;
+int hlbl_window_kernel_sum(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n_pairs,
+                           double xunit, int form, double* out, hlbl_stream stream);
+/* the surrogate's judge for the last form-2 window: max relative deviation of the
+ * surrogate kernel from the plain one over the sampled sites */
+int hlbl_window_judge(const hlbl_contractor* c, double* max_rel_err, int* n_samples);
+
+/* ---- sliding-hypercube schedule (src/schedule.cpp:34-214) -------------------- */
+hlbl_schedule* hlbl_schedule_build(hlbl_coord h, int stride, int n_steps, hlbl_coord origin);
+void hlbl_schedule_free(hlbl_schedule* s);
+int hlbl_schedule_n_windows(const hlbl_schedule* s);
+int hlbl_schedule_n_sources(const hlbl_schedule* s);
+/* source i in run order: its window, unwrapped position, partner count */
+int hlbl_schedule_source(const hlbl_schedule* s, int i, int* window, hlbl_coord* pos,
+                         int* n_partners);
+int hlbl_schedule_partner(const hlbl_schedule* s, int i, int j, hlbl_coord* pos);
+/* 0 if complete (check_complete, schedule.cpp:111-139), else -1 with the message */
+int hlbl_schedule_check_complete(const hlbl_schedule* s, int stride);
+/* simulate (schedule.cpp:168-214): mode 0 = serial, 1 = interleaved;
+ * stats = {n_propagators, n_solves, n_pairs, n_kernel_sums, peak_cache} */
+int hlbl_schedule_simulate(const hlbl_schedule* s, int mode, long long stats[5]);
+int hlbl_max_steps_without_wrap(hlbl_coord h, int stride, hlbl_coord lattice,
+                                hlbl_coord origin, int cap);
+
+/* ---- driver: the schedule loop on random fields (src/driver.cpp:23-208) ------
+ * Runs one chain: solve (device RandomProvider), cache (residency 0 = device,
+ * 1 = pinned host with two staging slots), contract per pair (form as above;
+ * batch 0 = one pair per call in solve order, 1 = window batches), free blocks
+ * after their last pair. rows gets n_pairs * 16 doubles:
+ * x[4], y[4], yv[4], window, chain(=0), ks0, ks1 (ints stored as doubles).
+ * stats = {pairs, positions_solved, cache_peak, seconds_contract, seconds_solve}. */
+int hlbl_run_chain(hlbl_contractor* c, const hlbl_schedule* s, uint64_t seed, double xunit,
+                   int residency, int form, int batch, double* rows, int max_rows,
+                   double stats[5]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HLBL_AMD_H */

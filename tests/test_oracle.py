@@ -1,0 +1,333 @@
+"""Pins of the oracle (the CPU restatement) against the reference's own test
+values, the reference's compiled sources (oracle/_ref), textbook identities and
+independent restatements. CPU only."""
+import ctypes
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import P, fill_block, oracle_pair
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_values.json")))
+
+
+def test_splitmix64_known_answer(oracle):
+    assert oracle.oracle_hash64(0) == int(GOLD["splitmix64"]["hash64_of_0"], 16)
+
+
+def test_sum_p1_golden(oracle):
+    """The reference's 2+2 golden sum of P1 needs no QED kernel: it pins the random
+    fields (hash, key, layout), the gamma basis and the conjugation convention."""
+    g = GOLD["sum_p1_2p2"]
+    blk = fill_block(oracle, g["lattice"], g["seed"])
+    oracle.oracle_sum_p1.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    s = oracle.oracle_sum_p1(P(blk), P(np.array(g["lattice"], np.int32)), g["flavour"])
+    assert abs(s - g["sum_p1"]) <= 1e-10 * abs(g["sum_p1"])
+
+
+def test_sum_p1_detects_a_wrong_seed(oracle):
+    g = GOLD["sum_p1_2p2"]
+    blk = fill_block(oracle, g["lattice"], g["seed"] + 1)
+    oracle.oracle_sum_p1.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    s = oracle.oracle_sum_p1(P(blk), P(np.array(g["lattice"], np.int32)), 0)
+    assert abs(s - g["sum_p1"]) > 1e-3
+
+
+def test_gamma_tables_match_reference(oracle, ref):
+    for g in (0, 1, 2, 3, 5):
+        p1, s1 = np.zeros(24, np.int32), np.zeros(24, np.int32)
+        p2, s2 = np.zeros(24, np.int32), np.zeros(24, np.int32)
+        oracle.oracle_gamma_tables(g, P(p1), P(s1))
+        ref.ref_gamma_row(g, P(p2), P(s2))
+        assert np.array_equal(p1, p2) and np.array_equal(s1, s2), g
+
+
+def gamma_matrices(oracle):
+    """12x12 complex matrices of gamma_0..3 and gamma_5 from the oracle's tables."""
+    mats = {}
+    for g in (0, 1, 2, 3, 5):
+        perm, sign = np.zeros(24, np.int32), np.zeros(24, np.int32)
+        oracle.oracle_gamma_tables(g, P(perm), P(sign))
+        M = np.zeros((12, 12), complex)
+        for comp in range(12):
+            e = np.zeros(24)
+            e[2 * comp] = 1.0
+            out = e[perm] * sign
+            M[:, comp] = out[0::2] + 1j * out[1::2]
+        mats[g] = M
+    return mats
+
+
+def test_gamma_clifford(oracle):
+    G = gamma_matrices(oracle)
+    I = np.eye(12)
+    for a in range(4):
+        for b in range(4):
+            assert np.allclose(G[a] @ G[b] + G[b] @ G[a], 2 * (a == b) * I)
+        assert np.allclose(G[5] @ G[a] + G[a] @ G[5], 0)
+    assert np.allclose(G[0] @ G[1] @ G[2] @ G[3], G[5])
+    assert np.allclose(G[5] @ G[5], I)
+
+
+def euclid_trace(idx):
+    """Tr of a product of Euclidean gammas by the delta expansion (4x4 spin trace)."""
+    if len(idx) == 0:
+        return 4.0
+    if len(idx) % 2:
+        return 0.0
+    a = idx[0]
+    s = 0.0
+    for j in range(1, len(idx)):
+        if idx[j] == a:
+            s += (-1) ** (j - 1) * euclid_trace(idx[1:j] + idx[j + 1:])
+    return s
+
+
+def test_qed_trace_tensors_textbook(oracle):
+    """G^A of the oracle against the delta expansion of gamma traces."""
+    comb = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    n = 4000
+    o, i, c = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n)
+    oracle.oracle_g_terms.restype = ctypes.c_int
+    cnt = oracle.oracle_g_terms(P(o), P(i), P(c), n)
+    got = np.zeros((384, 192))
+    got[o[:cnt], i[:cnt]] = c[:cnt]
+
+    def trP(de, rho, sig, rest):
+        # P = 2 g_de g_rho g_sig + 2 delta_{sig de} g_rho - 2 delta_{rho de} g_sig
+        t = 2 * euclid_trace([de, rho, sig] + rest)
+        if sig == de:
+            t += 2 * euclid_trace([rho] + rest)
+        if rho == de:
+            t -= 2 * euclid_trace([sig] + rest)
+        return t
+
+    want = np.zeros((384, 192))
+    for k, (rho, sig) in enumerate(comb):
+        for mu, nu, lam, al, be, de in itertools.product(range(4), repeat=6):
+            oo = ((k * 4 + mu) * 4 + nu) * 4 + lam
+            ii = (al * 4 + be) * 4 + de
+            want[oo, ii] = trP(de, rho, sig, [mu, al, nu, be, lam]) / 8
+            if be == lam:
+                want[oo, 64 + ii] = -trP(de, rho, sig, [mu, al, nu]) / 4
+            if al == mu:
+                want[oo, 128 + ii] = -trP(de, rho, sig, [nu, be, lam]) / 4
+    assert cnt == np.count_nonzero(want) == 2016
+    assert np.array_equal(got, want)
+
+
+def channels(oracle, u, v, w):
+    ch = np.zeros(32)
+    oracle.oracle_channels_exact(u, v, w, P(ch))
+    return ch
+
+
+def test_weight_channel_derivatives(oracle):
+    """The hyper-dual derivatives of the weight functions against central differences of
+    the values (l1, l2, l3 are channels; g0, g1, g2 enter through derivatives only)."""
+    u, v, w = 1.3, 0.7, 0.4
+    h = 1e-5
+    for base in (3, 7, 11):  # l1, l2, l3: (f, fu, fv, fw)
+        for d in range(3):
+            e = np.zeros(3)
+            e[d] = h
+            fp = channels(oracle, *(np.array([u, v, w]) + e))[base]
+            fm = channels(oracle, *(np.array([u, v, w]) - e))[base]
+            assert abs((fp - fm) / (2 * h) - channels(oracle, u, v, w)[base + 1 + d]) < 1e-8
+    # g1, g2: second derivatives from the first ones
+    names = {"uu": (0, 0), "uv": (0, 1), "uw": (0, 2), "vw": (1, 2), "ww": (2, 2)}
+    for base in (15, 23):
+        c0 = channels(oracle, u, v, w)
+        for k, (name, (a, b)) in enumerate(names.items()):
+            e = np.zeros(3)
+            e[b] = h
+            fp = channels(oracle, *(np.array([u, v, w]) + e))[base + a]
+            fm = channels(oracle, *(np.array([u, v, w]) - e))[base + a]
+            assert abs((fp - fm) / (2 * h) - c0[base + 3 + k]) < 1e-7, name
+
+
+def test_t_tensors_against_finite_differences(oracle):
+    """T^I = d^x_al (d^x_be + d^y_be) V_de, T^II = d^x_al (T_be_de + delta S/4),
+    T^III = (d^x_be + d^y_be)(T_al_de + delta S/4) from exact weight functions."""
+    rng = np.random.default_rng(3)
+    x, y = rng.uniform(-1, 1, 4), rng.uniform(-1, 1, 4)
+    u, v, w = x @ x, y @ y, x @ y
+    ch = channels(oracle, u, v, w)
+    T = np.zeros(192)
+    oracle.oracle_t_tensors(P(x), P(y), P(ch), P(T))
+    T = T.reshape(3, 4, 4, 4)
+
+    def svt(xx, yy):
+        S = np.zeros(1)
+        V = np.zeros(4)
+        Tt = np.zeros(16)
+        oracle.oracle_svt(P(np.ascontiguousarray(xx)), P(np.ascontiguousarray(yy)), P(S), P(V), P(Tt))
+        return S[0], V, Tt.reshape(4, 4)
+
+    h = 1e-4
+
+    def d_x(f, xx, yy, al):
+        e = np.zeros(4)
+        e[al] = h
+        return (f(xx + e, yy) - f(xx - e, yy)) / (2 * h)
+
+    def d_sum(f, xx, yy, be):
+        e = np.zeros(4)
+        e[be] = h
+        return (f(xx + e, yy + e) - f(xx - e, yy - e)) / (2 * h)
+
+    for al, be in itertools.product(range(4), repeat=2):
+        t1 = d_x(lambda a, b: d_sum(lambda c, d: svt(c, d)[1], a, b, be), x, y, al)
+        t2 = d_x(lambda a, b: svt(a, b)[2][be] + (np.eye(4)[be] * svt(a, b)[0] / 4), x, y, al)
+        t3 = d_sum(lambda a, b: svt(a, b)[2][al] + (np.eye(4)[al] * svt(a, b)[0] / 4), x, y, be)
+        assert np.allclose(T[0, al, be], t1, atol=1e-6), ("I", al, be)
+        assert np.allclose(T[1, al, be], t2, atol=1e-7), ("II", al, be)
+        assert np.allclose(T[2, al, be], t3, atol=1e-7), ("III", al, be)
+
+
+def qed(oracle, x, y):
+    x = np.ascontiguousarray(np.atleast_2d(x), float)
+    y = np.ascontiguousarray(np.atleast_2d(y), float)
+    k = np.zeros((len(x), 384))
+    oracle.oracle_qed_kernel(P(x), P(y), len(x), P(k))
+    return k
+
+
+def test_qed_kernel_vanishes_at_origin(oracle):
+    rng = np.random.default_rng(0)
+    for _ in range(5):
+        v = rng.uniform(-2, 2, 4)
+        assert np.all(qed(oracle, np.zeros(4), v) == 0.0)
+        assert np.all(qed(oracle, v, np.zeros(4)) == 0.0)
+        assert np.abs(qed(oracle, v, rng.uniform(-2, 2, 4))).max() > 1e-3
+
+
+def test_qed_kernel_cutoff(oracle):
+    R = 12.0 * 105.6583745 / 197.3269804
+    v = np.array([0.3, -0.2, 0.1, 0.5])
+    far = np.array([R * 1.01, 0, 0, 0])
+    assert np.all(qed(oracle, far, v) == 0.0) and np.all(qed(oracle, v, far) == 0.0)
+    near = np.array([R * 0.99, 0, 0, 0])
+    assert np.abs(qed(oracle, near, v)).max() > 0.0
+
+
+def test_table_nodes_exact(oracle):
+    """At grid nodes the interpolated channels are the exact ones."""
+    R = 12.0 * 105.6583745 / 197.3269804
+    for ia, ic, ib in [(0, 12, 0), (3, 5, 7), (48, 24, 48), (20, 0, 31)]:
+        a, c, b = R * ia / 48, -1 + 2 * ic / 24, R * ib / 48
+        got = np.zeros(32)
+        oracle.oracle_channels_interp(a, c, b, P(got))
+        want = channels(oracle, a * a, b * b, a * b * c)
+        assert np.allclose(got[:31], want[:31], rtol=1e-13, atol=1e-15)
+
+
+# ---- the contraction, restated independently with numpy on a tiny lattice --
+
+def np_kernel_sum(oracle, S, Y, dims, gsx, yv, xunit, probe=True):
+    """Appendix-A arithmetic (kernels/cuda_lattice.cu:160-484) in numpy, with the
+    oracle's probe kernel restated."""
+    G = gamma_matrices(oracle)
+    V = int(np.prod(dims))
+    Sc = (S[0::2] + 1j * S[1::2]).reshape(2, 12, V, 12)
+    Yc = (Y[0::2] + 1j * Y[1::2]).reshape(2, 12, V, 12)
+    coords = np.array(list(itertools.product(*[range(d) for d in dims])))
+    L = np.array(dims)
+    rel = (coords - np.array(gsx)) % L
+    xv = np.where(rel >= L // 2, rel - L, rel)
+    xz = np.where(rel > L // 2, rel - L, np.where(rel < L // 2, rel, 0))
+    comb = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    ym = np.array([((v % l) - l if (v % l) >= l // 2 else (v % l)) for v, l in zip(yv, dims)]) * xunit
+    dd = (rel - np.array(yv)) % L
+    dm = np.where(dd >= L // 2, dd - L, dd) * xunit
+    xm = xv * xunit
+
+    def probe_k(x, y):
+        k = np.zeros((6, 4, 4, 4))
+        for kk, mu, nu, lam in itertools.product(range(6), range(4), range(4), range(4)):
+            k[kk, mu, nu, lam] = np.sin(0.3 * kk + 0.7 * mu + 1.1 * nu + 1.3 * lam + x[mu] - 0.5 * y[nu] +
+                                        0.2 * x[lam] * y[lam])
+        return k
+
+    out = []
+    for f in range(2):
+        t = np.einsum("scd,azd->sazc", np.array([G[5] @ G[s] for s in range(4)]), Sc[f])
+        yb = np.conj(Yc[1 - f])  # [b][z][c]
+        dzsu = np.einsum("sazc,bzc->sab", t, yb)
+        dzu = np.zeros((6, 12, 12), complex)
+        for k, (rho, sig) in enumerate(comb):
+            dzu[k] = np.einsum("azc,bzc,z->ab", t[sig], yb, xz[:, rho]) - np.einsum("azc,bzc,z->ab", t[rho], yb, xz[:, sig])
+        Gn = np.array([G[nu] @ G[5] for nu in range(4)])
+        gdzu = np.einsum("nrb,kib->knir", Gn, dzu)
+        gdzsu = np.einsum("nrb,sib->snir", Gn, dzsu)
+        tm = np.einsum("mcd,azd->mazc", np.array([G[5] @ G[m] for m in range(4)]), Sc[1 - f])
+        dxu = -np.einsum("mazc,rzc->mzra", np.conj(tm), Yc[f])
+        q = np.einsum("lia,mzra->mlzri", Gn, dxu)
+        total = 0.0
+        for z in range(V):
+            cI = np.zeros((6, 4, 4, 4))
+            cII = np.zeros((6, 4, 4, 4))
+            for k, (rho, sig) in enumerate(comb):
+                cI[k] = -np.real(np.einsum("mlri,nir->mnl", q[:, :, z], gdzu[k]))
+                H = xz[z, rho] * gdzsu[sig] - xz[z, sig] * gdzsu[rho]
+                cII[k] = -np.real(np.einsum("mlri,nir->mnl", q[:, :, z], H))
+            K1 = probe_k(xm[z], ym)
+            K2 = probe_k(ym, xm[z])
+            K3 = probe_k(xm[z], dm[z])
+            total += np.sum(K1 * cI) + np.sum(np.transpose(K2, (0, 2, 1, 3)) * cI)
+            K3p = np.transpose(K3, (0, 3, 2, 1))
+            total += np.sum(K3p * (cII - cI))
+        out.append(total)
+    return np.array(out)
+
+
+def test_contraction_numpy_restatement(oracle):
+    dims = (4, 2, 2, 2)
+    S = fill_block(oracle, dims, 11)
+    Y = fill_block(oracle, dims, 12)
+    for gsx, yv, xu in [((0, 0, 0, 0), (1, 1, 1, 1), 0.5), ((1, 1, 0, 1), (-1, 1, -1, 1), 0.3)]:
+        want = np_kernel_sum(oracle, S, Y, dims, gsx, yv, xu)
+        got = oracle_pair(oracle, S, Y, dims, gsx, yv, xu, mode=2)
+        assert np.allclose(got, want, rtol=1e-11, atol=0), (got, want)
+
+
+def test_contraction_zero_kernel(oracle):
+    dims = (4, 2, 2, 2)
+    S, Y = fill_block(oracle, dims, 1), fill_block(oracle, dims, 2)
+    assert np.all(oracle_pair(oracle, S, Y, dims, (0, 0, 0, 0), (1, 1, 1, 1), 1.0, mode=1) == 0.0)
+
+
+def test_contraction_quadratic_in_each_block(oracle):
+    dims = (4, 2, 2, 2)
+    S, Y = fill_block(oracle, dims, 1), fill_block(oracle, dims, 2)
+    a = oracle_pair(oracle, S, Y, dims, (0, 0, 0, 0), (1, 1, 1, 1), 0.7)
+    b = oracle_pair(oracle, 2 * S, Y, dims, (0, 0, 0, 0), (1, 1, 1, 1), 0.7)
+    c = oracle_pair(oracle, S, 3 * Y, dims, (0, 0, 0, 0), (1, 1, 1, 1), 0.7)
+    assert np.allclose(b, 4 * a, rtol=1e-12) and np.allclose(c, 9 * a, rtol=1e-12)
+
+
+def test_dzsu_of_point_fields(oracle):
+    """Fields that are 1 in one spin-colour at one site: dzsu[s][a0][b0] is the
+    (c1, c0) entry of gamma_5 gamma_sigma (pass 1, cuda_lattice.cu:226-258)."""
+    dims = (4, 2, 2, 2)
+    V = int(np.prod(dims))
+    G = gamma_matrices(oracle)
+    for (a0, c0, b0, c1, z0) in [(0, 0, 0, 6, 0), (3, 4, 7, 10, 5), (11, 9, 2, 3, 31)]:
+        S = np.zeros(2 * 12 * 24 * V)
+        Y = np.zeros(2 * 12 * 24 * V)
+        f = 0
+        S[((f * 12 + a0) * V + z0) * 24 + 2 * c0] = 1.0
+        Y[(((1 - f) * 12 + b0) * V + z0) * 24 + 2 * c1] = 1.0
+        dzu = np.zeros(6 * 144 * 2)
+        dzsu = np.zeros(4 * 144 * 2)
+        g = np.array(dims, np.int32)
+        oracle.oracle_dzu_dzsu(P(S), P(Y), f, P(g), P(g), P(np.zeros(4, np.int32)), P(np.zeros(4, np.int32)),
+                               P(dzu), P(dzsu))
+        d = (dzsu[0::2] + 1j * dzsu[1::2]).reshape(4, 12, 12)
+        for s in range(4):
+            want = (G[5] @ G[s])[c1, c0]
+            assert np.isclose(d[s, a0, b0], want), (s, d[s, a0, b0], want)
+            assert np.count_nonzero(np.abs(d[s]) > 0) <= 1
