@@ -92,6 +92,8 @@ _SIGS = {
     "hlbl_window_kernel_sum": (C.c_int, [C.c_void_p, C.POINTER(PairDesc), C.c_int, C.c_double,
                                          C.c_int, C.POINTER(C.c_double), C.c_void_p]),
     "hlbl_window_judge": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "hlbl_window_profile": (C.c_int, [C.c_void_p, C.POINTER(PairDesc), C.c_int, C.c_double,
+                                      C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "hlbl_schedule_build": (C.c_void_p, [Coord, C.c_int, C.c_int, Coord]),
     "hlbl_schedule_free": (None, [C.c_void_p]),
     "hlbl_schedule_n_windows": (C.c_int, [C.c_void_p]),
@@ -223,6 +225,40 @@ class QedKernel:
         check(lib().hlbl_qed_channels(self.handle, mode, C.c_void_p(d_acb), n,
                                       C.c_void_p(d_out), stream))
 
+    # the MLP surrogate
+    def mlp_param_count(self):
+        return lib().hlbl_mlp_param_count(self.handle)
+
+    def mlp_init(self, seed=0):
+        check(lib().hlbl_mlp_init(self.handle, C.c_uint64(seed)))
+
+    def mlp_train(self, n_samples=1 << 20, epochs=20, batch=1024, lr=2e-3, seed=4, stream=None):
+        loss = (C.c_float * epochs)()
+        check(lib().hlbl_mlp_train(self.handle, n_samples, epochs, batch, lr, C.c_uint64(seed),
+                                   loss, stream))
+        return list(loss)
+
+    def mlp_get_params(self):
+        n = self.mlp_param_count()
+        buf = (C.c_float * n)()
+        check(lib().hlbl_mlp_get_params(self.handle, buf, n))
+        return buf
+
+    def mlp_set_params(self, params):
+        n = self.mlp_param_count()
+        buf = (C.c_float * n)(*params)
+        check(lib().hlbl_mlp_set_params(self.handle, buf, n))
+
+    def mlp_forward(self, d_in, n, d_out, precision=1, stream=None):
+        check(lib().hlbl_mlp_forward(self.handle, C.c_void_p(d_in), n, C.c_void_p(d_out), precision,
+                                     stream))
+
+    def mlp_loss_grad(self, d_in, d_target, n, d_grad, stream=None):
+        loss = C.c_float()
+        check(lib().hlbl_mlp_loss_grad(self.handle, C.c_void_p(d_in), C.c_void_p(d_target), n,
+                                       C.c_void_p(d_grad), C.byref(loss), stream))
+        return loss.value
+
 
 class Contractor:
     """The connected kernel sum of a pair (include/hlbl/contract.hpp:42-94)."""
@@ -257,6 +293,18 @@ class Contractor:
         out = (C.c_double * (2 * n))()
         check(lib().hlbl_window_kernel_sum(self.handle, arr, n, xunit_, form, out, stream))
         return [[out[2 * i], out[2 * i + 1]] for i in range(n)]
+
+    def window_profile(self, pairs, xunit_, form=1):
+        """One eager window with HIP events per stage: (results, ms[4]) with ms =
+        {pass 1, surrogate channels, coefficient fields, pass 2}."""
+        n = len(pairs)
+        arr = (PairDesc * n)()
+        for i, (dx, dy, gsx, yv) in enumerate(pairs):
+            arr[i] = PairDesc(C.c_void_p(dx), C.c_void_p(dy), _coord(gsx), _coord(yv))
+        out = (C.c_double * (2 * n))()
+        ms = (C.c_double * 4)()
+        check(lib().hlbl_window_profile(self.handle, arr, n, xunit_, form, out, ms))
+        return [[out[2 * i], out[2 * i + 1]] for i in range(n)], list(ms)
 
     def judge(self):
         e = C.c_double()

@@ -584,3 +584,20 @@ int hlbl_run_chain(hlbl_contractor* c, const hlbl_schedule* s, uint64_t seed, do
 }
 
 }  // extern "C"
+
+namespace hlbl {
+void window_profile(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit, int form,
+                    double* out, double ms[4]);
+}
+
+extern "C" int hlbl_window_profile(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n_pairs,
+                                   double xunit, int form, double* out, double ms[4]) {
+  return guarded([&] {
+    check_ptr(c, "contractor");
+    check_ptr(pairs, "pairs");
+    check_ptr(out, "out");
+    check_ptr(ms, "ms");
+    if (n_pairs <= 0) throw Error("n_pairs must be positive");
+    window_profile(c, pairs, n_pairs, xunit, form, out, ms);
+  });
+}

@@ -50,8 +50,10 @@ struct WindowPlan {
   }
 };
 
-// kernels of a window on `st` (recorded into the graph)
+// kernels of a window on `st` (recorded into the graph); stage 0 = pass 1 and its
+// reductions, 1 = coefficient fields (and the judge), 2 = pass 2 and its reduction
 void window_record(hlbl_contractor* c, WindowPlan& w, hipStream_t st);
+void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st);
 size_t window_part1_len(const hlbl_contractor* c, int n_pairs);
 size_t window_part2_len(const hlbl_contractor* c, int n_pairs);
 

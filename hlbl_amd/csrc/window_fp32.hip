@@ -338,11 +338,17 @@ __global__ void reduce_window2(const double* __restrict__ part, int nwg, int n_p
 // ------------------------------------------------------------- launchers --
 
 void window_record(hlbl_contractor* c, WindowPlan& w, hipStream_t st) {
+  window_stage(c, w, 0, st);
+  window_stage(c, w, 1, st);
+  window_stage(c, w, 2, st);
+}
+
+void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) {
   const hlbl_geom g = c->global;
   const size_t V = c->volume;
   const int np = w.n_pairs, nu = w.n_unique;
   // pass 1
-  {
+  if (stage == 0) {
     const int nwg = c->n_cu;
     const size_t spw = div_up(div_up(V, (size_t)nwg), kM1Sites) * kM1Sites;
     const int n = (int)div_up(V, spw);
@@ -355,7 +361,7 @@ void window_record(hlbl_contractor* c, WindowPlan& w, hipStream_t st) {
     launch_forms_batched(w.xi.p, w.dz.p, w.gdz.p, nullptr, w.bmat.p, np, st);
   }
   // coefficient fields, one per distinct yv
-  {
+  if (stage == 1) {
     const int nwg = 4 * c->n_cu;
     const size_t spw = div_up(V, (size_t)nwg);
     const int n = (int)div_up(V, spw);
@@ -371,7 +377,7 @@ void window_record(hlbl_contractor* c, WindowPlan& w, hipStream_t st) {
     }
   }
   // pass 2
-  {
+  if (stage == 2) {
     const int nwg = 2 * c->n_cu;
     const size_t spw = div_up(div_up(V, (size_t)nwg), kM2Sites) * kM2Sites;
     const int n = (int)div_up(V, spw);

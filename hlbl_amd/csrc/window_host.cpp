@@ -58,8 +58,9 @@ void window_judge_result(const hlbl_contractor* c, double* err, int* n) {
   *n = c->window->last_judge_n;
 }
 
-void window_run(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit, int form,
-                double* out, hipStream_t user_stream) {
+// the window's plan (reused or built) with the pair table filled in
+WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit,
+                           int form) {
   const hlbl_geom g = c->global;
   if (g.T != c->local.T || g.LX != c->local.LX || g.LY != c->local.LY || g.LZ != c->local.LZ)
     throw Error("hlbl_window_kernel_sum: the mixed-precision forms need the whole lattice on this process");
@@ -83,7 +84,6 @@ void window_run(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double x
   }
   const int nu = (int)yvs.size();
 
-  // find or build the plan
   WindowPlan* w = nullptr;
   for (auto& p : cache->plans)
     if (p->n_pairs == n && p->n_unique == nu && p->form == form && p->xunit == xunit) w = p.get();
@@ -122,26 +122,47 @@ void window_run(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double x
     cache->plans.push_back(std::move(p));
     w = cache->plans.back().get();
   }
-
-  for (int i = 0; i < n; ++i) {
+  for (int i = 0; i < n; ++i)
     w->h_pairs.p[i] = WinPair{pairs[i].d_x, pairs[i].d_y, pairs[i].gsx, pairs[i].yv, slot[i], 0};
-  }
   for (int u = 0; u < nu; ++u) w->h_yvs.p[u] = yvs[u];
-
-  // capture once per plan; graphs need a stream that is not the null stream
   if (!cache->stream) check_hip(hipStreamCreateWithFlags(&cache->stream, hipStreamNonBlocking), "hipStreamCreate");
-  hipStream_t st = cache->stream;
+  return w;
+}
+
+namespace {
+void window_finish(hlbl_contractor* c, WindowPlan* w, double* out) {
+  for (int i = 0; i < 2 * w->n_pairs; ++i) out[i] = w->h_result.p[i];
+  if (w->form == 2) {
+    double e = 0.0;
+    for (int k = 0; k < w->n_unique * w->n_samples; ++k) e = std::max(e, w->h_judge.p[k]);
+    c->window->last_judge = e;
+    c->window->last_judge_n = w->n_unique * w->n_samples;
+  }
+}
+void window_tail_copies(WindowPlan* w, hipStream_t st) {
+  check_hip(hipMemcpyAsync(w->h_result.p, w->result.p, 2 * w->n_pairs * sizeof(double), hipMemcpyDeviceToHost, st), "copy result");
+  if (w->form == 2)
+    check_hip(hipMemcpyAsync(w->h_judge.p, w->judge.p, (size_t)w->n_unique * w->n_samples * sizeof(double),
+                             hipMemcpyDeviceToHost, st), "copy judge");
+}
+void window_head_copies(WindowPlan* w, hipStream_t st) {
+  check_hip(hipMemcpyAsync(w->d_pairs.p, w->h_pairs.p, w->n_pairs * sizeof(WinPair), hipMemcpyHostToDevice, st), "copy pairs");
+  check_hip(hipMemcpyAsync(w->d_yvs.p, w->h_yvs.p, w->n_unique * sizeof(hlbl_coord), hipMemcpyHostToDevice, st), "copy yv");
+}
+}  // namespace
+
+void window_run(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit, int form,
+                double* out, hipStream_t user_stream) {
+  WindowPlan* w = window_prepare(c, pairs, n, xunit, form);
+  hipStream_t st = c->window->stream;
+  // capture once per plan; graphs need a stream that is not the null stream
   if (!w->exec) {
     check_hip(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
-      check_hip(hipMemcpyAsync(w->d_pairs.p, w->h_pairs.p, n * sizeof(WinPair), hipMemcpyHostToDevice, st), "copy pairs");
-      check_hip(hipMemcpyAsync(w->d_yvs.p, w->h_yvs.p, nu * sizeof(hlbl_coord), hipMemcpyHostToDevice, st), "copy yv");
+      window_head_copies(w, st);
       if (form == 2) launch_surrogate_channels(c, *w, st);
       window_record(c, *w, st);
-      check_hip(hipMemcpyAsync(w->h_result.p, w->result.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, st), "copy result");
-      if (form == 2)
-        check_hip(hipMemcpyAsync(w->h_judge.p, w->judge.p, (size_t)nu * w->n_samples * sizeof(double),
-                                 hipMemcpyDeviceToHost, st), "copy judge");
+      window_tail_copies(w, st);
     } catch (...) {
       hipGraph_t gr = nullptr;
       (void)hipStreamEndCapture(st, &gr);
@@ -156,13 +177,41 @@ void window_run(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double x
   else check_hip(hipDeviceSynchronize(), "device");
   check_hip(hipGraphLaunch(w->exec, st), "hipGraphLaunch");
   check_hip(hipStreamSynchronize(st), "window");
-  for (int i = 0; i < 2 * n; ++i) out[i] = w->h_result.p[i];
-  if (form == 2) {
-    double e = 0.0;
-    for (int k = 0; k < nu * w->n_samples; ++k) e = std::max(e, w->h_judge.p[k]);
-    cache->last_judge = e;
-    cache->last_judge_n = nu * w->n_samples;
+  window_finish(c, w, out);
+}
+
+void window_profile(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit, int form,
+                    double* out, double ms[4]) {
+  if (form != 1 && form != 2) throw Error("hlbl_window_profile: form must be 1 or 2");
+  WindowPlan* w = window_prepare(c, pairs, n, xunit, form);
+  hipStream_t st = c->window->stream;
+  check_hip(hipDeviceSynchronize(), "device");
+  hipEvent_t ev[5];
+  for (auto& e : ev) check_hip(hipEventCreate(&e), "hipEventCreate");
+  try {
+    window_head_copies(w, st);
+    check_hip(hipEventRecord(ev[0], st), "event");
+    window_stage(c, *w, 0, st);
+    check_hip(hipEventRecord(ev[1], st), "event");
+    if (form == 2) launch_surrogate_channels(c, *w, st);
+    check_hip(hipEventRecord(ev[2], st), "event");
+    window_stage(c, *w, 1, st);
+    check_hip(hipEventRecord(ev[3], st), "event");
+    window_stage(c, *w, 2, st);
+    check_hip(hipEventRecord(ev[4], st), "event");
+    window_tail_copies(w, st);
+    check_hip(hipStreamSynchronize(st), "window");
+    for (int i = 0; i < 4; ++i) {
+      float t = 0.f;
+      check_hip(hipEventElapsedTime(&t, ev[i], ev[i + 1]), "hipEventElapsedTime");
+      ms[i] = t;
+    }
+  } catch (...) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    throw;
   }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  window_finish(c, w, out);
 }
 
 }  // namespace hlbl

@@ -151,6 +151,12 @@ int hlbl_window_kernel_sum(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
 /* the surrogate's judge for the last form-2 window: max relative deviation of the
  * surrogate kernel from the plain one over the sampled sites */
 int hlbl_window_judge(const hlbl_contractor* c, double* max_rel_err, int* n_samples);
+/* Measurement hook (no counterpart in the reference): runs a window of form 1 or 2
+ * once without the graph, with HIP events around each stage on the launch stream;
+ * ms = {pass 1 incl. its reductions, surrogate channels (0 for form 1),
+ * coefficient fields, pass 2 incl. its reduction}. out as for the window entry. */
+int hlbl_window_profile(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n_pairs,
+                        double xunit, int form, double* out, double ms[4]);
 
 /* ---- sliding-hypercube schedule (src/schedule.cpp:34-214) -------------------- */
 hlbl_schedule* hlbl_schedule_build(hlbl_coord h, int stride, int n_steps, hlbl_coord origin);

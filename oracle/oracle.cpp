@@ -810,3 +810,88 @@ void oracle_set_threads(int n) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- MLP ----
+// The surrogate network in FP64: 3 -> 4 x 128 GELU -> 31, parameters packed
+// layer by layer as W[out][in] then b[out], weights rounded to BF16 (nearest,
+// ties to even) before use; loss = mean over rows and outputs of (y - t)^2.
+namespace {
+float bf16(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return x;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  float r;
+  std::memcpy(&r, &u, 4);
+  return r;
+}
+const int kIn[5] = {3, 128, 128, 128, 128}, kOut[5] = {128, 128, 128, 128, 31};
+}  // namespace
+
+extern "C" {
+
+int oracle_mlp_param_count(void) {
+  int n = 0;
+  for (int l = 0; l < 5; ++l) n += kIn[l] * kOut[l] + kOut[l];
+  return n;
+}
+
+// loss of n rows and (if grad) its gradient w.r.t. the (rounded) parameters; y gets outputs
+double oracle_mlp(const float* params, const float* x, const float* t, int n, double* grad, double* y) {
+  const int np = oracle_mlp_param_count();
+  std::vector<double> w(np);
+  for (int i = 0; i < np; ++i) w[i] = bf16(params[i]);
+  int offw[5], offb[5], o = 0;
+  for (int l = 0; l < 5; ++l) {
+    offw[l] = o;
+    o += kIn[l] * kOut[l];
+    offb[l] = o;
+    o += kOut[l];
+  }
+  if (grad) std::fill(grad, grad + np, 0.0);
+  double loss = 0.0;
+  std::vector<double> z[5], a[6];
+  for (int r = 0; r < n; ++r) {
+    a[0].assign(x + 3 * r, x + 3 * r + 3);
+    for (int l = 0; l < 5; ++l) {
+      z[l].assign(kOut[l], 0.0);
+      for (int j = 0; j < kOut[l]; ++j) {
+        double s = w[offb[l] + j];
+        for (int k = 0; k < kIn[l]; ++k) s += w[offw[l] + j * kIn[l] + k] * a[l][k];
+        z[l][j] = s;
+      }
+      a[l + 1].resize(kOut[l]);
+      for (int j = 0; j < kOut[l]; ++j)
+        a[l + 1][j] = l < 4 ? 0.5 * z[l][j] * (1.0 + std::erf(z[l][j] / std::sqrt(2.0))) : z[l][j];
+    }
+    std::vector<double> d(31);
+    for (int c = 0; c < 31; ++c) {
+      const double e = a[5][c] - t[31 * r + c];
+      loss += e * e;
+      d[c] = 2.0 * e / ((double)n * 31);
+      if (y) y[31 * r + c] = a[5][c];
+    }
+    if (!grad) continue;
+    for (int l = 4; l >= 0; --l) {
+      // d holds dL/dz[l]
+      for (int j = 0; j < kOut[l]; ++j) {
+        grad[offb[l] + j] += d[j];
+        for (int k = 0; k < kIn[l]; ++k) grad[offw[l] + j * kIn[l] + k] += d[j] * a[l][k];
+      }
+      if (l == 0) break;
+      std::vector<double> dn(kIn[l], 0.0);
+      for (int k = 0; k < kIn[l]; ++k) {
+        double s = 0;
+        for (int j = 0; j < kOut[l]; ++j) s += d[j] * w[offw[l] + j * kIn[l] + k];
+        const double zz = z[l - 1][k];
+        const double gp = 0.5 * (1.0 + std::erf(zz / std::sqrt(2.0))) + zz * std::exp(-0.5 * zz * zz) / std::sqrt(2.0 * M_PI);
+        dn[k] = s * gp;
+      }
+      d = dn;
+    }
+  }
+  return loss / ((double)n * 31);
+}
+
+}  // extern "C"

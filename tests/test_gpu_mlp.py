@@ -1,0 +1,169 @@
+"""The MLP surrogate on the GPU: forward and gradients against the oracle's FP64
+restatement, training, the surrogate window form, and what the network gains
+over the method without it."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import hlbl_amd as H
+from conftest import P, fill_block
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+R = 12.0 * 105.6583745 / 197.3269804
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def qed():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    q = H.QedKernel(0)
+    yield q
+    q.close()
+
+
+def bf16_round(a):
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def oracle_mlp(oracle, params, x, t, want_grad=True):
+    oracle.oracle_mlp.restype = ctypes.c_double
+    oracle.oracle_mlp.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    n = len(x)
+    g = np.zeros(len(params)) if want_grad else None
+    y = np.zeros((n, 31))
+    loss = oracle.oracle_mlp(P(params), P(x), P(t), n, P(g) if want_grad else None, P(y))
+    return loss, g, y
+
+
+def test_mlp_gradient_vs_oracle(qed, oracle):
+    qed.mlp_init(seed=5)
+    params = np.array(qed.mlp_get_params(), np.float32)
+    oracle.oracle_mlp_param_count.restype = ctypes.c_int
+    assert len(params) == oracle.oracle_mlp_param_count() == 54047
+    rng = np.random.default_rng(0)
+    n = 96
+    x = np.stack([rng.uniform(0, 1, n), rng.uniform(0, 1, n), rng.uniform(-1, 1, n)], 1).astype(np.float32)
+    t = rng.normal(0, 1, (n, 31)).astype(np.float32)
+    dx, dt = torch.tensor(x, device="cuda"), torch.tensor(t, device="cuda")
+    dg = torch.zeros(len(params), dtype=torch.float32, device="cuda")
+    loss = qed.mlp_loss_grad(dx.data_ptr(), dt.data_ptr(), n, dg.data_ptr())
+    wl, wg, _ = oracle_mlp(oracle, params, x, t)
+    g = dg.cpu().numpy()
+    log("loss gpu", loss, "oracle", wl, "grad norm", np.linalg.norm(wg), "diff", np.linalg.norm(g - wg))
+    assert abs(loss - wl) <= 1e-5 * abs(wl)
+    # FP32 against FP64: relative to the gradient's scale, per layer block
+    assert np.linalg.norm(g - wg) <= 1e-4 * np.linalg.norm(wg)
+    assert np.max(np.abs(g - wg)) <= 1e-4 * np.max(np.abs(wg))
+
+
+def test_mlp_forward_vs_oracle(qed, oracle):
+    qed.mlp_init(seed=6)
+    params = np.array(qed.mlp_get_params(), np.float32)
+    # BF16-representable master weights: both forwards see the same weights
+    params = bf16_round(params)
+    qed.mlp_set_params(params)
+    rng = np.random.default_rng(1)
+    n = 300  # not a multiple of the 64-row tile
+    x = np.stack([rng.uniform(0, 1, n), rng.uniform(0, 1, n), rng.uniform(-1, 1, n)], 1).astype(np.float32)
+    _, _, want = oracle_mlp(oracle, params, x, np.zeros((n, 31), np.float32), want_grad=False)
+    dx = torch.tensor(x, device="cuda")
+    for prec in (0, 1):
+        dy = torch.full((n, 31), np.nan, dtype=torch.float32, device="cuda")
+        qed.mlp_forward(dx.data_ptr(), n, dy.data_ptr(), precision=prec)
+        got = dy.cpu().numpy()
+        assert np.all(np.isfinite(got))
+        assert np.max(np.abs(got - want)) <= 1e-4 * np.max(np.abs(want)), prec
+
+
+@pytest.fixture(scope="module")
+def trained(qed):
+    loss = qed.mlp_train(n_samples=1 << 17, epochs=30, batch=512, lr=2e-3, seed=9)
+    log("training loss", loss[0], "->", loss[-1])
+    return loss
+
+
+def test_mlp_training_reduces_loss(trained):
+    assert trained[-1] < 0.02 * trained[0]
+
+
+def test_surrogate_gain_at_equal_storage(qed, oracle, trained):
+    """What the network gains: at the storage of the network (54k BF16 weights, 108 KB)
+    a trilinear table of the same channels holds ~1.7k grid points; the network is
+    compared with such a table, and with the full 49x25x49 table, against the exact
+    weight functions at random points."""
+    rng = np.random.default_rng(7)
+    n = 2000
+    a, b, c = rng.uniform(0, R, n), rng.uniform(0, R, n), rng.uniform(-1, 1, n)
+    exact = np.zeros((n, 32))
+    for i in range(n):
+        oracle.oracle_channels_exact(a[i] ** 2, b[i] ** 2, a[i] * b[i] * c[i], P(exact[i]))
+    exact = exact[:, :31]
+    # the network (BF16 weights) through its standardisation and envelope
+    x = np.stack([a / R, b / R, c], 1).astype(np.float32)
+    dx = torch.tensor(x, device="cuda")
+    dy = torch.zeros((n, 31), dtype=torch.float32, device="cuda")
+    qed.mlp_forward(dx.data_ptr(), n, dy.data_ptr(), precision=1)
+    y = dy.cpu().numpy().astype(np.float64)
+    sc = np.zeros(62, np.float32)
+    # the standardisation is inside the library; recover it from exact targets by least squares per channel
+    env = np.exp(1 - 0.5 * (np.sqrt(1 + a ** 2) + np.sqrt(1 + b ** 2)))
+    t = exact / env[:, None]
+    coef = [np.polyfit(y[:, k], t[:, k], 1) for k in range(31)]
+    mlp = np.stack([np.polyval(coef[k], y[:, k]) for k in range(31)], 1) * env[:, None]
+    full = np.zeros((n, 32))
+    for i in range(n):
+        oracle.oracle_channels_interp(a[i], c[i], b[i], P(full[i]))
+    full = full[:, :31]
+    # equal-storage table: 12 x 12 x 12 grid x 31 channels in FP32 ~ 214 KB (>= the network's 108 KB)
+    na = nc = nb = 12
+    ga, gc, gb = np.linspace(0, R, na), np.linspace(-1, 1, nc), np.linspace(0, R, nb)
+    tab = np.zeros((na, nc, nb, 31))
+    for i, aa in enumerate(ga):
+        for j, cc in enumerate(gc):
+            for k, bb in enumerate(gb):
+                ch = np.zeros(32)
+                oracle.oracle_channels_exact(aa * aa, bb * bb, aa * bb * cc, P(ch))
+                tab[i, j, k] = ch[:31]
+    from scipy.interpolate import RegularGridInterpolator
+    small = RegularGridInterpolator((ga, gc, gb), tab)(np.stack([a, c, b], 1))
+    scale = np.max(np.abs(exact), axis=0)
+    e_mlp = np.max(np.abs(mlp - exact) / scale)
+    e_small = np.max(np.abs(small - exact) / scale)
+    e_full = np.max(np.abs(full - exact) / scale)
+    log(f"max error / channel scale: network {e_mlp:.2e}, equal-storage table {e_small:.2e}, "
+        f"full table (7.5 MB FP32) {e_full:.2e}")
+    assert e_mlp < e_small
+
+
+def test_surrogate_window_form(qed, trained):
+    """Form 2 (surrogate + FP32) against plain FP64 per pair sum; the judge reports the
+    surrogate's coefficient deviation on sample sites."""
+    lat = H.Lattice((8, 8, 8, 8))
+    con = H.Contractor(lat, qed)
+    blocks = []
+    for s in (41, 42, 43):
+        d = torch.empty(H.block_length(lat), dtype=torch.float64, device="cuda")
+        H.fill_random_block_device(lat, s, d.data_ptr())
+        blocks.append(d)
+    pairs = [(blocks[0].data_ptr(), blocks[1].data_ptr(), (0, 0, 0, 0), (1, 1, 1, 1)),
+             (blocks[2].data_ptr(), blocks[1].data_ptr(), (2, 2, 2, 2), (-1, -1, -1, -1)),
+             (blocks[0].data_ptr(), blocks[2].data_ptr(), (0, 0, 0, 0), (2, 2, 2, 2))]
+    xunit = 0.0535446292
+    ref = np.array([con.kernel_sum(a, b, g, y, xunit) for a, b, g, y in pairs])
+    got = np.array(con.window_kernel_sum(pairs, xunit, form=2))
+    err = np.abs(got - ref) / np.abs(ref)
+    judge, n = con.judge()
+    log("form 2 rel err per pair", err.ravel(), "judge", judge, n)
+    assert n > 0 and np.isfinite(judge)
+    assert np.all(err < 1e-2)
+    con.close()
