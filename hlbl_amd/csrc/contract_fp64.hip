@@ -282,7 +282,7 @@ __global__ __launch_bounds__(kP2Threads) void pass2_fp64(
     aoff[jq] = pj;
     asg[jq] = (double)(gamma_sign24(4, pj) * gamma_sign24(alam, j));
   }
-  // C rows of this lane: (mu, lam) = (lane/16, i); columns 16*cb + lane%16
+  // C rows of this lane: (mu, lam) = 4*i + lane/16 for its element i; columns 16*cb + lane%16
 
   double dacc = 0.0;
   const size_t begin = (size_t)blockIdx.x * sites_per_wg;
@@ -318,11 +318,12 @@ __global__ __launch_bounds__(kP2Threads) void pass2_fp64(
           c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, breg[1][i], c1, 0, 0, 0);
           c2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, breg[2][i], c2, 0, 0, 0);
         }
-        const double* cs = coef + (z0 + zi) * kCoefField + (4 * (lane >> 4)) * kCoefUsed + (lane & 15);
+        // FP64 16x16x4 result: element i of lane l is row 4*i + l/16, column l%16
+        const double* cs = coef + (z0 + zi) * kCoefField + (lane >> 4) * kCoefUsed + (lane & 15);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          dacc += c0[i] * cs[i * kCoefUsed] + c1[i] * cs[i * kCoefUsed + 16];
-          if ((lane & 15) < kCoefUsed - 32) dacc += c2[i] * cs[i * kCoefUsed + 32];
+          dacc += c0[i] * cs[4 * i * kCoefUsed] + c1[i] * cs[4 * i * kCoefUsed + 16];
+          if ((lane & 15) < kCoefUsed - 32) dacc += c2[i] * cs[4 * i * kCoefUsed + 32];
         }
       }
     }

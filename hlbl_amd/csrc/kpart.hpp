@@ -41,6 +41,7 @@ HLBL_HD void site_geom(size_t site, const hlbl_geom& global, const hlbl_geom& lo
                        double xunit, SiteGeom<R>& g) {
   int c[4];
   site_coords(site, local, c);
+#pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int L = geom_dim(global, r);
     const int xr = mod_pos(c[r] + coord_dim(proc, r) * geom_dim(local, r) - coord_dim(gsx, r), L);
@@ -59,23 +60,15 @@ HLBL_HD void site_geom(size_t site, const hlbl_geom& global, const hlbl_geom& lo
 // Argument pair of evaluation e of L0 at the site:
 //   0: (x, y)  1: (x, 0)  2: (y, x)  3: (0, x)  4: (x, d)  5: (0, d)
 // and the pair constants 6: (0, y)  7: (y, 0).
+// (Selects instead of pointers into g, so that g stays in registers.)
 template <class R>
 HLBL_HD void eval_args(int e, const SiteGeom<R>& g, R u[4], R v[4]) {
-  const R* p = nullptr;
-  const R* q = nullptr;
-  switch (e) {
-    case 0: p = g.xm; q = g.ym; break;
-    case 1: p = g.xm; break;
-    case 2: p = g.ym; q = g.xm; break;
-    case 3: q = g.xm; break;
-    case 4: p = g.xm; q = g.dm; break;
-    case 5: q = g.dm; break;
-    case 6: q = g.ym; break;
-    default: p = g.ym; break;
-  }
+  const bool ux = e == 0 || e == 1 || e == 4, uy = e == 2 || e == 7;
+  const bool vx = e == 2 || e == 3, vy = e == 0 || e == 6, vd = e == 4 || e == 5;
+#pragma unroll
   for (int r = 0; r < 4; ++r) {
-    u[r] = p ? p[r] : R(0);
-    v[r] = q ? q[r] : R(0);
+    u[r] = ux ? g.xm[r] : (uy ? g.ym[r] : R(0));
+    v[r] = vx ? g.xm[r] : (vy ? g.ym[r] : (vd ? g.dm[r] : R(0)));
   }
 }
 
@@ -97,14 +90,22 @@ __device__ inline R kp_tensor(int e, int i, const SiteGeom<R>& g, const R* ch_e)
   return qed::t_component(i, u, v, ch_e);
 }
 
-// D_{m a}(p) = (delta - Lambda p_m p_a) exp(-Lambda p^2 / 2), all 16 entries;
-// d = [3][16] for p = xm, ym, dm
+// v[i] as a blend of the four values: the compiler cannot turn it into a load
+// through a selected pointer, which would put the array in private memory.
 template <class R>
-__device__ inline R kp_dmat(int i, const SiteGeom<R>& g) {
-  const R* p = (i < 16) ? g.xm : (i < 32 ? g.ym : g.dm);
-  const int m = (i >> 2) & 3, a = i & 3;
-  const R e = exp(-R(qed::kLambda) * norm2(p) * R(0.5));
-  return (((m == a) ? R(1) : R(0)) - R(qed::kLambda) * p[m] * p[a]) * e;
+__device__ inline R blend4(const R v[4], int i) {
+  return v[0] * R(i == 0) + v[1] * R(i == 1) + v[2] * R(i == 2) + v[3] * R(i == 3);
+}
+
+// D_{m a}(p) = (delta - Lambda p_m p_a) exp(-Lambda p^2 / 2), entry i = m*4+a,
+// into d[i], d[16+i], d[32+i] for p = xm, ym, dm
+template <class R>
+__device__ inline void kp_dmat(int i, const SiteGeom<R>& g, R* d) {
+  const int m = i >> 2, a = i & 3;
+  const R dl = (m == a) ? R(1) : R(0), lam = R(qed::kLambda);
+  d[i] = (dl - lam * blend4(g.xm, m) * blend4(g.xm, a)) * exp(-lam * norm2(g.xm) * R(0.5));
+  d[16 + i] = (dl - lam * blend4(g.ym, m) * blend4(g.ym, a)) * exp(-lam * norm2(g.ym) * R(0.5));
+  d[32 + i] = (dl - lam * blend4(g.dm, m) * blend4(g.dm, a)) * exp(-lam * norm2(g.dm) * R(0.5));
 }
 
 // Phase 4: component o of K_m (m = 0: K1, 1: K2, 2: K3). L0 holds the eight
@@ -146,8 +147,9 @@ __device__ inline R kp_coef(int row, int col, const SiteGeom<R>& g, const R* K) 
   if (col < 40) {
     const int s = (col - 24) >> 2, nu = (col - 24) & 3;
     R r = R(0);
+#pragma unroll
     for (int k = 0; k < 6; ++k) {
-      const int rho = comb_rho(k), sigma = comb_sigma(k);
+      const int rho = comb_rho(k), sigma = comb_sigma(k);  // compile-time after unrolling
       const R b = K3[idx(k, lam, nu, mu)];
       if (sigma == s) r -= b * g.xz[rho];
       if (rho == s) r += b * g.xz[sigma];
@@ -166,8 +168,8 @@ constexpr int kKfieldLds = 8 * qed::kNCh + 8 * qed::kNT + 8 * qed::kNL + 48;
 // Pair constants L0(0, y), L0(y, 0) (evaluations 6 and 7) into the L0 area of
 // lds; the whole workgroup calls it and it ends with a barrier.
 template <class R, class TT>
-__device__ void kfield_constants(const TT* __restrict__ table, const qed::TermTable& tt,
-                                 const SiteGeom<R>& g, R* lds) {
+__device__ __forceinline__ void kfield_constants(const TT* __restrict__ table, const qed::TermTable& tt,
+                                                 const SiteGeom<R>& g, R* lds) {
   R* sCh = lds;
   R* sT = sCh + 8 * qed::kNCh;
   R* sL0 = sT + 8 * qed::kNT;
@@ -184,15 +186,32 @@ __device__ void kfield_constants(const TT* __restrict__ table, const qed::TermTa
   __syncthreads();
 }
 
+// Where the channels of the six site evaluations come from when they are not
+// interpolated: p[e] points at the 32 channels of evaluation e of this site
+// (none: interpolate). Passed by value and read through selects, so that it
+// stays in registers.
+struct ChannelRows {
+  const float* p[6];
+  bool given;
+  __device__ const float* row(int e) const {
+    return e == 0 ? p[0] : e == 1 ? p[1] : e == 2 ? p[2] : e == 3 ? p[3] : e == 4 ? p[4] : p[5];
+  }
+  __device__ static ChannelRows none() {
+    ChannelRows r{};
+    r.given = false;
+    return r;
+  }
+};
+
 // The 16 x 40 coefficients of one site into out[640] (element type O). The
-// channels of the six site evaluations come from the table, or from chbuf
-// ([6][32] of this site) when it is given. Requires kfield_constants on the
-// same lds (same pair). Whole workgroup; no barrier at the end (the next call
-// writes sCh first, and sT only after its first barrier).
+// channels of the six site evaluations come from the table, or from rows when
+// given. Requires kfield_constants on the same lds (same pair). Whole
+// workgroup; no barrier at the end (the next call writes sCh first, and sT only
+// after its first barrier).
 template <class R, class TT, class O>
-__device__ void kfield_site(const TT* __restrict__ table, const qed::TermTable& tt,
-                            const SiteGeom<R>& g, const float* __restrict__ chbuf, R* lds,
-                            O* __restrict__ out) {
+__device__ __forceinline__ void kfield_site(const TT* __restrict__ table, const qed::TermTable& tt,
+                                            const SiteGeom<R>& g, const ChannelRows rows, R* lds,
+                                            O* __restrict__ out) {
   R* sCh = lds;                     // [8][32]
   R* sT = sCh + 8 * qed::kNCh;      // [8][192]; K [3][384] over the first 1152
   R* sL0 = sT + 8 * qed::kNT;       // [8][384]
@@ -200,9 +219,9 @@ __device__ void kfield_site(const TT* __restrict__ table, const qed::TermTable& 
   R* sK = sT;
   const int tid = threadIdx.x, nt = blockDim.x;
   constexpr int nch = qed::kNChUsed;
-  if (chbuf) {
+  if (rows.given) {
     for (int t = tid; t < 6 * nch; t += nt)
-      sCh[(t / nch) * qed::kNCh + t % nch] = (R)chbuf[(t / nch) * qed::kNCh + t % nch];
+      sCh[(t / nch) * qed::kNCh + t % nch] = (R)rows.row(t / nch)[t % nch];
   } else {
     for (int t = tid; t < 6 * nch; t += nt)
       sCh[(t / nch) * qed::kNCh + t % nch] = kp_channel(t / nch, t % nch, g, table);
@@ -213,7 +232,7 @@ __device__ void kfield_site(const TT* __restrict__ table, const qed::TermTable& 
   __syncthreads();
   for (int t = tid; t < 6 * qed::kNL; t += nt)
     sL0[t] = qed::l0_component(tt, t % qed::kNL, sT + (t / qed::kNL) * qed::kNT);
-  if (tid < 48) sD[tid] = kp_dmat(tid, g);
+  if (tid < 16) kp_dmat(tid, g, sD);
   __syncthreads();
   for (int t = tid; t < 3 * qed::kNL; t += nt) sK[t] = kp_kernel(t / qed::kNL, t % qed::kNL, g, sL0, sD);
   __syncthreads();
@@ -223,16 +242,16 @@ __device__ void kfield_site(const TT* __restrict__ table, const qed::TermTable& 
 // The coefficient field coef[site - begin][16][40] of sites [begin, end) of a
 // pair, plain form (channels interpolated in `table`), whole workgroup.
 template <class R, class TT>
-__device__ void kfield_sites(const TT* __restrict__ table, const qed::TermTable& tt,
-                             hlbl_geom global, hlbl_geom local, hlbl_coord proc, hlbl_coord gsx,
-                             hlbl_coord yv, double xunit, size_t begin, size_t end,
-                             R* __restrict__ coef, R* lds) {
+__device__ __forceinline__ void kfield_sites(const TT* __restrict__ table, const qed::TermTable& tt,
+                                             hlbl_geom global, hlbl_geom local, hlbl_coord proc,
+                                             hlbl_coord gsx, hlbl_coord yv, double xunit,
+                                             size_t begin, size_t end, R* __restrict__ coef, R* lds) {
   SiteGeom<R> g;
   site_geom(0, global, local, proc, gsx, yv, xunit, g);
   kfield_constants(table, tt, g, lds);
   for (size_t site = begin; site < end; ++site) {
     site_geom(site, global, local, proc, gsx, yv, xunit, g);
-    kfield_site<R, TT, R>(table, tt, g, nullptr, lds, coef + (site - begin) * kCoefField);
+    kfield_site<R, TT, R>(table, tt, g, ChannelRows::none(), lds, coef + (site - begin) * kCoefField);
   }
 }
 

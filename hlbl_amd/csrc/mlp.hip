@@ -242,9 +242,12 @@ double mlp_loss_grad(hlbl_qed* q, const float* x, const float* t, int n, float* 
 
 // ---------------------------------------------------- training-set and stats --
 
-// envelope dividing the channels: exp(1 - (sqrt(1+a^2) + sqrt(1+b^2)) / 2)
-__host__ __device__ inline double mlp_envelope(double a, double b) {
-  return exp(1.0 - 0.5 * (sqrt(1.0 + a * a) + sqrt(1.0 + b * b)));
+// envelope dividing the channels: the exponential decay the six weight
+// functions share, exp(3/2 - (sqrt(1+a^2) + sqrt(1+b^2))/2 - sqrt(4+|x-y|^2)/4),
+// |x-y|^2 = a^2 + b^2 - 2abc
+__host__ __device__ inline double mlp_envelope(double a, double b, double c) {
+  const double d2 = a * a + b * b - 2.0 * a * b * c;
+  return exp(1.5 - 0.5 * (sqrt(1.0 + a * a) + sqrt(1.0 + b * b)) - 0.25 * sqrt(4.0 + (d2 > 0 ? d2 : 0.0)));
 }
 
 // counter-based uniform in [0, 1)
@@ -273,7 +276,7 @@ void mlp_build_set(uint64_t seed, int n, std::vector<float>& x, std::vector<doub
         double a, c, b;
         mlp_sample_inputs(seed, (size_t)i, a, c, b);
         qed::weight_channels(a * a, b * b, a * b * c, ch);
-        const double env = mlp_envelope(a, b);
+        const double env = mlp_envelope(a, b, c);
         x[(size_t)i * 3 + 0] = (float)(a / qed::kRadius);
         x[(size_t)i * 3 + 1] = (float)(b / qed::kRadius);
         x[(size_t)i * 3 + 2] = (float)c;
@@ -367,8 +370,9 @@ __global__ __launch_bounds__(kInfThreads) void mlp_infer_rows(const unsigned sho
   }
 }
 
-// Channels of the six site evaluations of every site for every distinct yv
-// (gsx = 0): chbuf[u][site][e][32] = envelope * (mean + sd * net).
+// Surrogate channels of a window (gsx = 0), chbuf[block][site][32] =
+// envelope * (mean + sd * net), for the blocks 0: (x, 0) and 1: (0, x), shared
+// by every yv, and 2 + 3u + j: (x, y), (y, x), (x, x - y) of the distinct yv u.
 __global__ __launch_bounds__(kInfThreads) void mlp_surrogate_channels(
     const unsigned short* __restrict__ w16g, const float* __restrict__ scale,
     const hlbl_coord* __restrict__ yvs, int n_unique, hlbl_geom global, double xunit,
@@ -385,7 +389,7 @@ __global__ __launch_bounds__(kInfThreads) void mlp_surrogate_channels(
   }
   __syncthreads();
   const size_t V = geom_volume(global);
-  const size_t rows = (size_t)n_unique * V * 6;
+  const size_t rows = (size_t)(2 + 3 * n_unique) * V;
   const hlbl_coord zero = {0, 0, 0, 0};
   const float inv_r = (float)(1.0 / qed::kRadius);
   for (size_t t0 = (size_t)blockIdx.x * kInfRows; t0 < rows; t0 += (size_t)gridDim.x * kInfRows) {
@@ -394,15 +398,16 @@ __global__ __launch_bounds__(kInfThreads) void mlp_surrogate_channels(
       const size_t row = t0 + r;
       float a = 0.f, c = 0.f, b = 0.f, env = 0.f;
       if (row < rows) {
-        const int e = (int)(row % 6);
-        const size_t site = (row / 6) % V;
-        const int u = (int)(row / (6 * V));
+        const int blk = (int)(row / V);
+        const size_t site = row % V;
+        const int u = blk < 2 ? 0 : (blk - 2) / 3;
+        const int e = blk == 0 ? 1 : blk == 1 ? 3 : 2 * ((blk - 2) % 3);
         SiteGeom<float> g;
         site_geom(site, global, global, zero, zero, yvs[u], xunit, g);
         float pu[4], pv[4];
         eval_args(e, g, pu, pv);
         qed::acb(pu, pv, a, c, b);
-        env = (float)mlp_envelope(a, b);
+        env = (float)mlp_envelope(a, b, c);
       }
       act[r * kMlpHidden + 0] = a * inv_r;
       act[r * kMlpHidden + 1] = b * inv_r;
@@ -426,6 +431,41 @@ void launch_surrogate_channels(hlbl_contractor* c, WindowPlan& w, hipStream_t st
                      q->mlp_w_bf16.p, q->mlp_scale.p, w.d_yvs.p, w.n_unique, c->global, w.xunit,
                      w.chbuf.p);
   check_hip(hipGetLastError(), "mlp_surrogate_channels launch");
+}
+
+// surrogate channels at points (a, cos beta, b): inputs, network, envelope and scale
+__global__ void k_surrogate_inputs(const double* __restrict__ acb, int n, float* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[3 * i + 0] = (float)(acb[3 * i + 0] / qed::kRadius);
+  x[3 * i + 1] = (float)(acb[3 * i + 2] / qed::kRadius);
+  x[3 * i + 2] = (float)acb[3 * i + 1];
+}
+__global__ void k_surrogate_outputs(const double* __restrict__ acb, const float* __restrict__ y,
+                                    const float* __restrict__ scale, int n, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * kMlpOut) return;
+  const int r = i / kMlpOut, c = i % kMlpOut;
+  const double env = mlp_envelope(acb[3 * r], acb[3 * r + 2], acb[3 * r + 1]);
+  out[i] = env * ((double)scale[c] + (double)scale[kMlpOut + c] * (double)y[i]);
+}
+
+void mlp_infer(hlbl_qed* q, const float* x, int n, float* y, hipStream_t st);
+
+void launch_surrogate_points(hlbl_qed* q, const double* acb, int n, double* out, hipStream_t st) {
+  if (!q->mlp_ready) throw Error("hlbl_qed_channels: mode 1 needs a trained surrogate");
+  mlp_configure_kernels();
+  float *x = nullptr, *y = nullptr;
+  check_hip(hipMallocAsync((void**)&x, (size_t)n * 3 * sizeof(float), st), "hipMallocAsync");
+  check_hip(hipMallocAsync((void**)&y, (size_t)n * kMlpOut * sizeof(float), st), "hipMallocAsync");
+  hipLaunchKernelGGL(k_surrogate_inputs, dim3((unsigned)div_up(n, 256)), dim3(256), 0, st, acb, n, x);
+  mlp_infer(q, x, n, y, st);
+  hipLaunchKernelGGL(k_surrogate_outputs, dim3((unsigned)div_up((size_t)n * kMlpOut, 256)), dim3(256), 0,
+                     st, acb, y, q->mlp_scale.p, n, out);
+  hipError_t e = hipGetLastError();
+  (void)hipFreeAsync(x, st);
+  (void)hipFreeAsync(y, st);
+  check_hip(e, "surrogate channels launch");
 }
 
 void mlp_infer(hlbl_qed* q, const float* x, int n, float* y, hipStream_t st) {

@@ -130,23 +130,29 @@ HLBL_HD void acb(const R x[4], const R y[4], R& a, R& c, R& b) {
 
 // ---- T tensors -------------------------------------------------------------
 
+// v[i] for a runtime i without indexing the array (keeps v in registers)
+template <class R>
+HLBL_HD R pick4(const R v[4], int i) {
+  return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3]));
+}
+
 // d^x_al F = 2 F_u x_al + F_w y_al
 template <class R>
-HLBL_HD R dx(R fu, R fw, const R x[4], const R y[4], int al) {
-  return R(2) * fu * x[al] + fw * y[al];
+HLBL_HD R dx(R fu, R fw, R xa, R ya) {
+  return R(2) * fu * xa + fw * ya;
 }
 // (d^x_be + d^y_be) F = (2 F_u + F_w) x_be + (2 F_v + F_w) y_be
 template <class R>
-HLBL_HD R dsum(R fu, R fv, R fw, const R x[4], const R y[4], int be) {
-  return (R(2) * fu + fw) * x[be] + (R(2) * fv + fw) * y[be];
+HLBL_HD R dsum(R fu, R fv, R fw, R xb, R yb) {
+  return (R(2) * fu + fw) * xb + (R(2) * fv + fw) * yb;
 }
 // d^x_al (d^x_be + d^y_be) F, from F's channels starting at base (U V W UU UV UW VW WW)
 template <class R>
-HLBL_HD R dxdsum(const R* f, const R x[4], const R y[4], int al, int be) {
+HLBL_HD R dxdsum(const R* f, R xa, R ya, R xb, R yb, bool ab) {
   R fu = f[0], fw = f[2], fuu = f[3], fuv = f[4], fuw = f[5], fvw = f[6], fww = f[7];
-  R r = (al == be) ? (R(2) * fu + fw) : R(0);
-  r += (R(2) * (R(2) * fuu + fuw) * x[al] + (R(2) * fuw + fww) * y[al]) * x[be];
-  r += (R(2) * (R(2) * fuv + fuw) * x[al] + (R(2) * fvw + fww) * y[al]) * y[be];
+  R r = ab ? (R(2) * fu + fw) : R(0);
+  r += (R(2) * (R(2) * fuu + fuw) * xa + (R(2) * fuw + fww) * ya) * xb;
+  r += (R(2) * (R(2) * fuv + fuw) * xa + (R(2) * fvw + fww) * ya) * yb;
   return r;
 }
 
@@ -158,11 +164,13 @@ HLBL_HD R t_component(int i, const R x[4], const R y[4], const R* ch) {
   const R dab = al == be ? R(1) : R(0);
   const R dad = al == de ? R(1) : R(0);
   const R dbd = be == de ? R(1) : R(0);
+  const R xa = pick4(x, al), xb = pick4(x, be), xd = pick4(x, de);
+  const R ya = pick4(y, al), yb = pick4(y, be), yd = pick4(y, de);
   if (A == 0) {
-    R r = dbd * (dx(ch[G1U], ch[G1W], x, y, al) + dx(ch[G2U], ch[G2W], x, y, al));
-    r += dad * dsum(ch[G1U], ch[G1V], ch[G1W], x, y, be);
-    r += x[de] * dxdsum(ch + G1U, x, y, al, be);
-    r += y[de] * dxdsum(ch + G2U, x, y, al, be);
+    R r = dbd * (dx(ch[G1U], ch[G1W], xa, ya) + dx(ch[G2U], ch[G2W], xa, ya));
+    r += dad * dsum(ch[G1U], ch[G1V], ch[G1W], xb, yb);
+    r += xd * dxdsum(ch + G1U, xa, ya, xb, yb, al == be);
+    r += yd * dxdsum(ch + G2U, xa, ya, xb, yb, al == be);
     return r;
   }
   const R u = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
@@ -170,21 +178,21 @@ HLBL_HD R t_component(int i, const R x[4], const R y[4], const R* ch) {
   const R w = x[0] * y[0] + x[1] * y[1] + x[2] * y[2] + x[3] * y[3];
   const R q = R(0.25), h = R(0.5);
   if (A == 1) {
-    R r = (dab * x[de] + x[be] * dad - h * dbd * x[al]) * ch[L1];
-    r += (x[be] * x[de] - q * dbd * u) * dx(ch[L1U], ch[L1W], x, y, al);
-    r += (y[be] * y[de] - q * dbd * v) * dx(ch[L2U], ch[L2W], x, y, al);
-    r += (dab * y[de] + y[be] * dad - h * dbd * y[al]) * ch[L3];
-    r += (x[be] * y[de] + y[be] * x[de] - h * dbd * w) * dx(ch[L3U], ch[L3W], x, y, al);
-    r += q * dbd * dx(ch[G0U], ch[G0W], x, y, al);
+    R r = (dab * xd + xb * dad - h * dbd * xa) * ch[L1];
+    r += (xb * xd - q * dbd * u) * dx(ch[L1U], ch[L1W], xa, ya);
+    r += (yb * yd - q * dbd * v) * dx(ch[L2U], ch[L2W], xa, ya);
+    r += (dab * yd + yb * dad - h * dbd * ya) * ch[L3];
+    r += (xb * yd + yb * xd - h * dbd * w) * dx(ch[L3U], ch[L3W], xa, ya);
+    r += q * dbd * dx(ch[G0U], ch[G0W], xa, ya);
     return r;
   }
-  R r = (dab * x[de] + x[al] * dbd - h * dad * x[be]) * ch[L1];
-  r += (x[al] * x[de] - q * dad * u) * dsum(ch[L1U], ch[L1V], ch[L1W], x, y, be);
-  r += (dab * y[de] + y[al] * dbd - h * dad * y[be]) * ch[L2];
-  r += (y[al] * y[de] - q * dad * v) * dsum(ch[L2U], ch[L2V], ch[L2W], x, y, be);
-  r += (dab * (x[de] + y[de]) + dbd * (x[al] + y[al]) - h * dad * (x[be] + y[be])) * ch[L3];
-  r += (x[al] * y[de] + y[al] * x[de] - h * dad * w) * dsum(ch[L3U], ch[L3V], ch[L3W], x, y, be);
-  r += q * dad * dsum(ch[G0U], ch[G0V], ch[G0W], x, y, be);
+  R r = (dab * xd + xa * dbd - h * dad * xb) * ch[L1];
+  r += (xa * xd - q * dad * u) * dsum(ch[L1U], ch[L1V], ch[L1W], xb, yb);
+  r += (dab * yd + ya * dbd - h * dad * yb) * ch[L2];
+  r += (ya * yd - q * dad * v) * dsum(ch[L2U], ch[L2V], ch[L2W], xb, yb);
+  r += (dab * (xd + yd) + dbd * (xa + ya) - h * dad * (xb + yb)) * ch[L3];
+  r += (xa * yd + ya * xd - h * dad * w) * dsum(ch[L3U], ch[L3V], ch[L3W], xb, yb);
+  r += q * dad * dsum(ch[G0U], ch[G0V], ch[G0W], xb, yb);
   return r;
 }
 

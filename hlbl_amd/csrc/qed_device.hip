@@ -78,10 +78,16 @@ void launch_qed_kernel(const hlbl_qed* q, int mode, const double* x, const doubl
   check_hip(e, "qed_kernel_points launch");
 }
 
+void launch_surrogate_points(hlbl_qed* q, const double* acb, int n, double* out, hipStream_t st);
+
 void launch_qed_channels(const hlbl_qed* q, int mode, const double* acb, int n, double* out,
                          hipStream_t st) {
-  if (mode != 0) throw Error("hlbl_qed_channels: mode 1 goes through hlbl_mlp_forward");
+  if (mode != 0 && mode != 1) throw Error("hlbl_qed_channels: mode must be 0 (table) or 1 (surrogate)");
   if (n <= 0) return;
+  if (mode == 1) {
+    launch_surrogate_points(const_cast<hlbl_qed*>(q), acb, n, out, st);
+    return;
+  }
   hipLaunchKernelGGL(qed_channels_points, dim3((n + 127) / 128), dim3(128), 0, st, q->table64.p,
                      acb, n, out);
   check_hip(hipGetLastError(), "qed_channels_points launch");

@@ -36,7 +36,7 @@ struct WindowPlan {
   DevBuf<double> part1, xi, dz, gdz;
   DevBuf<float> bmat;
   DevBuf<float> coef;      // [n_unique][V][640]
-  DevBuf<float> chbuf;     // surrogate channels [n_unique][V][6][32]
+  DevBuf<float> chbuf;     // surrogate channels [2 + 3 n_unique][V][32]
   DevBuf<double> part2, result;
   PinBuf<double> h_result;
   DevBuf<size_t> d_samples;

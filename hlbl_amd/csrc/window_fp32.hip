@@ -147,7 +147,7 @@ __global__ void reduce_window1(const double* __restrict__ part, int nwg, int len
 // The field F_yv[rel site][16][40] (FP32) for each distinct yv, gsx = 0. A site
 // with |x| or |x - y| below kNearRadius is evaluated in FP64 with the FP64
 // table; the others in FP32 from the FP32 table, or from the surrogate's
-// channels chbuf[u][site][6][32] when given.
+// channels (chbuf, see below) when given.
 constexpr int kKmThreads = 256;
 
 __global__ __launch_bounds__(kKmThreads) void kfield_mixed(
@@ -178,11 +178,31 @@ __global__ __launch_bounds__(kKmThreads) void kfield_mixed(
       site_geom(site, global, global, zero, zero, yv, xunit, g64);
       const bool near = norm2(g64.xm) < near2 || norm2(g64.dm) < near2;
       if (near) {
-        kfield_site<double, double, float>(table64, c_terms32, g64, nullptr, lds64, cu + site * kCoefField);
+        kfield_site<double, double, float>(table64, c_terms32, g64, ChannelRows::none(), lds64,
+                                           cu + site * kCoefField);
       } else {
         site_geom(site, global, global, zero, zero, yv, xunit, g32);
-        const float* cb = chbuf ? chbuf + ((size_t)u * V + site) * 6 * qed::kNCh : nullptr;
-        kfield_site<float, float, float>(table32, c_terms32, g32, cb, lds32, cu + site * kCoefField);
+        if (chbuf) {
+          // surrogate channel rows: blocks 0 (x,0) and 1 (0,x) shared by every yv,
+          // block 2+3u+j the yv-dependent (x,y), (y,x), (x,d); (0,d) is (0,x) at
+          // the site d = x - y
+          int crd[4], dsh[4];
+          site_coords(site, global, crd);
+          for (int r = 0; r < 4; ++r) dsh[r] = mod_pos(crd[r] - coord_dim(yv, r), geom_dim(global, r));
+          const size_t sd = (((size_t)dsh[0] * global.LX + dsh[1]) * global.LY + dsh[2]) * global.LZ + dsh[3];
+          ChannelRows rows;
+          rows.given = true;
+          rows.p[0] = chbuf + ((size_t)(2 + 3 * u + 0) * V + site) * qed::kNCh;
+          rows.p[1] = chbuf + ((size_t)0 * V + site) * qed::kNCh;
+          rows.p[2] = chbuf + ((size_t)(2 + 3 * u + 1) * V + site) * qed::kNCh;
+          rows.p[3] = chbuf + ((size_t)1 * V + site) * qed::kNCh;
+          rows.p[4] = chbuf + ((size_t)(2 + 3 * u + 2) * V + site) * qed::kNCh;
+          rows.p[5] = chbuf + ((size_t)1 * V + sd) * qed::kNCh;
+          kfield_site<float, float, float>(table32, c_terms32, g32, rows, lds32, cu + site * kCoefField);
+        } else {
+          kfield_site<float, float, float>(table32, c_terms32, g32, ChannelRows::none(), lds32,
+                                           cu + site * kCoefField);
+        }
       }
       __syncthreads();
     }
@@ -207,7 +227,7 @@ __global__ __launch_bounds__(kKmThreads) void judge_coef(
   kfield_constants(table64, c_terms32, g, lds64);
   const size_t site = samples[k];
   site_geom(site, global, global, zero, zero, yvs[u], xunit, g);
-  kfield_site<double, double, double>(table64, c_terms32, g, nullptr, lds64, ref);
+  kfield_site<double, double, double>(table64, c_terms32, g, ChannelRows::none(), lds64, ref);
   __syncthreads();
   const float* c = coef + ((size_t)u * V + site) * kCoefField;
   double dmax = 0.0, rmax = 0.0;

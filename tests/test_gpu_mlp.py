@@ -37,7 +37,7 @@ def bf16_round(a):
 
 def oracle_mlp(oracle, params, x, t, want_grad=True):
     oracle.oracle_mlp.restype = ctypes.c_double
-    oracle.oracle_mlp.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    oracle.oracle_mlp.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
     n = len(x)
     g = np.zeros(len(params)) if want_grad else None
     y = np.zeros((n, 31))
@@ -109,21 +109,12 @@ def test_surrogate_gain_at_equal_storage(qed, oracle, trained):
         oracle.oracle_channels_exact(a[i] ** 2, b[i] ** 2, a[i] * b[i] * c[i], P(exact[i]))
     exact = exact[:, :31]
     # the network (BF16 weights) through its standardisation and envelope
-    x = np.stack([a / R, b / R, c], 1).astype(np.float32)
-    dx = torch.tensor(x, device="cuda")
-    dy = torch.zeros((n, 31), dtype=torch.float32, device="cuda")
-    qed.mlp_forward(dx.data_ptr(), n, dy.data_ptr(), precision=1)
-    y = dy.cpu().numpy().astype(np.float64)
-    sc = np.zeros(62, np.float32)
-    # the standardisation is inside the library; recover it from exact targets by least squares per channel
-    env = np.exp(1 - 0.5 * (np.sqrt(1 + a ** 2) + np.sqrt(1 + b ** 2)))
-    t = exact / env[:, None]
-    coef = [np.polyfit(y[:, k], t[:, k], 1) for k in range(31)]
-    mlp = np.stack([np.polyval(coef[k], y[:, k]) for k in range(31)], 1) * env[:, None]
-    full = np.zeros((n, 32))
-    for i in range(n):
-        oracle.oracle_channels_interp(a[i], c[i], b[i], P(full[i]))
-    full = full[:, :31]
+    d_acb = torch.tensor(np.stack([a, c, b], 1), device="cuda")
+    d_out = torch.zeros((n, 31), dtype=torch.float64, device="cuda")
+    qed.channels(d_acb.data_ptr(), n, d_out.data_ptr(), mode=1)
+    mlp = d_out.cpu().numpy()
+    qed.channels(d_acb.data_ptr(), n, d_out.data_ptr(), mode=0)
+    full = d_out.cpu().numpy()
     # equal-storage table: 12 x 12 x 12 grid x 31 channels in FP32 ~ 214 KB (>= the network's 108 KB)
     na = nc = nb = 12
     ga, gc, gb = np.linspace(0, R, na), np.linspace(-1, 1, nc), np.linspace(0, R, nb)
