@@ -220,7 +220,6 @@ int hlbl_contractor_create(hlbl_contractor** out, hlbl_geom global, hlbl_geom lo
       c->volume = geom_volume(local);
       check_hip(hipSetDevice(qed->device), "hipSetDevice");
       c->n_cu = device_cu_count(qed->device);
-      c->nwg1 = 2 * c->n_cu;
       c->nwg2 = 2 * c->n_cu;
       c->result.ensure(2);
       c->host_result.ensure(2);

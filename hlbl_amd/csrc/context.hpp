@@ -116,8 +116,9 @@ struct hlbl_contractor {
   hlbl_qed* qed = nullptr;
   size_t volume = 0;
   int n_cu = 0;
-  int nwg1 = 0, nwg2 = 0;
-  hlbl::DevBuf<double> part1;   // pass-1 partial sums [nwg1][23040]
+  int nwg2 = 0;
+  hlbl::DevBuf<double> part1;   // pass-1 plane partials [2][planes][3][2304]
+  hlbl::DevBuf<double> red1;    // pass-1 chunk sums [2][chunks][5][2304]
   hlbl::DevBuf<double> xi;      // reduced [2][8][5][144][2]
   hlbl::DevBuf<double> dz;      // dzu, dzsu of both flavours [2][10][144][2]
   hlbl::DevBuf<double> gdz;     // g_dzu, g_dzsu of both flavours [2][40][144][2]

@@ -9,6 +9,7 @@
 #include "contract_kernels.hpp"
 #include "kpart.hpp"
 #include "lattice.hpp"
+#include "pass1.hpp"
 #include "pass_common.hpp"
 #include "qed.hpp"
 #include "window.hpp"
@@ -24,116 +25,8 @@ void upload_terms_fp64(const qed::TermTable& t) {
 typedef double d4v __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------- pass 1 --
-// Thread t owns flavour f = t/128, spin pair p = (t/16)%8 and a 3x3 block of
-// (a, b). Pair p = cls*4 + s couples sink spin s of y with spin s' of the
-// source block: s' = s^2 (cls 0, gamma_0 and gamma_3) or 3-s (cls 1,
-// gamma_1 and gamma_2). It accumulates, for the five weights 1, xz_0..xz_3,
-//   Xi^W[p][a][b] = sum_z W(z) sum_colour S_f[a](z)[s',colour] conj(Y_{1-f}[b](z)[s,colour]).
-constexpr int kP1Threads = 256;
-constexpr int kP1Sites = 4;
-constexpr int kXiLen = 2 * 8 * 5 * 144 * 2;  // per workgroup partial
-
-__global__ __launch_bounds__(kP1Threads) void pass1_fp64(
-    const double* __restrict__ S, const double* __restrict__ Y, hlbl_geom global, hlbl_geom local,
-    hlbl_coord proc, hlbl_coord gsx, int flav_mask, size_t sites_per_wg,
-    double* __restrict__ part) {
-  __shared__ double sh[kP1Sites][2][2][12][24];  // [site][S/Y][flavour][a][24]
-  const int tid = threadIdx.x;
-  const int f = tid >> 7;
-  const int p = (tid >> 4) & 7;
-  const int blk = tid & 15;
-  const int a0 = 3 * (blk >> 2), b0 = 3 * (blk & 3);
-  const int s = p & 3;
-  const int sp = (p < 4) ? (s ^ 2) : (3 - s);
-  const size_t V = geom_volume(local);
-  const size_t begin = (size_t)blockIdx.x * sites_per_wg;
-  const size_t end = begin + sites_per_wg < V ? begin + sites_per_wg : V;
-  const bool active = (flav_mask >> f) & 1;
-
-  double acc[5][9][2];
-#pragma unroll
-  for (int w = 0; w < 5; ++w)
-#pragma unroll
-    for (int i = 0; i < 9; ++i) acc[w][i][0] = acc[w][i][1] = 0.0;
-
-  for (size_t z0 = begin; z0 < end; z0 += kP1Sites) {
-    const int ns = (end - z0) < (size_t)kP1Sites ? (int)(end - z0) : kP1Sites;
-    __syncthreads();
-    // stage: segment q = (blk, flavour, a) holds kP1Sites*24 contiguous doubles
-    for (int i = tid; i < 48 * kP1Sites * 24; i += kP1Threads) {
-      const int q = i / (kP1Sites * 24), r = i % (kP1Sites * 24);
-      const int zi = r / 24, c = r % 24;
-      const int bl = q / 24, ff = (q / 12) & 1, aa = q % 12;
-      double v = 0.0;
-      if (zi < ns) v = (bl ? Y : S)[(((size_t)(ff * 12 + aa)) * V + z0 + zi) * 24 + c];
-      sh[zi][bl][ff][aa][c] = v;
-    }
-    __syncthreads();
-    if (!active) continue;
-    for (int zi = 0; zi < ns; ++zi) {
-      int crd[4];
-      site_coords(z0 + zi, local, crd);
-      double w[5];
-      w[0] = 1.0;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int L = geom_dim(global, r);
-        const int xr = mod_pos(crd[r] + coord_dim(proc, r) * geom_dim(local, r) - coord_dim(gsx, r), L);
-        w[1 + r] = (double)coord_map_zerohalf(xr, L);
-      }
-      double sr[3][3], si[3][3], yr[3][3], yi[3][3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int col = 0; col < 3; ++col) {
-          const double* ps = &sh[zi][0][f][a0 + i][2 * (3 * sp + col)];
-          const double* py = &sh[zi][1][1 - f][b0 + i][2 * (3 * s + col)];
-          sr[i][col] = ps[0];
-          si[i][col] = ps[1];
-          yr[i][col] = py[0];
-          yi[i][col] = py[1];
-        }
-#pragma unroll
-      for (int ia = 0; ia < 3; ++ia)
-#pragma unroll
-        for (int ib = 0; ib < 3; ++ib) {
-          double re = 0.0, im = 0.0;
-#pragma unroll
-          for (int col = 0; col < 3; ++col) {
-            re += sr[ia][col] * yr[ib][col] + si[ia][col] * yi[ib][col];
-            im += si[ia][col] * yr[ib][col] - sr[ia][col] * yi[ib][col];
-          }
-#pragma unroll
-          for (int ww = 0; ww < 5; ++ww) {
-            acc[ww][ia * 3 + ib][0] += w[ww] * re;
-            acc[ww][ia * 3 + ib][1] += w[ww] * im;
-          }
-        }
-    }
-  }
-  if (!active) return;
-  double* out = part + (size_t)blockIdx.x * kXiLen;
-#pragma unroll
-  for (int ww = 0; ww < 5; ++ww)
-#pragma unroll
-    for (int ia = 0; ia < 3; ++ia)
-#pragma unroll
-      for (int ib = 0; ib < 3; ++ib) {
-        const size_t o = ((((size_t)f * 8 + p) * 5 + ww) * 144 + (a0 + ia) * 12 + (b0 + ib)) * 2;
-        out[o] = acc[ww][ia * 3 + ib][0];
-        out[o + 1] = acc[ww][ia * 3 + ib][1];
-      }
-}
-
-// fixed-order sum of the workgroup partials; xi[i] (+)= sum_wg part[wg][i]
-__global__ void reduce_partials(const double* __restrict__ part, int nwg, int len,
-                                double* __restrict__ out, int accumulate) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= len) return;
-  double s = 0.0;
-  for (int w = 0; w < nwg; ++w) s += part[(size_t)w * len + i];
-  out[i] = accumulate ? out[i] + s : s;
-}
+// Pass 1 itself runs in pass1.hip; xi per pair is [2][8][5][144][2].
+constexpr int kXiLen = 2 * 8 * 5 * 144 * 2;
 
 // Per pair (blockIdx.y), with strides kXiLen (xi), kDzLen (dz), kGdzLen (gdz),
 // kBmatLen (bmat):
@@ -361,16 +254,15 @@ size_t div_up(size_t a, size_t b) { return (a + b - 1) / b; }
 void launch_pass1(hlbl_contractor* c, const double* S, const double* Y, hlbl_coord proc,
                   hlbl_coord gsx, hlbl_geom global, hlbl_geom local, int flav_mask,
                   hipStream_t st) {
-  const size_t V = geom_volume(local);
-  const size_t spw = div_up(div_up(V, (size_t)c->nwg1), kP1Sites) * kP1Sites;
-  const int nwg = (int)div_up(V, spw);
-  c->part1.ensure((size_t)c->nwg1 * kXiLen);
+  c->part1.ensure(pass1_part_len(local, 1));
+  c->red1.ensure(pass1_red_len(local, 1));
   c->xi.ensure(kXiLen);
-  hipLaunchKernelGGL(pass1_fp64, dim3(nwg), dim3(kP1Threads), 0, st, S, Y, global, local, proc,
-                     gsx, flav_mask, spw, c->part1.p);
-  check_hip(hipGetLastError(), "pass1_fp64 launch");
-  hipLaunchKernelGGL(reduce_partials, dim3(div_up(kXiLen, 256)), dim3(256), 0, st, c->part1.p,
-                     nwg, kXiLen, c->xi.p, 0);
+  WinPair single{};
+  single.S = S;
+  single.Y = Y;
+  single.gsx = gsx;
+  launch_pass1_planes<double>(nullptr, single, 1, global, local, proc, flav_mask, c->part1.p,
+                              c->red1.p, c->xi.p, st);
   c->dz.ensure(2 * 10 * 144 * 2);
   hipLaunchKernelGGL(form_dz, dim3(div_up(2 * 10 * 144, 256), 1), dim3(256), 0, st, c->xi.p, c->dz.p);
   check_hip(hipGetLastError(), "pass1 reduction launch");

@@ -33,7 +33,8 @@ struct WindowPlan {
   DevBuf<hlbl_coord> d_yvs;
   PinBuf<WinPair> h_pairs;
   PinBuf<hlbl_coord> h_yvs;
-  DevBuf<double> part1, xi, dz, gdz;
+  DevBuf<float> part1;     // pass-1 plane partials
+  DevBuf<double> red1, xi, dz, gdz;
   DevBuf<float> bmat;
   DevBuf<float> coef;      // [n_unique][V][640]
   DevBuf<float> chbuf;     // surrogate channels [2 + 3 n_unique][V][32]
@@ -54,7 +55,6 @@ struct WindowPlan {
 // reductions, 1 = coefficient fields (and the judge), 2 = pass 2 and its reduction
 void window_record(hlbl_contractor* c, WindowPlan& w, hipStream_t st);
 void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st);
-size_t window_part1_len(const hlbl_contractor* c, int n_pairs);
 size_t window_part2_len(const hlbl_contractor* c, int n_pairs);
 
 // dz, gdz (FP64) and the B operands (FP64 into bmat64 or FP32 into bmat32) of

@@ -1,8 +1,8 @@
 // Window-batched mixed-precision contraction: all pairs of a window and both
-// flavours per call. Pass 1 and pass 2 are each one persistent launch over the
-// pairs. Propagator products run in FP32 (pass-2 traces on FP32 MFMA), the
-// per-workgroup lattice sums and the per-pair sums are FP64, combined with
-// Kahan compensation. The coefficient field of the QED kernel depends only on
+// flavours per call. Pass 1 (pass1.hip) is one launch over (plane, flavour,
+// pair), pass 2 one persistent launch over the pairs. Propagator products run in
+// FP32 (pass-2 traces on FP32 MFMA), the lattice sums and the per-pair sums are
+// FP64, combined with Kahan compensation. The coefficient field of the QED kernel depends only on
 // (x - gsx, yv), so it is built once per distinct yv of the window (at gsx = 0)
 // and read at the shifted site; sites near the origin (|x| or |x - y| below
 // kNearRadius) evaluate the kernel in FP64.
@@ -16,6 +16,7 @@
 #include "context.hpp"
 #include "contract_kernels.hpp"
 #include "kpart.hpp"
+#include "pass1.hpp"
 #include "pass_common.hpp"
 #include "qed.hpp"
 #include "window.hpp"
@@ -33,115 +34,6 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 namespace {
 size_t div_up(size_t a, size_t b) { return (a + b - 1) / b; }
 }  // namespace
-
-// ---------------------------------------------------------------- pass 1 --
-// As pass1_fp64 (thread = flavour, spin pair, 3x3 block of (a, b)), with the
-// spinors staged in LDS as FP32 and FP32 colour products; the weighted sums
-// over sites are FP64. One launch runs every pair of the window.
-constexpr int kM1Threads = 256;
-constexpr int kM1Sites = 8;
-constexpr int kXiLenM = 2 * 8 * 5 * 144 * 2;
-
-__global__ __launch_bounds__(kM1Threads) void pass1_mixed(
-    const WinPair* __restrict__ pairs, int n_pairs, hlbl_geom global, hlbl_geom local,
-    hlbl_coord proc, size_t sites_per_wg, int nwg, double* __restrict__ part) {
-  __shared__ float sh[kM1Sites][2][2][12][24];
-  const int tid = threadIdx.x;
-  const int f = tid >> 7;
-  const int p = (tid >> 4) & 7;
-  const int blk = tid & 15;
-  const int a0 = 3 * (blk >> 2), b0 = 3 * (blk & 3);
-  const int s = p & 3;
-  const int sp = (p < 4) ? (s ^ 2) : (3 - s);
-  const size_t V = geom_volume(local);
-  const size_t begin = (size_t)blockIdx.x * sites_per_wg;
-  const size_t end = begin + sites_per_wg < V ? begin + sites_per_wg : V;
-
-  for (int ip = 0; ip < n_pairs; ++ip) {
-    const WinPair wp = pairs[ip];
-    double acc[5][9][2];
-#pragma unroll
-    for (int w = 0; w < 5; ++w)
-#pragma unroll
-      for (int i = 0; i < 9; ++i) acc[w][i][0] = acc[w][i][1] = 0.0;
-    for (size_t z0 = begin; z0 < end; z0 += kM1Sites) {
-      const int ns = (end - z0) < (size_t)kM1Sites ? (int)(end - z0) : kM1Sites;
-      __syncthreads();
-      for (int i = tid; i < 48 * kM1Sites * 24; i += kM1Threads) {
-        const int q = i / (kM1Sites * 24), r = i % (kM1Sites * 24);
-        const int zi = r / 24, c = r % 24;
-        const int bl = q / 24, ff = (q / 12) & 1, aa = q % 12;
-        float v = 0.f;
-        if (zi < ns) v = (float)(bl ? wp.Y : wp.S)[(((size_t)(ff * 12 + aa)) * V + z0 + zi) * 24 + c];
-        sh[zi][bl][ff][aa][c] = v;
-      }
-      __syncthreads();
-      for (int zi = 0; zi < ns; ++zi) {
-        int crd[4];
-        site_coords(z0 + zi, local, crd);
-        double w[5];
-        w[0] = 1.0;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int L = geom_dim(global, r);
-          const int xr = mod_pos(crd[r] + coord_dim(proc, r) * geom_dim(local, r) - coord_dim(wp.gsx, r), L);
-          w[1 + r] = (double)coord_map_zerohalf(xr, L);
-        }
-        float sr[3][3], si[3][3], yr[3][3], yi[3][3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-          for (int col = 0; col < 3; ++col) {
-            const float* ps = &sh[zi][0][f][a0 + i][2 * (3 * sp + col)];
-            const float* py = &sh[zi][1][1 - f][b0 + i][2 * (3 * s + col)];
-            sr[i][col] = ps[0];
-            si[i][col] = ps[1];
-            yr[i][col] = py[0];
-            yi[i][col] = py[1];
-          }
-#pragma unroll
-        for (int ia = 0; ia < 3; ++ia)
-#pragma unroll
-          for (int ib = 0; ib < 3; ++ib) {
-            float re = 0.f, im = 0.f;
-#pragma unroll
-            for (int col = 0; col < 3; ++col) {
-              re += sr[ia][col] * yr[ib][col] + si[ia][col] * yi[ib][col];
-              im += si[ia][col] * yr[ib][col] - sr[ia][col] * yi[ib][col];
-            }
-#pragma unroll
-            for (int ww = 0; ww < 5; ++ww) {
-              acc[ww][ia * 3 + ib][0] += w[ww] * (double)re;
-              acc[ww][ia * 3 + ib][1] += w[ww] * (double)im;
-            }
-          }
-      }
-    }
-    double* out = part + ((size_t)ip * nwg + blockIdx.x) * kXiLenM;
-#pragma unroll
-    for (int ww = 0; ww < 5; ++ww)
-#pragma unroll
-      for (int ia = 0; ia < 3; ++ia)
-#pragma unroll
-        for (int ib = 0; ib < 3; ++ib) {
-          const size_t o = ((((size_t)f * 8 + p) * 5 + ww) * 144 + (a0 + ia) * 12 + (b0 + ib)) * 2;
-          out[o] = acc[ww][ia * 3 + ib][0];
-          out[o + 1] = acc[ww][ia * 3 + ib][1];
-        }
-  }
-}
-
-// xi[pair][i] = Kahan sum over workgroups (fixed order) of part[pair][wg][i]
-__global__ void reduce_window1(const double* __restrict__ part, int nwg, int len,
-                               double* __restrict__ xi) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ip = blockIdx.y;
-  if (i >= len) return;
-  Kahan k;
-  const double* p = part + (size_t)ip * nwg * len + i;
-  for (int w = 0; w < nwg; ++w) k.add(p[(size_t)w * len]);
-  xi[(size_t)ip * len + i] = k.s;
-}
 
 // ------------------------------------------------------------ coefficients --
 // The field F_yv[rel site][16][40] (FP32) for each distinct yv, gsx = 0. A site
@@ -369,15 +261,8 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
   const int np = w.n_pairs, nu = w.n_unique;
   // pass 1
   if (stage == 0) {
-    const int nwg = c->n_cu;
-    const size_t spw = div_up(div_up(V, (size_t)nwg), kM1Sites) * kM1Sites;
-    const int n = (int)div_up(V, spw);
-    hipLaunchKernelGGL(pass1_mixed, dim3(n), dim3(kM1Threads), 0, st, w.d_pairs.p, np, g, g,
-                       c->proc, spw, n, w.part1.p);
-    check_hip(hipGetLastError(), "pass1_mixed launch");
-    hipLaunchKernelGGL(reduce_window1, dim3((unsigned)div_up(kXiLenM, 256), np), dim3(256), 0, st,
-                       w.part1.p, n, kXiLenM, w.xi.p);
-    check_hip(hipGetLastError(), "reduce_window1 launch");
+    launch_pass1_planes<float>(w.d_pairs.p, WinPair{}, np, g, g, c->proc, 3, w.part1.p, w.red1.p,
+                               w.xi.p, st);
     launch_forms_batched(w.xi.p, w.dz.p, w.gdz.p, nullptr, w.bmat.p, np, st);
   }
   // coefficient fields, one per distinct yv
@@ -410,9 +295,6 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
   }
 }
 
-size_t window_part1_len(const hlbl_contractor* c, int n_pairs) {
-  return (size_t)n_pairs * c->n_cu * kXiLenM;
-}
 size_t window_part2_len(const hlbl_contractor* c, int n_pairs) {
   return (size_t)n_pairs * 2 * c->n_cu * kM2Waves;
 }

@@ -10,6 +10,7 @@
 #include "../../include/hlbl_amd.h"
 #include "context.hpp"
 #include "contract_kernels.hpp"
+#include "pass1.hpp"
 #include "window.hpp"
 
 namespace hlbl {
@@ -101,7 +102,8 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
     p->h_pairs.ensure(n);
     p->d_yvs.ensure(nu);
     p->h_yvs.ensure(nu);
-    p->part1.ensure(window_part1_len(c, n));
+    p->part1.ensure(pass1_part_len(c->global, n));
+    p->red1.ensure(pass1_red_len(c->global, n));
     p->xi.ensure((size_t)n * 2 * 8 * 5 * 144 * 2);
     p->dz.ensure((size_t)n * 2 * 10 * 144 * 2);
     p->gdz.ensure((size_t)n * 2 * 40 * 144 * 2);
