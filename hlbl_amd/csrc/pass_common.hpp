@@ -23,9 +23,9 @@ HLBL_HD void mul_ipow(R& re, R& im, int q) {
 //                = - sum_s eps_s i^{-ph_mu(s)} Xi[cls(mu)][s],
 //   Xi[cls][s] = sum_colour conj(S_a[(s', colour)]) Y_r[(s, colour)],
 // s' = s^2 (cls 0: gamma_0, gamma_3) or 3-s (cls 1: gamma_1, gamma_2).
-// out[mu * stride] receives (re, im).
-template <class R, class A>
-__device__ inline void dxu_site(const R* __restrict__ sa, const R* __restrict__ ur, R* out,
+// out[mu * stride] receives (re, im). Inputs R, arithmetic A, output O.
+template <class R, class A, class O = R>
+__device__ inline void dxu_site(const R* __restrict__ sa, const R* __restrict__ ur, O* out,
                                 int stride) {
   A xr[2][4], xim[2][4];
 #pragma unroll
@@ -56,8 +56,8 @@ __device__ inline void dxu_site(const R* __restrict__ sa, const R* __restrict__ 
       re += eps * a_r;
       im += eps * a_i;
     }
-    out[mu * stride] = (R)(-re);
-    out[mu * stride + 1] = (R)(-im);
+    out[mu * stride] = (O)(-re);
+    out[mu * stride + 1] = (O)(-im);
   }
 }
 

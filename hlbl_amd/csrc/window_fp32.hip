@@ -141,24 +141,77 @@ __global__ __launch_bounds__(kKmThreads) void judge_coef(
 }
 
 // ---------------------------------------------------------------- pass 2 --
-// As pass2_fp64 on FP32 MFMA: 8 waves = (flavour, quarter of K), three
-// 16-column blocks each. Per lane and site the 12 weighted products are summed
-// in FP32 and added to an FP64 Kahan sum. Persistent over the pairs.
-constexpr int kM2Threads = 512;
-constexpr int kM2Waves = 8;
-constexpr int kM2KSteps = 18;
-constexpr int kM2Sites = 4;
+// Per site and flavour f the traces
+//   C[(mu lam)][col] = Re sum_{r,i} q^{mu lam}_r[i] X_col[i][r],  q^{mu lam}_r = gamma_lam gamma_5 dxu_mu[r],
+// (cuda_lattice.cu:324-344, 403-476) as a 16 x 48 x 288 product on FP32 MFMA,
+// A = q (from dxu in LDS, gamma_lam gamma_5 as a per-lane signed permutation), B =
+// the pair's X (registers), then the kernel sum += sum coef * C. Waves: (flavour,
+// quarter of K); the sum over K quarters is linear in the coefficients, so every
+// wave dots its own partial C with them. Software pipeline over groups of 4
+// sites, one barrier per group: in step g the waves run the MFMAs of group g,
+// build dxu of group g+1 (second dxu buffer) and stream the propagators of group
+// g+2 into a staging buffer with direct global-to-LDS loads.
+constexpr int kW2Threads = 512;
+constexpr int kW2Waves = 8;
+constexpr int kW2Sites = 4;
+constexpr int kW2KSteps = 18;             // k-steps (of 4) per wave: a quarter of 288
+constexpr int kW2RowLen = kW2Sites * 24;  // doubles of one staged spinor row (4 sites)
+constexpr int kW2RowPad = kW2RowLen + 2;  // row stride in LDS: rows start 4 banks apart
+constexpr int kW2BufLen = 48 * kW2RowPad; // doubles per staging buffer: (S/Y, f, a) rows
+constexpr int kW2DxuLen = kW2Sites * 2 * 4 * 288;  // floats per dxu buffer [site][f][mu][r][24]
 
-__global__ __launch_bounds__(kM2Threads) void pass2_mixed(
+// barrier that waits for this wave's LDS operations and, with vm, for its
+// global loads (direct loads into LDS included)
+__device__ inline void barrier_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n s_barrier" ::: "memory"); }
+__device__ inline void barrier_all() { asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n s_barrier" ::: "memory"); }
+
+// Issue the direct loads of sites [z0, z0 + 4) of pair wp into buf (FP64, rows
+// (S/Y, f, a) of 96 doubles at stride kW2RowPad): one load of 48 lanes x 16 bytes
+// per row. Sites at or beyond V are read from site V-1; their coefficients are
+// masked.
+__device__ inline void w2_issue(const WinPair& wp, size_t V, size_t z0, double* buf) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane >= kW2RowLen / 2) return;
+  const int within = 2 * lane;
+  size_t z = z0 + within / 24;
+  z = z < V ? z : V - 1;
+  for (int row = wave; row < 48; row += kW2Waves) {
+    const int bl = row / 24, ff = (row / 12) & 1, a = row % 12;
+    const double* src = (bl ? wp.Y : wp.S) + (((size_t)(ff * 12 + a)) * V + z) * 24 + within % 24;
+    __builtin_amdgcn_global_load_lds((const void*)src,
+                                     (__attribute__((address_space(3))) void*)(buf + row * kW2RowPad), 16, 0, 0);
+  }
+}
+
+// dxu of both flavours of a staged group: task (site, f2, r, a)
+__device__ inline void w2_dxu(const double* buf, float* dxu) {
+  for (int t = threadIdx.x; t < kW2Sites * 288; t += kW2Threads) {
+    const int zi = t / 288, f2 = (t / 144) & 1, r = (t / 12) % 12, a = t % 12;
+    const double* sa = buf + ((1 - f2) * 12 + a) * kW2RowPad + zi * 24;
+    const double* ur = buf + (24 + f2 * 12 + r) * kW2RowPad + zi * 24;
+    dxu_site<double, float, float>(sa, ur, dxu + ((zi * 2 + f2) * 4) * 288 + r * 24 + 2 * a, 288);
+  }
+}
+
+__global__ __launch_bounds__(kW2Threads) void pass2_window(
     const WinPair* __restrict__ pairs, int n_pairs, const float* __restrict__ bmat,
     const float* __restrict__ coef, hlbl_geom global, size_t sites_per_wg, int nwg,
     double* __restrict__ part) {
-  __shared__ float sSY[kM2Sites][2][2][12][24];
-  __shared__ float sDxu[kM2Sites][2][4][12][24];
+  __shared__ __attribute__((aligned(16))) double sBuf[2][kW2BufLen];
+  __shared__ float sDxu[2][kW2DxuLen];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int f = wave / 4, kq = wave & 3;
+  const int f = wave >> 2, kq = wave & 3;
   const size_t V = geom_volume(global);
+  const size_t begin = (size_t)blockIdx.x * sites_per_wg;
+  const size_t end = begin + sites_per_wg < V ? begin + sites_per_wg : V;
+  if (begin >= end) return;
+  const int ng = (int)((end - begin + kW2Sites - 1) / kW2Sites);
+  const int total = ng * n_pairs;
+  auto group_pair = [&](int it) { return it / ng; };
+  auto group_z0 = [&](int it) { return begin + (size_t)(it - (it / ng) * ng) * kW2Sites; };
+
+  // A fragments: row (mu, lam) = lane%16, K index r*24 + j, j = 4*jq + lane/16
   const int arow = lane & 15, amu = arow >> 2, alam = arow & 3;
   int aoff[6];
   float asg[6];
@@ -169,69 +222,91 @@ __global__ __launch_bounds__(kM2Threads) void pass2_mixed(
     aoff[jq] = pj;
     asg[jq] = (float)(gamma_sign24(4, pj) * gamma_sign24(alam, j));
   }
-  const size_t begin = (size_t)blockIdx.x * sites_per_wg;
-  const size_t end = begin + sites_per_wg < V ? begin + sites_per_wg : V;
+  const bool c2on = (lane & 15) < kCoefUsed - 32;
 
-  for (int ip = 0; ip < n_pairs; ++ip) {
+  // prologue: group 0 staged and its dxu built, group 1 in flight
+  w2_issue(pairs[0], V, begin, sBuf[0]);
+  barrier_all();
+  w2_dxu(sBuf[0], sDxu[0]);
+  if (total > 1) w2_issue(pairs[group_pair(1)], V, group_z0(1), sBuf[1]);
+
+  float breg[3][kW2KSteps];
+  Kahan acc;
+  for (int it = 0; it < total; ++it) {
+    const int ip = group_pair(it);
+    const size_t z0 = group_z0(it);
     const WinPair wp = pairs[ip];
-    const float* bm = bmat + (size_t)ip * 2 * 288 * kCoefCols;
-    float breg[3][kM2KSteps];
+    if (z0 == begin) {
+      const float* bm = bmat + (size_t)ip * 2 * 288 * kCoefCols;
 #pragma unroll
-    for (int cb = 0; cb < 3; ++cb)
+      for (int cb = 0; cb < 3; ++cb)
 #pragma unroll
-      for (int i = 0; i < kM2KSteps; ++i)
-        breg[cb][i] = bm[((size_t)f * 288 + 4 * (kq * kM2KSteps + i) + (lane >> 4)) * kCoefCols +
-                         16 * cb + (lane & 15)];
-    const float* cu = coef + (size_t)wp.yv_slot * V * kCoefField;
-    Kahan acc;
-    for (size_t z0 = begin; z0 < end; z0 += kM2Sites) {
-      const int ns = (end - z0) < (size_t)kM2Sites ? (int)(end - z0) : kM2Sites;
-      __syncthreads();
-      for (int i = tid; i < kM2Sites * 1152; i += kM2Threads) {
-        const int zi = i / 1152, rr = i % 1152;
-        const int bl = rr / 576, rem = rr % 576, ff = rem / 288, a = (rem / 24) % 12, c = rem % 24;
-        float v = 0.f;
-        if (zi < ns) v = (float)(bl ? wp.Y : wp.S)[(((size_t)(ff * 12 + a)) * V + z0 + zi) * 24 + c];
-        sSY[zi][bl][ff][a][c] = v;
-      }
-      __syncthreads();
-      for (int t = tid; t < kM2Sites * 288; t += kM2Threads) {
-        const int zi = t / 288, f2 = (t / 144) % 2, r = (t / 12) % 12, a = t % 12;
-        dxu_site<float, float>(sSY[zi][0][1 - f2][a], sSY[zi][1][f2][r], &sDxu[zi][f2][0][r][2 * a], 12 * 24);
-      }
-      __syncthreads();
-      for (int zi = 0; zi < ns; ++zi) {
-        const float* dx = &sDxu[zi][f][amu][0][0];
-        f4v c0 = {0.f, 0.f, 0.f, 0.f}, c1 = c0, c2 = c0;
-#pragma unroll
-        for (int i = 0; i < kM2KSteps; ++i) {
-          const int st = kq * kM2KSteps + i;
-          const int r = st / 6, jq = i % 6;
-          const float a = asg[jq] * dx[r * 24 + aoff[jq]];
-          c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[0][i], c0, 0, 0, 0);
-          c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[1][i], c1, 0, 0, 0);
-          c2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[2][i], c2, 0, 0, 0);
-        }
-        // the coefficient field at the site relative to gsx
-        int crd[4], rel[4];
-        site_coords(z0 + zi, global, crd);
-        for (int r = 0; r < 4; ++r) rel[r] = mod_pos(crd[r] - coord_dim(wp.gsx, r), geom_dim(global, r));
-        const size_t rs = (((size_t)rel[0] * global.LX + rel[1]) * global.LY + rel[2]) * global.LZ + rel[3];
-        const float* cs = cu + rs * kCoefField + (4 * (lane >> 4)) * kCoefUsed + (lane & 15);
-        float v = 0.f;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          v += c0[i] * cs[i * kCoefUsed] + c1[i] * cs[i * kCoefUsed + 16];
-          if ((lane & 15) < kCoefUsed - 32) v += c2[i] * cs[i * kCoefUsed + 32];
-        }
-        acc.add((double)v);
-      }
+        for (int i = 0; i < kW2KSteps; ++i)
+          breg[cb][i] = bm[((size_t)f * 288 + 4 * (kq * kW2KSteps + i) + (lane >> 4)) * kCoefCols +
+                           16 * cb + (lane & 15)];
+      acc = Kahan();
     }
-    // wave sum (pairwise, FP64), one partial per wave
-    double s = acc.s - acc.c;
+    // dxu of group it ready, staging of group it+1 landed, buffers of it-1 free
+    barrier_all();
+    if (it + 2 < total) w2_issue(pairs[group_pair(it + 2)], V, group_z0(it + 2), sBuf[it & 1]);
+    if (it + 1 < total) w2_dxu(sBuf[(it + 1) & 1], sDxu[(it + 1) & 1]);
+    // MFMA over the group's sites
+    const float* dg = sDxu[it & 1];
+    // coordinates of z0, stepped along the site order; relative to gsx (inside
+    // the lattice) for the coefficient field
+    int crd[4];
+    site_coords(z0, global, crd);
+    float v = 0.f;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if (lane == 0) part[((size_t)ip * nwg + blockIdx.x) * kM2Waves + wave] = s;
+    for (int zi = 0; zi < kW2Sites; ++zi) {
+      const size_t z = z0 + zi;
+      if (z >= end) break;
+      if (zi > 0 && ++crd[3] == global.LZ) {
+        crd[3] = 0;
+        if (++crd[2] == global.LY) {
+          crd[2] = 0;
+          if (++crd[1] == global.LX) {
+            crd[1] = 0;
+            ++crd[0];
+          }
+        }
+      }
+      int rel[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        rel[r] = crd[r] - coord_dim(wp.gsx, r);
+        rel[r] += rel[r] < 0 ? geom_dim(global, r) : 0;
+      }
+      const size_t rs = (((size_t)rel[0] * global.LX + rel[1]) * global.LY + rel[2]) * global.LZ + rel[3];
+      // FP32 16x16x4 result: element i of lane l is row 4*(l/16) + i, column l%16
+      const float* cs = coef + ((size_t)wp.yv_slot * V + rs) * kCoefField + (4 * (lane >> 4)) * kCoefUsed + (lane & 15);
+      float cv[4][3];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        cv[i][0] = cs[i * kCoefUsed];
+        cv[i][1] = cs[i * kCoefUsed + 16];
+        cv[i][2] = c2on ? cs[i * kCoefUsed + 32] : 0.f;
+      }
+      const float* dx = dg + ((zi * 2 + f) * 4 + amu) * 288;
+      f4v c0 = {0.f, 0.f, 0.f, 0.f}, c1 = c0, c2 = c0;
+#pragma unroll
+      for (int i = 0; i < kW2KSteps; ++i) {
+        const int st = kq * kW2KSteps + i;  // kq*18 is a multiple of 6: jq = i % 6
+        const float a = asg[i % 6] * dx[(st / 6) * 24 + aoff[i % 6]];
+        c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[0][i], c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[1][i], c1, 0, 0, 0);
+        c2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[2][i], c2, 0, 0, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v += c0[i] * cv[i][0] + c1[i] * cv[i][1] + c2[i] * cv[i][2];
+    }
+    acc.add((double)v);
+    if (z0 + kW2Sites >= end) {
+      double s = acc.s - acc.c;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+      if (lane == 0) part[((size_t)ip * nwg + blockIdx.x) * kW2Waves + wave] = s;
+    }
   }
 }
 
@@ -243,7 +318,7 @@ __global__ void reduce_window2(const double* __restrict__ part, int nwg, int n_p
   const int ip = i / 2, f = i % 2;
   Kahan k;
   for (int w = 0; w < nwg; ++w)
-    for (int j = 0; j < kM2Waves / 2; ++j) k.add(part[((size_t)ip * nwg + w) * kM2Waves + f * (kM2Waves / 2) + j]);
+    for (int j = 0; j < kW2Waves / 2; ++j) k.add(part[((size_t)ip * nwg + w) * kW2Waves + f * (kW2Waves / 2) + j]);
   out[i] = k.s;
 }
 
@@ -283,12 +358,12 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
   }
   // pass 2
   if (stage == 2) {
-    const int nwg = 2 * c->n_cu;
-    const size_t spw = div_up(div_up(V, (size_t)nwg), kM2Sites) * kM2Sites;
+    const int nwg = c->n_cu;
+    const size_t spw = div_up(div_up(V, (size_t)nwg), kW2Sites) * kW2Sites;
     const int n = (int)div_up(V, spw);
-    hipLaunchKernelGGL(pass2_mixed, dim3(n), dim3(kM2Threads), 0, st, w.d_pairs.p, np, w.bmat.p,
+    hipLaunchKernelGGL(pass2_window, dim3(n), dim3(kW2Threads), 0, st, w.d_pairs.p, np, w.bmat.p,
                        w.coef.p, g, spw, n, w.part2.p);
-    check_hip(hipGetLastError(), "pass2_mixed launch");
+    check_hip(hipGetLastError(), "pass2_window launch");
     hipLaunchKernelGGL(reduce_window2, dim3((unsigned)div_up(2 * np, 64)), dim3(64), 0, st,
                        w.part2.p, n, np, w.result.p);
     check_hip(hipGetLastError(), "reduce_window2 launch");
@@ -296,7 +371,8 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
 }
 
 size_t window_part2_len(const hlbl_contractor* c, int n_pairs) {
-  return (size_t)n_pairs * 2 * c->n_cu * kM2Waves;
+  return (size_t)n_pairs * c->n_cu * kW2Waves;
 }
+
 
 }  // namespace hlbl

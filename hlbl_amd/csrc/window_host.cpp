@@ -124,8 +124,12 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
     cache->plans.push_back(std::move(p));
     w = cache->plans.back().get();
   }
-  for (int i = 0; i < n; ++i)
-    w->h_pairs.p[i] = WinPair{pairs[i].d_x, pairs[i].d_y, pairs[i].gsx, pairs[i].yv, slot[i], 0};
+  for (int i = 0; i < n; ++i) {
+    // gsx inside the lattice: the kernels step site coordinates relative to it
+    const hlbl_coord gs = {mod_pos(pairs[i].gsx.t, g.T), mod_pos(pairs[i].gsx.x, g.LX),
+                           mod_pos(pairs[i].gsx.y, g.LY), mod_pos(pairs[i].gsx.z, g.LZ)};
+    w->h_pairs.p[i] = WinPair{pairs[i].d_x, pairs[i].d_y, gs, pairs[i].yv, slot[i], 0};
+  }
   for (int u = 0; u < nu; ++u) w->h_yvs.p[u] = yvs[u];
   if (!cache->stream) check_hip(hipStreamCreateWithFlags(&cache->stream, hipStreamNonBlocking), "hipStreamCreate");
   return w;
