@@ -2,13 +2,15 @@
 
 Workload (one step = one window): the sliding-hypercube chain K=8, stride n=3,
 h=(1,1,1,1); window 1 = 65 pairs x 2 flavours over 30 distinct propagator blocks
-(3.06 GB each, FP64, resident in HBM), random fields from seed 2
-(RandomProvider, generated on the device). The timed form is the surrogate +
-FP32 window-batched path (one hipGraph per window); the plain FP64 per-pair path
-is timed on the same fields for the ratio, and every pair of the window is
-compared to FP64 (max relative error reported).
+(3.06 GB each, FP64, resident in HBM), random fields from seed 2 (RandomProvider,
+generated on the device). Two window-batched forms run as one hipGraph per
+window: FP32 products with the plain QED kernel (form 1), and the same with the
+MLP surrogate's channels (form 2, network trained first over the lattice's
+reach, seed 4). The plain FP64 per-pair path runs on the same fields as the
+reference; every pair sum of both forms is compared to it. The headline value is
+the faster form whose largest per-pair relative error is within 1e-3.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--epochs E]
 """
 import argparse
 import ctypes
@@ -62,6 +64,21 @@ def cpu_baseline(seconds_target=12.0):
                       f"scaled by sites to 24^3x48"}
 
 
+def time_form(con, descs, form, warmup, steps):
+    """seconds per window (host clock around synchronised graph launches) and the
+    last result"""
+    import torch
+    res = None
+    for _ in range(warmup):
+        res = con.window_kernel_sum(descs, XUNIT, form=form)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(steps):
+        res = con.window_kernel_sum(descs, XUNIT, form=form)
+    torch.cuda.synchronize()
+    return (time.time() - t0) / steps, res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,7 +100,9 @@ def main():
     qed = H.QedKernel(0)
     con = H.Contractor(lat, qed)
 
-    # surrogate: trained on the GPU by the library's own kernels (not timed)
+    # surrogate: trained on the GPU by the library's own kernels (not timed), over
+    # the distances this lattice reaches (outside: the table)
+    qed.mlp_set_domain(lat.reach(XUNIT))
     t0 = time.time()
     loss = qed.mlp_train(n_samples=1 << 20, epochs=args.epochs, batch=1024, lr=2e-3, seed=4)
     t_train = time.time() - t0
@@ -91,12 +110,9 @@ def main():
 
     # window 1 of the chain: its 65 pairs and 30 blocks
     sch = H.Schedule((1, 1, 1, 1), 3, 8)
-    src = sch.sources()
-    win = 1
-    pairs_w = [(q, p) for w, p, parts in src if w == win for q in parts]
+    pairs_w = [(q, p) for w, p, parts in sch.sources() if w == 1 for q in parts]
     assert len(pairs_w) == 65
     positions = sorted({q for q, p in pairs_w} | {p for q, p in pairs_w})
-    assert len(positions) == 30
     wrap = lambda v: tuple(a % L for a, L in zip(v, DIMS))
     nblk = H.block_length(lat)
     blocks = {}
@@ -110,66 +126,63 @@ def main():
     descs = [(blocks[q].data_ptr(), blocks[p].data_ptr(), wrap(q), lat.site_map_diff(p, q))
              for q, p in pairs_w]
 
-    # timed: surrogate + FP32 window form
-    for _ in range(args.warmup):
-        res = con.window_kernel_sum(descs, XUNIT, form=2)
-    torch.cuda.synchronize()
-    t0 = time.time()
-    for _ in range(args.steps):
-        res = con.window_kernel_sum(descs, XUNIT, form=2)
-    torch.cuda.synchronize()
-    dt = (time.time() - t0) / args.steps
-    value = 65.0 / dt
+    # the two window forms, then their stage times (HIP events on the window stream)
+    dt1, res1 = time_form(con, descs, 1, args.warmup, args.steps)
+    dt2, res2 = time_form(con, descs, 2, args.warmup, args.steps)
     judge, nsamp = con.judge()
-    log(f"form 2: {dt * 1e3:.1f} ms per window, {value:.2f} pairs/s, judge {judge:.2e} ({nsamp} sites)")
-
-    # FP32 + plain kernel (form 1) and its stage times
-    t0 = time.time()
-    res1 = con.window_kernel_sum(descs, XUNIT, form=1)
-    res1 = con.window_kernel_sum(descs, XUNIT, form=1)
-    dt1 = 0
-    for _ in range(2):
-        t0 = time.time()
-        res1 = con.window_kernel_sum(descs, XUNIT, form=1)
-        dt1 += (time.time() - t0) / 2
     _, ms1 = con.window_profile(descs, XUNIT, form=1)
     _, ms2 = con.window_profile(descs, XUNIT, form=2)
-    log(f"form 1: {dt1 * 1e3:.1f} ms per window; stages form1 {ms1} form2 {ms2} ms")
+    log(f"form 1: {dt1 * 1e3:.1f} ms per window ({65 / dt1:.2f} pairs/s), stages {ms1} ms")
+    log(f"form 2: {dt2 * 1e3:.1f} ms per window ({65 / dt2:.2f} pairs/s), stages {ms2} ms, "
+        f"judge {judge:.2e} ({nsamp} sites)")
 
-    # plain FP64, one pair per call, over the whole window (reference for the error)
+    # plain FP64, one pair per call, over the whole window (the reference sums)
     ref = []
+    torch.cuda.synchronize()
     t0 = time.time()
     for d in descs:
         ref.append(con.kernel_sum(*d, XUNIT))
     dt64 = (time.time() - t0) / len(descs)
     ref = np.array(ref)
-    err2 = np.abs(np.array(res) - ref) / np.abs(ref)
     err1 = np.abs(np.array(res1) - ref) / np.abs(ref)
-    log(f"plain FP64: {dt64 * 1e3:.1f} ms per pair ({1 / dt64:.2f} pairs/s); max rel err form2 "
-        f"{err2.max():.2e} form1 {err1.max():.2e}")
+    err2 = np.abs(np.array(res2) - ref) / np.abs(ref)
+    scale = np.sqrt(np.mean(ref ** 2))
+    log(f"plain FP64: {dt64 * 1e3:.1f} ms per pair ({1 / dt64:.2f} pairs/s); max rel err form1 "
+        f"{err1.max():.2e} form2 {err2.max():.2e} (median {np.median(err2):.2e}, "
+        f"max |diff| / rms pair sum {np.max(np.abs(np.array(res2) - ref)) / scale:.2e})")
 
-    # roofline of the dominant kernel (pass 2 traces): algorithmic flops per launch
+    # headline: the faster form within the 1e-3 error bound
+    forms = {1: (dt1, err1.max(), ms1), 2: (dt2, err2.max(), ms2)}
+    ok = [f for f in forms if forms[f][1] <= 1e-3]
+    head = min(ok, key=lambda f: forms[f][0]) if ok else 1
+    dt, err, ms = forms[head]
+    value = 65.0 / dt
+    # roofline of the dominant kernel, pass 2: algorithmic flops per launch
+    # (65 pairs x 2 flavours x (dxu products 8*6912 + 640 traces 4*144*640) per site)
     V = lat.volume()
-    flops_pass2 = 65 * 2 * (8 * 6912 + 4 * 144 * 640) * V  # dxu products + 640 traces, both flavours
-    achieved = flops_pass2 / (ms2[3] * 1e-3)
+    flops_pass2 = 65 * 2 * (8 * 6912 + 4 * 144 * 640) * V
+    achieved = flops_pass2 / (ms[3] * 1e-3)
     cpu = None if args.no_cpu else cpu_baseline()
+    names = {1: "FP32 window-batched, plain QED kernel", 2: "FP32 window-batched, MLP surrogate channels"}
     line = {
         "metric": "pairs/sec at 24^3x48 on 1 GPU, surrogate+FP32 vs plain FP64, rel err <=1e-3",
         "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "24^3x48 window 1 of chain K=8 n=3 h=(1,1,1,1): 65 pairs x 2 flavours, "
-                               "30 resident FP64 blocks, seed 2; surrogate + FP32 window-batched form",
-                   "lattice": list(DIMS), "pairs_per_step": 65, "xunit": XUNIT},
+                               f"30 resident FP64 blocks, seed 2; {names[head]}",
+                   "lattice": list(DIMS), "pairs_per_step": 65, "xunit": XUNIT, "form": head},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": MI355X_FP32_MFMA / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / MI355X_FP32_MFMA, "traffic": None,
-                     "kernel": "pass2_mixed", "ms": ms2[3]},
+                     "kernel": "pass2_window", "ms": ms[3]},
+        "max_pair_rel_err": float(err),
         "plain_fp64_pairs_per_s": 1.0 / dt64,
-        "fp32_plain_kernel_pairs_per_s": 65.0 / dt1,
         "speedup_vs_plain_fp64": value * dt64,
-        "max_pair_rel_err": float(err2.max()), "max_pair_rel_err_fp32_plain": float(err1.max()),
+        "form1_pairs_per_s": 65.0 / dt1, "form1_max_pair_rel_err": float(err1.max()),
+        "form2_pairs_per_s": 65.0 / dt2, "form2_max_pair_rel_err": float(err2.max()),
+        "form2_median_pair_rel_err": float(np.median(err2)),
         "surrogate_judge_max_rel": judge, "surrogate_train_s": t_train,
-        "stage_ms_form2": ms2, "stage_ms_form1": ms1,
+        "stage_ms_form1": ms1, "stage_ms_form2": ms2,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

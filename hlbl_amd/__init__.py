@@ -14,6 +14,7 @@ Device buffers are passed as integer addresses (e.g. torch.Tensor.data_ptr()).
 The library must be built (hlbl_amd.build); there is no CPU fallback.
 """
 import ctypes as C
+import math
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -73,6 +74,7 @@ _SIGS = {
     "hlbl_mlp_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64,
                                  C.c_void_p, C.c_void_p]),
     "hlbl_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "hlbl_mlp_set_domain": (C.c_int, [C.c_void_p, C.c_double]),
     "hlbl_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "hlbl_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                    C.c_void_p]),
@@ -156,6 +158,12 @@ class Lattice:
         t, x, y, z = c
         return ((t * self.local[1] + x) * self.local[2] + y) * self.local[3] + z
 
+    def reach(self, xunit):
+        """Largest distance (muon units) between two sites with minimum-image
+        coordinates: xunit * |(T/2, LX/2, LY/2, LZ/2)|. Every argument of the
+        QED kernel on this lattice lies within it."""
+        return xunit * math.sqrt(sum((L / 2) ** 2 for L in self.global_))
+
     def site_map_diff(self, a, b):
         """Minimum-image a - b in [-L/2, L/2) (include/hlbl/vec4.hpp:40-44)."""
         out = []
@@ -237,6 +245,11 @@ class QedKernel:
         check(lib().hlbl_mlp_train(self.handle, n_samples, epochs, batch, lr, C.c_uint64(seed),
                                    loss, stream))
         return list(loss)
+
+    def mlp_set_domain(self, radius):
+        """Train (and use) the surrogate on |x|, |y| <= radius (muon units); outside,
+        the table's channels are used. Discards the trained state."""
+        check(lib().hlbl_mlp_set_domain(self.handle, float(radius)))
 
     def mlp_get_params(self):
         n = self.mlp_param_count()

@@ -483,7 +483,6 @@ int hlbl_mlp_init(hlbl_qed* q, uint64_t seed) {
   return guarded([&] {
     check_ptr(q, "qed");
     check_hip(hipSetDevice(q->device), "hipSetDevice");
-    mlp_configure_kernels();
     mlp_init_params(q, seed);
     q->mlp_ready = false;
   });
@@ -496,8 +495,17 @@ int hlbl_mlp_train(hlbl_qed* q, int n_samples, int epochs, int batch, float lr, 
     if (n_samples <= 0 || epochs <= 0 || batch <= 0 || batch > n_samples)
       throw Error("hlbl_mlp_train: need 0 < batch <= n_samples and epochs > 0");
     check_hip(hipSetDevice(q->device), "hipSetDevice");
-    mlp_configure_kernels();
     mlp_train(q, n_samples, epochs, batch, lr, seed, loss, as_stream(stream));
+  });
+}
+
+int hlbl_mlp_set_domain(hlbl_qed* q, double radius) {
+  return guarded([&] {
+    check_ptr(q, "qed");
+    if (!(radius > 0.0) || radius > qed::kRadius * (1.0 + 1e-12))
+      throw Error("hlbl_mlp_set_domain: need 0 < radius <= the kernel radius");
+    q->mlp_radius = radius;
+    q->mlp_ready = false;
   });
 }
 
@@ -534,7 +542,6 @@ int hlbl_mlp_forward(hlbl_qed* q, const float* d_in, int n, float* d_out, int pr
     if (n == 0) return;
     check_ptr(d_in, "d_in");
     check_ptr(d_out, "d_out");
-    mlp_configure_kernels();
     if (precision == 0) mlp_forward_fp32(q, d_in, n, d_out, as_stream(stream));
     else mlp_infer(q, d_in, n, d_out, as_stream(stream));
     check_hip(hipStreamSynchronize(as_stream(stream)), "mlp forward");

@@ -103,6 +103,7 @@ struct hlbl_qed {
   hlbl::DevBuf<unsigned short> mlp_w_bf16;
   hlbl::DevBuf<float> mlp_scale;  // [2][kMlpOut]: offset, scale (normalised -> physical)
   bool mlp_ready = false;
+  double mlp_radius = hlbl::qed::kRadius;  // trained domain: |x|, |y| <= mlp_radius (muon units)
 };
 
 namespace hlbl {

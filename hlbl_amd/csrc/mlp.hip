@@ -111,9 +111,11 @@ __global__ void k_round_bf16(const float* __restrict__ w, float* __restrict__ wq
   if (w16) w16[i] = (unsigned short)(u >> 16);
 }
 
-__global__ void k_gelu(const float* __restrict__ z, float* __restrict__ a, size_t n) {
+// a = gelu(z), rounded to BF16 when round_act (the activations of the BF16
+// inference; the backward pass treats the rounding as identity)
+__global__ void k_gelu(const float* __restrict__ z, float* __restrict__ a, size_t n, int round_act) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = gelu(z[i]);
+  if (i < n) a[i] = round_act ? bf16_round(gelu(z[i])) : gelu(z[i]);
 }
 // dz = da * gelu'(z)
 __global__ void k_gelu_back(const float* __restrict__ z, const float* __restrict__ da,
@@ -171,8 +173,10 @@ struct MlpWork {
   DevBuf<double> row_loss;
 };
 
-// forward of B rows (x: [B][3]) with weights wq (BF16-rounded floats); keeps z/a
-void mlp_forward_train(const float* wq, const float* x, int B, MlpWork& wk, hipStream_t st) {
+// forward of B rows (x: [B][3]) with weights wq (BF16-rounded floats); keeps z/a.
+// round_act: hidden activations rounded to BF16, as the inference kernel does.
+void mlp_forward_train(const float* wq, const float* x, int B, MlpWork& wk, hipStream_t st,
+                       bool round_act) {
   const MlpLayout L = mlp_layout();
   const float* in = x;
   for (int l = 0; l < 5; ++l) {
@@ -183,7 +187,8 @@ void mlp_forward_train(const float* wq, const float* x, int B, MlpWork& wk, hipS
     if (l < 4) {
       wk.a[l].ensure((size_t)B * L.out[l]);
       const size_t n = (size_t)B * L.out[l];
-      hipLaunchKernelGGL(k_gelu, dim3((unsigned)div_up(n, 256)), dim3(256), 0, st, wk.z[l].p, wk.a[l].p, n);
+      hipLaunchKernelGGL(k_gelu, dim3((unsigned)div_up(n, 256)), dim3(256), 0, st, wk.z[l].p, wk.a[l].p, n,
+                         round_act ? 1 : 0);
       in = wk.a[l].p;
     }
   }
@@ -191,9 +196,9 @@ void mlp_forward_train(const float* wq, const float* x, int B, MlpWork& wk, hipS
 
 // loss and gradient (into grad, all parameters) of a batch
 double mlp_backward(const float* wq, const float* x, const float* target, int B, MlpWork& wk,
-                    float* grad, hipStream_t st, bool want_loss) {
+                    float* grad, hipStream_t st, bool want_loss, bool round_act) {
   const MlpLayout L = mlp_layout();
-  mlp_forward_train(wq, x, B, wk, st);
+  mlp_forward_train(wq, x, B, wk, st, round_act);
   wk.da.ensure((size_t)B * kMlpHidden);
   wk.dz.ensure((size_t)B * kMlpHidden);
   wk.dtmp.ensure((size_t)B * kMlpHidden);
@@ -229,7 +234,7 @@ double mlp_backward(const float* wq, const float* x, const float* target, int B,
 
 void mlp_forward_fp32(hlbl_qed* q, const float* x, int n, float* y, hipStream_t st) {
   MlpWork wk;
-  mlp_forward_train(q->mlp_w.p, x, n, wk, st);
+  mlp_forward_train(q->mlp_w.p, x, n, wk, st, false);
   check_hip(hipMemcpyAsync(y, wk.z[4].p, (size_t)n * kMlpOut * sizeof(float), hipMemcpyDeviceToDevice, st), "copy");
   check_hip(hipStreamSynchronize(st), "mlp forward");
 }
@@ -237,7 +242,7 @@ void mlp_forward_fp32(hlbl_qed* q, const float* x, int n, float* y, hipStream_t 
 double mlp_loss_grad(hlbl_qed* q, const float* x, const float* t, int n, float* grad, hipStream_t st) {
   MlpWork wk;
   mlp_requantise(q, st);
-  return mlp_backward(q->mlp_wq.p, x, t, n, wk, grad, st, true);
+  return mlp_backward(q->mlp_wq.p, x, t, n, wk, grad, st, true, false);
 }
 
 // ---------------------------------------------------- training-set and stats --
@@ -255,14 +260,18 @@ __host__ __device__ inline double mlp_uniform(uint64_t seed, uint64_t i, int k) 
   return (double)(hash64(seed * 0x9E3779B97F4A7C15ULL + i * 3 + k) >> 11) * (1.0 / 9007199254740992.0);
 }
 
-void mlp_sample_inputs(uint64_t seed, size_t i, double& a, double& c, double& b) {
-  a = qed::kRadius * mlp_uniform(seed, i, 0);
-  b = qed::kRadius * mlp_uniform(seed, i, 1);
+void mlp_sample_inputs(uint64_t seed, size_t i, double radius, double& a, double& c, double& b) {
+  a = radius * mlp_uniform(seed, i, 0);
+  b = radius * mlp_uniform(seed, i, 1);
   c = 2.0 * mlp_uniform(seed, i, 2) - 1.0;
 }
 
-// exact channels of n samples (host threads): x [n][3] normalised inputs, t [n][31] raw/envelope
-void mlp_build_set(uint64_t seed, int n, std::vector<float>& x, std::vector<double>& t) {
+// Training set of n samples (host threads): the channels of the plain evaluation
+// (the table, interpolated as the plain kernel does) at |x|, |y| uniform in
+// [0, radius], cos beta uniform in [-1, 1]; x [n][3] normalised inputs, t [n][31]
+// channels / envelope.
+void mlp_build_set(uint64_t seed, int n, double radius, const std::vector<double>& table,
+                   std::vector<float>& x, std::vector<double>& t) {
   x.resize((size_t)n * 3);
   t.resize((size_t)n * kMlpOut);
   int nt = (int)std::thread::hardware_concurrency();
@@ -274,11 +283,12 @@ void mlp_build_set(uint64_t seed, int n, std::vector<float>& x, std::vector<doub
       double ch[qed::kNCh];
       for (int i = th; i < n; i += nt) {
         double a, c, b;
-        mlp_sample_inputs(seed, (size_t)i, a, c, b);
-        qed::weight_channels(a * a, b * b, a * b * c, ch);
+        mlp_sample_inputs(seed, (size_t)i, radius, a, c, b);
+        const qed::Cell<double> cell = qed::locate(a, c, b);
+        for (int k = 0; k < kMlpOut; ++k) ch[k] = qed::interp(table.data(), cell, k);
         const double env = mlp_envelope(a, b, c);
-        x[(size_t)i * 3 + 0] = (float)(a / qed::kRadius);
-        x[(size_t)i * 3 + 1] = (float)(b / qed::kRadius);
+        x[(size_t)i * 3 + 0] = (float)(a / radius);
+        x[(size_t)i * 3 + 1] = (float)(b / radius);
         x[(size_t)i * 3 + 2] = (float)c;
         for (int k = 0; k < kMlpOut; ++k) t[(size_t)i * kMlpOut + k] = ch[k] / env;
       }
@@ -287,181 +297,235 @@ void mlp_build_set(uint64_t seed, int n, std::vector<float>& x, std::vector<doub
 }
 
 // ---------------------------------------------------------------- inference --
-// Persistent FP32-MFMA forward: all BF16 weights (as floats) in LDS would not fit,
-// so the weights stay in BF16 in LDS (108 KB) and the 64-row activation tile in
-// FP32 (32 KB). Rows are generated by a functor of the row index.
+// BF16 forward on MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation): weights and
+// hidden activations in BF16, the 3-input first layer in FP32 on the vector ALU.
+// A wave owns 64 rows (four 16-row tiles) and keeps their activations in LDS as
+// BF16 rows of kActStride (padded against bank conflicts); the weights of a layer
+// are read from global memory (L2-resident, 108 KB) into registers once per 64
+// rows as B fragments: 8 BF16 of W[col][k..k+7] per lane, col = lane%16 of the
+// column block, k = 8*(lane/16) of the k-step. A fragments are the same 8-wide
+// slices of an activation row, so A and B pair the same k.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f4m __attribute__((ext_vector_type(4)));
 
 constexpr int kInfThreads = 256;
-constexpr int kInfRows = 64;
+constexpr int kInfWaveRows = 64;
+constexpr int kInfRows = (kInfThreads / 64) * kInfWaveRows;  // rows per workgroup step
+constexpr int kActStride = kMlpHidden + 8;                   // BF16 per activation row
 
-__device__ inline float bf16_to_f(unsigned short h) {
-  unsigned u = (unsigned)h << 16;
+__device__ inline unsigned short f2bf(float x) {
+  unsigned u;
+  memcpy(&u, &x, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);  // nearest, ties to even (inputs are finite)
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float bf2f(unsigned short h) {
+  const unsigned u = (unsigned)h << 16;
   float r;
   memcpy(&r, &u, 4);
   return r;
 }
 
-// One 64-row tile through the network. act holds the inputs [64][128] (first 3
-// columns used), out receives the 31 outputs of row r via out_fn(r, c, v).
-template <class OutFn>
-__device__ void mlp_tile(const unsigned short* __restrict__ w16, float* act, OutFn out_fn) {
-  const MlpLayout L = mlp_layout();
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int row0 = wave * 16;  // this wave's 16 rows
-  for (int l = 0; l < 5; ++l) {
-    const int K = L.in[l], N = L.out[l];
-    const int ksteps = (K + 3) / 4, nblk = (N + 15) / 16;
-    f4m c[8];
+// One MFMA layer (K = 128) over the wave's 64 rows: NB column blocks of 16; a
+// hidden layer writes gelu(z) back into act as BF16, the output layer hands its
+// N columns to out_fn.
+template <int NB, bool LAST, class OutFn>
+__device__ __forceinline__ void mlp_layer(const unsigned short* __restrict__ w, const unsigned short* __restrict__ b,
+                                 int N, unsigned short* act, OutFn& out_fn) {
+  const int lane = threadIdx.x & 63;
+  bf16x8 breg[4][NB];
+  float bias[NB];
 #pragma unroll
-    for (int cb = 0; cb < 8; ++cb) {
-      const int col = 16 * cb + (lane & 15);
-      const float bias = (cb < nblk && col < N) ? bf16_to_f(w16[L.b[l] + col]) : 0.f;
-      c[cb] = {bias, bias, bias, bias};
-    }
-    for (int s = 0; s < ksteps; ++s) {
-      const int k = 4 * s + (lane >> 4);
-      const float a = k < K ? act[(row0 + (lane & 15)) * kMlpHidden + k] : 0.f;
+  for (int cb = 0; cb < NB; ++cb) {
+    const int col = 16 * cb + (lane & 15);
+    const int cc = col < N ? col : N - 1;  // clamped read; the column is dropped
+    bias[cb] = bf2f(b[cc]);
 #pragma unroll
-      for (int cb = 0; cb < 8; ++cb) {
-        if (cb >= nblk) break;
-        const int col = 16 * cb + (lane & 15);
-        const float b = (k < K && col < N) ? bf16_to_f(w16[L.w[l] + col * K + k]) : 0.f;
-        c[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c[cb], 0, 0, 0);
-      }
-    }
-    __syncthreads();  // every wave has read act
+    for (int ks = 0; ks < 4; ++ks)
+      breg[ks][cb] = *reinterpret_cast<const bf16x8*>(w + cc * kMlpHidden + 32 * ks + 8 * (lane >> 4));
+  }
+#pragma unroll 1
+  for (int t = 0; t < kInfWaveRows / 16; ++t) {
+    f4m c[NB];
 #pragma unroll
-    for (int cb = 0; cb < 8; ++cb) {
-      if (cb >= nblk) break;
+    for (int cb = 0; cb < NB; ++cb) c[cb] = f4m{bias[cb], bias[cb], bias[cb], bias[cb]};
+    const unsigned short* arow = act + (16 * t + (lane & 15)) * kActStride + 8 * (lane >> 4);
+    bf16x8 a[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) a[ks] = *reinterpret_cast<const bf16x8*>(arow + 32 * ks);
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+      for (int cb = 0; cb < NB; ++cb)
+        c[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ks], breg[ks][cb], c[cb], 0, 0, 0);
+    // result: element i of lane l is row 4*(l/16) + i of the tile, column l%16 of the block
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb) {
       const int col = 16 * cb + (lane & 15);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int row = row0 + 4 * (lane >> 4) + i;
-        if (l < 4) {
-          act[row * kMlpHidden + col] = gelu(c[cb][i]);
-        } else if (col < N) {
-          out_fn(row, col, c[cb][i]);
-        }
+        const int row = 16 * t + 4 * (lane >> 4) + i;
+        if (!LAST) act[row * kActStride + col] = f2bf(gelu(c[cb][i]));
+        else if (col < N) out_fn(row, col, c[cb][i]);
       }
     }
-    __syncthreads();
   }
+  __builtin_amdgcn_sched_barrier(0);  // keep the next layer's weight loads after this layer
 }
 
-// plain forward of n rows x [n][3] -> y [n][31] (normalised), BF16 weights
-__global__ __launch_bounds__(kInfThreads) void mlp_infer_rows(const unsigned short* __restrict__ w16g,
+// The wave's 64 rows through the network. Lane r supplies the inputs (x0, x1, x2)
+// of row r; out_fn(row, col, value) receives the 31 outputs. act: the wave's
+// [64][kActStride] BF16 in LDS. Rows are private to the wave, and a wave's LDS
+// operations complete in order, so no barrier is needed.
+template <class OutFn>
+__device__ __forceinline__ void mlp_wave64(const unsigned short* __restrict__ w16, float x0, float x1, float x2,
+                           unsigned short* act, OutFn out_fn) {
+  const MlpLayout L = mlp_layout();
+  const int lane = threadIdx.x & 63;
+  // layer 0 (3 -> 128), lane = row; weights wave-uniform
+  {
+    unsigned* arow = reinterpret_cast<unsigned*>(act + lane * kActStride);
+#pragma unroll 2
+    for (int j = 0; j < kMlpHidden; j += 2) {
+      float z[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int o = j + q;
+        z[q] = bf2f(w16[L.b[0] + o]) + bf2f(w16[L.w[0] + 3 * o]) * x0 +
+               bf2f(w16[L.w[0] + 3 * o + 1]) * x1 + bf2f(w16[L.w[0] + 3 * o + 2]) * x2;
+      }
+      arow[j / 2] = (unsigned)f2bf(gelu(z[0])) | ((unsigned)f2bf(gelu(z[1])) << 16);
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  mlp_layer<8, false>(w16 + L.w[1], w16 + L.b[1], kMlpHidden, act, out_fn);
+  mlp_layer<8, false>(w16 + L.w[2], w16 + L.b[2], kMlpHidden, act, out_fn);
+  mlp_layer<8, false>(w16 + L.w[3], w16 + L.b[3], kMlpHidden, act, out_fn);
+  mlp_layer<2, true>(w16 + L.w[4], w16 + L.b[4], kMlpOut, act, out_fn);
+}
+
+// plain forward of n rows x [n][3] -> y [n][31] (normalised), BF16
+__global__ __launch_bounds__(kInfThreads, 2) void mlp_infer_rows(const unsigned short* __restrict__ w16,
                                                               const float* __restrict__ x, int n,
                                                               float* __restrict__ y) {
-  extern __shared__ unsigned char smem[];
-  unsigned short* w16 = reinterpret_cast<unsigned short*>(smem);
-  float* act = reinterpret_cast<float*>(smem + kMlpParamsPadded * 2);
-  for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) w16[i] = w16g[i];
-  __syncthreads();
-  for (int t0 = blockIdx.x * kInfRows; t0 < n; t0 += gridDim.x * kInfRows) {
-    for (int e = threadIdx.x; e < kInfRows * 3; e += blockDim.x) {
-      const int r = e / 3, k = e % 3;
-      act[r * kMlpHidden + k] = (t0 + r < n) ? x[(size_t)(t0 + r) * 3 + k] : 0.f;
-    }
-    __syncthreads();
-    mlp_tile(w16, act, [&](int r, int c, float v) {
-      if (t0 + r < n) y[(size_t)(t0 + r) * kMlpOut + c] = v;
-    });
+  __shared__ __attribute__((aligned(16))) unsigned short act[kInfThreads / 64][kInfWaveRows * kActStride];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // one workgroup per kInfRows rows (a persistent loop would let the compiler hoist
+  // every layer's weight fragments out of it)
+  const size_t r0 = (size_t)blockIdx.x * kInfRows + (size_t)wave * kInfWaveRows;
+  if (r0 >= (size_t)n) return;
+  const size_t row = r0 + lane;
+  float x0 = 0.f, x1 = 0.f, x2 = 0.f;
+  if (row < (size_t)n) {
+    x0 = x[row * 3];
+    x1 = x[row * 3 + 1];
+    x2 = x[row * 3 + 2];
   }
+  mlp_wave64(w16, x0, x1, x2, act[wave], [=](int r, int c, float v) {
+    if (r0 + r < (size_t)n) y[(r0 + r) * kMlpOut + c] = v;
+  });
 }
 
 // Surrogate channels of a window (gsx = 0), chbuf[block][site][32] =
 // envelope * (mean + sd * net), for the blocks 0: (x, 0) and 1: (0, x), shared
 // by every yv, and 2 + 3u + j: (x, y), (y, x), (x, x - y) of the distinct yv u.
-__global__ __launch_bounds__(kInfThreads) void mlp_surrogate_channels(
-    const unsigned short* __restrict__ w16g, const float* __restrict__ scale,
-    const hlbl_coord* __restrict__ yvs, int n_unique, hlbl_geom global, double xunit,
-    float* __restrict__ chbuf) {
-  extern __shared__ unsigned char smem[];
-  unsigned short* w16 = reinterpret_cast<unsigned short*>(smem);
-  float* act = reinterpret_cast<float*>(smem + kMlpParamsPadded * 2);
-  __shared__ float s_env[kInfRows];
-  __shared__ float s_mean[kMlpOut], s_sd[kMlpOut];
-  for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) w16[i] = w16g[i];
-  for (int i = threadIdx.x; i < kMlpOut; i += blockDim.x) {
-    s_mean[i] = scale[i];
-    s_sd[i] = scale[kMlpOut + i];
-  }
-  __syncthreads();
+// Arguments outside the trained domain (|x| or |y| above radius) take the
+// channels from the table instead.
+__global__ __launch_bounds__(kInfThreads, 2) void mlp_surrogate_channels(
+    const unsigned short* __restrict__ w16, const float* __restrict__ scale,
+    const float* __restrict__ table32, float radius, const hlbl_coord* __restrict__ yvs,
+    int n_unique, hlbl_geom global, double xunit, float* __restrict__ chbuf) {
+  __shared__ __attribute__((aligned(16))) unsigned short act[kInfThreads / 64][kInfWaveRows * kActStride];
+  __shared__ float s_env[kInfThreads / 64][kInfWaveRows];
+  __shared__ float s_acb[kInfThreads / 64][kInfWaveRows][3];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const size_t V = geom_volume(global);
   const size_t rows = (size_t)(2 + 3 * n_unique) * V;
   const hlbl_coord zero = {0, 0, 0, 0};
-  const float inv_r = (float)(1.0 / qed::kRadius);
-  for (size_t t0 = (size_t)blockIdx.x * kInfRows; t0 < rows; t0 += (size_t)gridDim.x * kInfRows) {
-    if (threadIdx.x < kInfRows) {
-      const int r = threadIdx.x;
-      const size_t row = t0 + r;
-      float a = 0.f, c = 0.f, b = 0.f, env = 0.f;
-      if (row < rows) {
-        const int blk = (int)(row / V);
-        const size_t site = row % V;
-        const int u = blk < 2 ? 0 : (blk - 2) / 3;
-        const int e = blk == 0 ? 1 : blk == 1 ? 3 : 2 * ((blk - 2) % 3);
-        SiteGeom<float> g;
-        site_geom(site, global, global, zero, zero, yvs[u], xunit, g);
-        float pu[4], pv[4];
-        eval_args(e, g, pu, pv);
-        qed::acb(pu, pv, a, c, b);
-        env = (float)mlp_envelope(a, b, c);
-      }
-      act[r * kMlpHidden + 0] = a * inv_r;
-      act[r * kMlpHidden + 1] = b * inv_r;
-      act[r * kMlpHidden + 2] = c;
-      s_env[r] = env;
+  const float inv_r = 1.0f / radius;
+  {
+    const size_t r0 = (size_t)blockIdx.x * kInfRows + (size_t)wave * kInfWaveRows;
+    if (r0 >= rows) return;
+    const size_t row = r0 + lane;
+    float a = 0.f, c = 0.f, b = 0.f, env = 0.f;
+    if (row < rows) {
+      const int blk = (int)(row / V);
+      const size_t site = row - (size_t)blk * V;
+      const int u = blk < 2 ? 0 : (blk - 2) / 3;
+      const int e = blk == 0 ? 1 : blk == 1 ? 3 : 2 * ((blk - 2) % 3);
+      SiteGeom<float> g;
+      site_geom(site, global, global, zero, zero, yvs[u], xunit, g);
+      float pu[4], pv[4];
+      eval_args(e, g, pu, pv);
+      qed::acb(pu, pv, a, c, b);
+      env = (float)mlp_envelope(a, b, c);
+      if (a > radius || b > radius) env = -1.f;  // outside the trained domain
     }
-    __syncthreads();
-    mlp_tile(w16, act, [&](int r, int c, float v) {
-      const size_t row = t0 + r;
-      if (row < rows) chbuf[row * qed::kNCh + c] = s_env[r] * (s_mean[c] + s_sd[c] * v);
+    s_env[wave][lane] = env;
+    s_acb[wave][lane][0] = a;
+    s_acb[wave][lane][1] = c;
+    s_acb[wave][lane][2] = b;
+    const float* envw = s_env[wave];
+    const float(*acbw)[3] = s_acb[wave];
+    mlp_wave64(w16, fminf(a * inv_r, 1.f), fminf(b * inv_r, 1.f), c, act[wave], [=](int r, int col, float v) {
+      if (r0 + r >= rows) return;
+      float ch;
+      if (envw[r] >= 0.f) {
+        ch = envw[r] * (scale[col] + scale[kMlpOut + col] * v);
+      } else {
+        const qed::Cell<float> cell = qed::locate(acbw[r][0], acbw[r][1], acbw[r][2]);
+        ch = qed::interp(table32, cell, col);
+      }
+      chbuf[(r0 + r) * qed::kNCh + col] = ch;
     });
   }
 }
 
-size_t mlp_infer_smem() { return (size_t)kMlpParamsPadded * 2 + (size_t)kInfRows * kMlpHidden * 4; }
-
 void launch_surrogate_channels(hlbl_contractor* c, WindowPlan& w, hipStream_t st) {
   hlbl_qed* q = c->qed;
-  const int nwg = 2 * c->n_cu;
-  hipLaunchKernelGGL(mlp_surrogate_channels, dim3(nwg), dim3(kInfThreads), mlp_infer_smem(), st,
-                     q->mlp_w_bf16.p, q->mlp_scale.p, w.d_yvs.p, w.n_unique, c->global, w.xunit,
-                     w.chbuf.p);
+  const size_t rows = (size_t)(2 + 3 * w.n_unique) * c->volume;
+  const unsigned nwg = (unsigned)div_up(rows, kInfRows);
+  hipLaunchKernelGGL(mlp_surrogate_channels, dim3(nwg), dim3(kInfThreads), 0, st, q->mlp_w_bf16.p,
+                     q->mlp_scale.p, q->table32.p, (float)q->mlp_radius, w.d_yvs.p, w.n_unique,
+                     c->global, w.xunit, w.chbuf.p);
   check_hip(hipGetLastError(), "mlp_surrogate_channels launch");
 }
 
-// surrogate channels at points (a, cos beta, b): inputs, network, envelope and scale
-__global__ void k_surrogate_inputs(const double* __restrict__ acb, int n, float* __restrict__ x) {
+// surrogate channels at points (a, cos beta, b): inputs, network, envelope and
+// scale; points outside the trained domain take the table's channels
+__global__ void k_surrogate_inputs(const double* __restrict__ acb, int n, double radius,
+                                   float* __restrict__ x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  x[3 * i + 0] = (float)(acb[3 * i + 0] / qed::kRadius);
-  x[3 * i + 1] = (float)(acb[3 * i + 2] / qed::kRadius);
+  x[3 * i + 0] = (float)fmin(acb[3 * i + 0] / radius, 1.0);
+  x[3 * i + 1] = (float)fmin(acb[3 * i + 2] / radius, 1.0);
   x[3 * i + 2] = (float)acb[3 * i + 1];
 }
 __global__ void k_surrogate_outputs(const double* __restrict__ acb, const float* __restrict__ y,
-                                    const float* __restrict__ scale, int n, double* __restrict__ out) {
+                                    const float* __restrict__ scale, const double* __restrict__ table,
+                                    double radius, int n, double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * kMlpOut) return;
   const int r = i / kMlpOut, c = i % kMlpOut;
-  const double env = mlp_envelope(acb[3 * r], acb[3 * r + 2], acb[3 * r + 1]);
+  const double a = acb[3 * r], cb = acb[3 * r + 1], b = acb[3 * r + 2];
+  if (a > radius || b > radius) {
+    out[i] = qed::interp(table, qed::locate(a, cb, b), c);
+    return;
+  }
+  const double env = mlp_envelope(a, b, cb);
   out[i] = env * ((double)scale[c] + (double)scale[kMlpOut + c] * (double)y[i]);
 }
 
-void mlp_infer(hlbl_qed* q, const float* x, int n, float* y, hipStream_t st);
-
 void launch_surrogate_points(hlbl_qed* q, const double* acb, int n, double* out, hipStream_t st) {
   if (!q->mlp_ready) throw Error("hlbl_qed_channels: mode 1 needs a trained surrogate");
-  mlp_configure_kernels();
   float *x = nullptr, *y = nullptr;
   check_hip(hipMallocAsync((void**)&x, (size_t)n * 3 * sizeof(float), st), "hipMallocAsync");
   check_hip(hipMallocAsync((void**)&y, (size_t)n * kMlpOut * sizeof(float), st), "hipMallocAsync");
-  hipLaunchKernelGGL(k_surrogate_inputs, dim3((unsigned)div_up(n, 256)), dim3(256), 0, st, acb, n, x);
+  hipLaunchKernelGGL(k_surrogate_inputs, dim3((unsigned)div_up(n, 256)), dim3(256), 0, st, acb, n,
+                     q->mlp_radius, x);
   mlp_infer(q, x, n, y, st);
   hipLaunchKernelGGL(k_surrogate_outputs, dim3((unsigned)div_up((size_t)n * kMlpOut, 256)), dim3(256), 0,
-                     st, acb, y, q->mlp_scale.p, n, out);
+                     st, acb, y, q->mlp_scale.p, q->table64.p, q->mlp_radius, n, out);
   hipError_t e = hipGetLastError();
   (void)hipFreeAsync(x, st);
   (void)hipFreeAsync(y, st);
@@ -469,22 +533,10 @@ void launch_surrogate_points(hlbl_qed* q, const double* acb, int n, double* out,
 }
 
 void mlp_infer(hlbl_qed* q, const float* x, int n, float* y, hipStream_t st) {
-  int dev = 0, ncu = 0;
-  check_hip(hipGetDevice(&dev), "hipGetDevice");
-  check_hip(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "attr");
-  const int nwg = (int)std::min<size_t>(div_up(n, kInfRows), (size_t)2 * ncu);
-  if (nwg <= 0) return;
-  hipLaunchKernelGGL(mlp_infer_rows, dim3(nwg), dim3(kInfThreads), mlp_infer_smem(), st,
-                     q->mlp_w_bf16.p, x, n, y);
+  if (n <= 0) return;
+  const unsigned nwg = (unsigned)div_up(n, kInfRows);
+  hipLaunchKernelGGL(mlp_infer_rows, dim3(nwg), dim3(kInfThreads), 0, st, q->mlp_w_bf16.p, x, n, y);
   check_hip(hipGetLastError(), "mlp_infer_rows launch");
-}
-
-void mlp_configure_kernels() {
-  check_hip(hipFuncSetAttribute((const void*)mlp_infer_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)mlp_infer_smem()), "hipFuncSetAttribute");
-  check_hip(hipFuncSetAttribute((const void*)mlp_surrogate_channels,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)mlp_infer_smem()),
-            "hipFuncSetAttribute");
 }
 
 // ------------------------------------------------------------------ training --
@@ -528,7 +580,7 @@ void mlp_train(hlbl_qed* q, int n_samples, int epochs, int batch, float lr, uint
   if (!q->mlp_w.p) mlp_init_params(q, seed ^ 0x5eedULL);
   std::vector<float> x;
   std::vector<double> t;
-  mlp_build_set(seed, n_samples, x, t);
+  mlp_build_set(seed, n_samples, q->mlp_radius, q->host_table, x, t);
   // per-channel standardisation of the enveloped targets
   std::vector<float> sc(2 * kMlpOut);
   std::vector<float> tn((size_t)n_samples * kMlpOut);
@@ -582,7 +634,7 @@ void mlp_train(hlbl_qed* q, int n_samples, int epochs, int batch, float lr, uint
       hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)div_up(batch * kMlpOut, 256)), dim3(256), 0, st, dt.p, didx.p, batch, kMlpOut, bt.p);
       mlp_requantise(q, st);
       const bool want = (s == steps_per_epoch - 1);
-      const double l = mlp_backward(q->mlp_wq.p, bx.p, bt.p, batch, wk, grad.p, st, want);
+      const double l = mlp_backward(q->mlp_wq.p, bx.p, bt.p, batch, wk, grad.p, st, want, true);
       if (want) eloss = l;
       ++step;
       const double frac = (double)step / ((double)epochs * steps_per_epoch);

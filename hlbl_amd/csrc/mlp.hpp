@@ -28,7 +28,6 @@ HLBL_HD MlpLayout mlp_layout() {
 }
 constexpr int kMlpParamsPadded = (kMlpParams + 7) / 8 * 8;
 
-void mlp_configure_kernels();
 void mlp_init_params(hlbl_qed* q, uint64_t seed);
 void mlp_requantise(hlbl_qed* q, hipStream_t st);
 void mlp_train(hlbl_qed* q, int n_samples, int epochs, int batch, float lr, uint64_t seed,

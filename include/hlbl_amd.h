@@ -89,6 +89,10 @@ int hlbl_mlp_init(hlbl_qed* q, uint64_t seed);
 /* one full training run; loss history (one value per epoch) goes to host `loss` */
 int hlbl_mlp_train(hlbl_qed* q, int n_samples, int epochs, int batch, float lr,
                    uint64_t seed, float* loss, hlbl_stream stream);
+/* trained domain of the surrogate: |x|, |y| <= radius (muon units, 0 < radius <=
+ * the kernel radius, 12 fm; default the kernel radius). Arguments outside it take
+ * the table's channels. Changing it discards the trained state (train again). */
+int hlbl_mlp_set_domain(hlbl_qed* q, double radius);
 /* parameters (FP32 master copy, packed layer by layer: W then b) to/from host */
 int hlbl_mlp_get_params(const hlbl_qed* q, float* host, int n);
 int hlbl_mlp_set_params(hlbl_qed* q, const float* host, int n);

@@ -894,4 +894,33 @@ double oracle_mlp(const float* params, const float* x, const float* t, int n, do
   return loss / ((double)n * 31);
 }
 
+// Forward of the BF16 inference: BF16 weights, every hidden activation rounded
+// to BF16 after the GELU, FP64 arithmetic otherwise. y[n][31].
+void oracle_mlp_forward_bf16(const float* params, const float* x, int n, double* y) {
+  const int np = oracle_mlp_param_count();
+  std::vector<double> w(np);
+  for (int i = 0; i < np; ++i) w[i] = bf16(params[i]);
+  int offw[5], offb[5], o = 0;
+  for (int l = 0; l < 5; ++l) {
+    offw[l] = o;
+    o += kIn[l] * kOut[l];
+    offb[l] = o;
+    o += kOut[l];
+  }
+  std::vector<double> a, z;
+  for (int r = 0; r < n; ++r) {
+    a.assign(x + 3 * r, x + 3 * r + 3);
+    for (int l = 0; l < 5; ++l) {
+      z.assign(kOut[l], 0.0);
+      for (int j = 0; j < kOut[l]; ++j) {
+        double s = w[offb[l] + j];
+        for (int k = 0; k < kIn[l]; ++k) s += w[offw[l] + j * kIn[l] + k] * a[k];
+        z[j] = l < 4 ? (double)bf16((float)(0.5 * s * (1.0 + std::erf(s / std::sqrt(2.0))))) : s;
+      }
+      a = z;
+    }
+    for (int c = 0; c < 31; ++c) y[31 * r + c] = a[c];
+  }
+}
+
 }  // extern "C"

@@ -331,3 +331,90 @@ def test_dzsu_of_point_fields(oracle):
             want = (G[5] @ G[s])[c1, c0]
             assert np.isclose(d[s, a0, b0], want), (s, d[s, a0, b0], want)
             assert np.count_nonzero(np.abs(d[s]) > 0) <= 1
+
+
+# ---- the MLP restatement ------------------------------------------------------
+
+MLP_IN, MLP_OUT = (3, 128, 128, 128, 128), (128, 128, 128, 128, 31)
+
+
+def mlp_offsets():
+    off, o = [], 0
+    for i, j in zip(MLP_IN, MLP_OUT):
+        off.append((o, o + i * j))
+        o += i * j + j
+    return off, o
+
+
+def mlp_fns(oracle):
+    oracle.oracle_mlp.restype = ctypes.c_double
+    oracle.oracle_mlp.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    oracle.oracle_mlp_forward_bf16.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    oracle.oracle_mlp_param_count.restype = ctypes.c_int
+
+
+def bf16_np(a):
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def gelu_exact(z):
+    import math
+    return 0.5 * z * (1.0 + math.erf(z / math.sqrt(2.0)))
+
+
+def test_mlp_chain_known_answer(oracle):
+    """A network that passes x0 along unit 0 of every layer: output 0 is
+    GELU(GELU(GELU(GELU(x0)))) (closed form with math.erf), output 5 is its bias,
+    and W[out][in] indexing is fixed by where the unit weights sit."""
+    mlp_fns(oracle)
+    off, n = mlp_offsets()
+    assert n == oracle.oracle_mlp_param_count()
+    p = np.zeros(n, np.float32)
+    p[off[0][0] + 0 * 3 + 0] = 1.0          # W0[0][0]
+    for l in (1, 2, 3, 4):
+        p[off[l][0] + 0 * 128 + 0] = 1.0    # Wl[0][0]
+    p[off[4][1] + 5] = 0.375                # b4[5]
+    x = np.array([[0.8125, -3.0, 7.0]], np.float32)
+    y = np.zeros((1, 31))
+    oracle.oracle_mlp(P(p), P(x), P(np.zeros((1, 31), np.float32)), 1, None, P(y))
+    v = 0.8125
+    for _ in range(4):
+        v = gelu_exact(v)
+    assert abs(y[0, 0] - v) <= 1e-15
+    assert y[0, 5] == 0.375 and np.count_nonzero(y[0]) == 2
+    # the BF16 inference forward rounds each hidden activation to BF16
+    yb = np.zeros((1, 31))
+    oracle.oracle_mlp_forward_bf16(P(p), P(x), 1, P(yb))
+    v = 0.8125
+    for _ in range(4):
+        v = float(bf16_np(np.array([gelu_exact(v)], np.float64).astype(np.float32))[0])
+    assert abs(yb[0, 0] - v) <= 1e-12 and yb[0, 5] == 0.375
+
+
+def test_mlp_gradient_finite_differences(oracle):
+    """The backward pass of the restatement against central differences of its own
+    loss, at parameters that are BF16-representable (so the rounding is exact)."""
+    mlp_fns(oracle)
+    off, n = mlp_offsets()
+    rng = np.random.default_rng(3)
+    p = bf16_np(rng.normal(0, 0.2, n).astype(np.float32))
+    x = rng.uniform(-1, 1, (5, 3)).astype(np.float32)
+    t = rng.normal(0, 1, (5, 31)).astype(np.float32)
+    g = np.zeros(n)
+    oracle.oracle_mlp(P(p), P(x), P(t), 5, P(g), None)
+    # one weight and one bias of every layer; steps of one BF16 ulp-pair (exact)
+    idx = []
+    for l in range(5):
+        idx += [off[l][0] + 7 % (MLP_IN[l] * MLP_OUT[l]), off[l][1] + 3]
+    for i in idx:
+        h = 2.0 ** -7
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        assert bf16_np(pp)[i] == pp[i] and bf16_np(pm)[i] == pm[i]
+        lp = oracle.oracle_mlp(P(pp), P(x), P(t), 5, None, None)
+        lm = oracle.oracle_mlp(P(pm), P(x), P(t), 5, None, None)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[i]) <= 2e-3 * max(1e-3, abs(g[i])) + 1e-6, (i, fd, g[i])
