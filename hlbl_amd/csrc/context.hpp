@@ -125,6 +125,7 @@ struct hlbl_contractor {
   hlbl::DevBuf<double> gdz;     // g_dzu, g_dzsu of both flavours [2][40][144][2]
   hlbl::DevBuf<double> bmat;    // pass-2 B operands [2][288][48]
   hlbl::DevBuf<double> coef;    // pass-2 coefficient field [V][16][40]
+  hlbl::DevBuf<double> kcst;    // the pair constants L0(0,y), L0(y,0) [2][384]
   hlbl::DevBuf<double> part2;   // pass-2 partials [nwg2][12]
   hlbl::DevBuf<double> result;  // [2]
   hlbl::PinBuf<double> host_result;

@@ -5,6 +5,8 @@
 // with the 640 traces per site on FP64 MFMA.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "context.hpp"
 #include "contract_kernels.hpp"
 #include "kpart.hpp"
@@ -116,18 +118,42 @@ __global__ void form_bmat(const double* __restrict__ gdz, B* __restrict__ bmat) 
 // ---------------------------------------------------------------- pass 2 --
 // (a) the coefficient field of the pair (the QED kernel of every site, both
 //     flavours at once), (b) the traces on MFMA weighted by it.
-constexpr int kKfThreads = 512;
+constexpr int kKfThreads = 256;  // 4 waves, one site each at a time
 
+// the pair constants [L0(0,y) | L0(y,0)] into cst[768], one wave
+__global__ __launch_bounds__(64) void kfield_consts64(const double* __restrict__ table, hlbl_geom global,
+                                                      hlbl_coord yv, double xunit, double* __restrict__ cst) {
+  __shared__ TermLds tt;
+  __shared__ double w[kKwLen];
+  __shared__ double c[2 * qed::kNL];
+  stage_terms(c_terms64, &tt);
+  __syncthreads();
+  const hlbl_coord zero = {0, 0, 0, 0};
+  SiteGeom<double> g;
+  site_geom(0, global, global, zero, zero, yv, xunit, g);
+  kfield_wave_constants<double, double>(table, tt, g, w, c);
+  for (int i = threadIdx.x; i < 2 * qed::kNL; i += 64) cst[i] = c[i];
+}
+
+// coef[site][640] of every local site of the pair; wave w of workgroup b takes
+// sites b*4 + w, then strides by the grid's wave count
 __global__ __launch_bounds__(kKfThreads) void kfield_fp64(
     const double* __restrict__ table, hlbl_geom global, hlbl_geom local, hlbl_coord proc,
-    hlbl_coord gsx, hlbl_coord yv, double xunit, size_t sites_per_wg, double* __restrict__ coef) {
-  __shared__ double lds[kKfieldLds];
+    hlbl_coord gsx, hlbl_coord yv, double xunit, const double* __restrict__ cst,
+    double* __restrict__ coef) {
+  __shared__ TermLds tt;
+  __shared__ double w[kKfThreads / 64][kKwLen];
+  stage_terms(c_terms64, &tt);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6;
   const size_t V = geom_volume(local);
-  const size_t begin = (size_t)blockIdx.x * sites_per_wg;
-  const size_t end = begin + sites_per_wg < V ? begin + sites_per_wg : V;
-  if (begin >= end) return;
-  kfield_sites<double, double>(table, c_terms64, global, local, proc, gsx, yv, xunit, begin, end,
-                               coef + begin * kCoefField, lds);
+  const size_t nw = (size_t)gridDim.x * (kKfThreads / 64);
+  for (size_t site = (size_t)blockIdx.x * (kKfThreads / 64) + wave; site < V; site += nw) {
+    SiteGeom<double> g;
+    site_geom(site, global, local, proc, gsx, yv, xunit, g);
+    kfield_wave<double, double, double>(table, tt, g, ChannelRows::none(), cst, w[wave],
+                                        coef + site * kCoefField);
+  }
 }
 
 // 8 waves: wave = (flavour, quarter of the K range); each covers the three
@@ -298,10 +324,13 @@ void launch_kfield(hlbl_contractor* c, hlbl_coord proc, hlbl_coord gsx, hlbl_coo
                    double xunit, hlbl_geom global, hlbl_geom local, hipStream_t st) {
   const size_t V = geom_volume(local);
   c->coef.ensure(V * kCoefField);
-  const size_t spw = div_up(V, (size_t)(4 * c->n_cu));
-  const int nwg = (int)div_up(V, spw);
+  c->kcst.ensure(2 * qed::kNL);
+  hipLaunchKernelGGL(kfield_consts64, dim3(1), dim3(64), 0, st, c->qed->table64.p, global, yv, xunit,
+                     c->kcst.p);
+  check_hip(hipGetLastError(), "kfield_consts64 launch");
+  const int nwg = (int)std::min<size_t>(div_up(V, kKfThreads / 64), (size_t)2 * c->n_cu);
   hipLaunchKernelGGL(kfield_fp64, dim3(nwg), dim3(kKfThreads), 0, st, c->qed->table64.p, global,
-                     local, proc, gsx, yv, xunit, spw, c->coef.p);
+                     local, proc, gsx, yv, xunit, c->kcst.p, c->coef.p);
   check_hip(hipGetLastError(), "kfield_fp64 launch");
 }
 

@@ -108,83 +108,8 @@ __device__ inline void kp_dmat(int i, const SiteGeom<R>& g, R* d) {
   d[32 + i] = (dl - lam * blend4(g.dm, m) * blend4(g.dm, a)) * exp(-lam * norm2(g.dm) * R(0.5));
 }
 
-// Phase 4: component o of K_m (m = 0: K1, 1: K2, 2: K3). L0 holds the eight
-// evaluations [8][384] (six site evaluations, then (0,y) and (y,0)); D the
-// subtraction matrices of xm, ym, dm.
-template <class R>
-__device__ inline R kp_kernel(int m, int o, const SiteGeom<R>& g, const R* L0, const R* D) {
-  const R *l0xy, *l00y, *l0x0, *du, *dv;
-  bool in;
-  const R* Dx = D;
-  const R* Dy = D + 16;
-  const R* Dd = D + 32;
-  if (m == 0) {         // L(x, y)
-    l0xy = L0 + 0 * qed::kNL; l00y = L0 + 6 * qed::kNL; l0x0 = L0 + 1 * qed::kNL;
-    du = Dx; dv = Dy; in = g.in12;
-  } else if (m == 1) {  // L(y, x)
-    l0xy = L0 + 2 * qed::kNL; l00y = L0 + 3 * qed::kNL; l0x0 = L0 + 7 * qed::kNL;
-    du = Dy; dv = Dx; in = g.in12;
-  } else {              // L(x, d)
-    l0xy = L0 + 4 * qed::kNL; l00y = L0 + 5 * qed::kNL; l0x0 = L0 + 1 * qed::kNL;
-    du = Dx; dv = Dd; in = g.in3;
-  }
-  if (!in) return R(0);
-  return qed::lambda_subtract(o, l0xy, l00y, l0x0, du, dv);
-}
-
-// Phase 5: coefficient (row, col) from K = [3][384] (K1, K2, K3).
-template <class R>
-__device__ inline R kp_coef(int row, int col, const SiteGeom<R>& g, const R* K) {
-  const int mu = row >> 2, lam = row & 3;
-  const R* K1 = K;
-  const R* K2 = K + qed::kNL;
-  const R* K3 = K + 2 * qed::kNL;
-  auto idx = [](int k, int a, int b, int c) { return ((k * 4 + a) * 4 + b) * 4 + c; };
-  if (col < 24) {
-    const int k = col >> 2, nu = col & 3;
-    return -(K1[idx(k, mu, nu, lam)] + K2[idx(k, nu, mu, lam)] - K3[idx(k, lam, nu, mu)]);
-  }
-  if (col < 40) {
-    const int s = (col - 24) >> 2, nu = (col - 24) & 3;
-    R r = R(0);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      const int rho = comb_rho(k), sigma = comb_sigma(k);  // compile-time after unrolling
-      const R b = K3[idx(k, lam, nu, mu)];
-      if (sigma == s) r -= b * g.xz[rho];
-      if (rho == s) r += b * g.xz[sigma];
-    }
-    return r;
-  }
-  return R(0);
-}
-
 constexpr int kCoefUsed = 40;
 constexpr int kCoefField = 16 * kCoefUsed;  // coefficients per site
-
-// LDS of one workgroup running the kfield functions (R elements)
-constexpr int kKfieldLds = 8 * qed::kNCh + 8 * qed::kNT + 8 * qed::kNL + 48;
-
-// Pair constants L0(0, y), L0(y, 0) (evaluations 6 and 7) into the L0 area of
-// lds; the whole workgroup calls it and it ends with a barrier.
-template <class R, class TT>
-__device__ __forceinline__ void kfield_constants(const TT* __restrict__ table, const qed::TermTable& tt,
-                                                 const SiteGeom<R>& g, R* lds) {
-  R* sCh = lds;
-  R* sT = sCh + 8 * qed::kNCh;
-  R* sL0 = sT + 8 * qed::kNT;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  constexpr int nch = qed::kNChUsed;
-  for (int t = tid; t < 2 * nch; t += nt)
-    sCh[(6 + t / nch) * qed::kNCh + t % nch] = kp_channel(6 + t / nch, t % nch, g, table);
-  __syncthreads();
-  for (int t = tid; t < 2 * qed::kNT; t += nt)
-    sT[6 * qed::kNT + t] = kp_tensor(6 + t / qed::kNT, t % qed::kNT, g, sCh + (6 + t / qed::kNT) * qed::kNCh);
-  __syncthreads();
-  for (int t = tid; t < 2 * qed::kNL; t += nt)
-    sL0[6 * qed::kNL + t] = qed::l0_component(tt, t % qed::kNL, sT + (6 + t / qed::kNL) * qed::kNT);
-  __syncthreads();
-}
 
 // Where the channels of the six site evaluations come from when they are not
 // interpolated: p[e] points at the 32 channels of evaluation e of this site
@@ -203,56 +128,152 @@ struct ChannelRows {
   }
 };
 
-// The 16 x 40 coefficients of one site into out[640] (element type O). The
-// channels of the six site evaluations come from the table, or from rows when
-// given. Requires kfield_constants on the same lds (same pair). Whole
-// workgroup; no barrier at the end (the next call writes sCh first, and sT only
-// after its first barrier).
-template <class R, class TT, class O>
-__device__ __forceinline__ void kfield_site(const TT* __restrict__ table, const qed::TermTable& tt,
-                                            const SiteGeom<R>& g, const ChannelRows rows, R* lds,
-                                            O* __restrict__ out) {
-  R* sCh = lds;                     // [8][32]
-  R* sT = sCh + 8 * qed::kNCh;      // [8][192]; K [3][384] over the first 1152
-  R* sL0 = sT + 8 * qed::kNT;       // [8][384]
-  R* sD = sL0 + 8 * qed::kNL;       // [48]
-  R* sK = sT;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  constexpr int nch = qed::kNChUsed;
-  if (rows.given) {
-    for (int t = tid; t < 6 * nch; t += nt)
-      sCh[(t / nch) * qed::kNCh + t % nch] = (R)rows.row(t / nch)[t % nch];
-  } else {
-    for (int t = tid; t < 6 * nch; t += nt)
-      sCh[(t / nch) * qed::kNCh + t % nch] = kp_channel(t / nch, t % nch, g, table);
-  }
-  __syncthreads();
-  for (int t = tid; t < 6 * qed::kNT; t += nt)
-    sT[t] = kp_tensor(t / qed::kNT, t % qed::kNT, g, sCh + (t / qed::kNT) * qed::kNCh);
-  __syncthreads();
-  for (int t = tid; t < 6 * qed::kNL; t += nt)
-    sL0[t] = qed::l0_component(tt, t % qed::kNL, sT + (t / qed::kNL) * qed::kNT);
-  if (tid < 16) kp_dmat(tid, g, sD);
-  __syncthreads();
-  for (int t = tid; t < 3 * qed::kNL; t += nt) sK[t] = kp_kernel(t / qed::kNL, t % qed::kNL, g, sL0, sD);
-  __syncthreads();
-  for (int t = tid; t < kCoefField; t += nt) out[t] = (O)kp_coef(t / kCoefUsed, t % kCoefUsed, g, sK);
+// ---- the coefficient field, one wave per site --------------------------------
+// A wave computes the 16 x 40 coefficients of one site in its own LDS area, with
+// no workgroup barrier: a wave's LDS operations complete in order, and
+// wave_sync() keeps the compiler from moving accesses across phases.
+//   ch[32]  channels of the current evaluation
+//   T[192]  its T tensors
+//   L[3][384]  L0 slots: 0 = (x, 0), 1 = the partner (0, x) / (0, d), 2 = the main
+//           evaluation, turned in place into K_m (each lane updates its own entries)
+//   C[640]  coefficients; the K_m scatter into the 24 G columns as they complete
+//           (one coefficient per K element), the H columns gather from K3 at the end
+//   D[48]   subtraction matrices of xm, ym, dm
+constexpr int kKwLen = 32 + qed::kNT + 3 * qed::kNL + kCoefField + 48;
+
+// L0 sparse terms, staged in LDS by the workgroup
+struct TermLds {
+  unsigned short term[qed::kNL][qed::kMaxTerms];
+  unsigned char nterm[qed::kNL];
+};
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The coefficient field coef[site - begin][16][40] of sites [begin, end) of a
-// pair, plain form (channels interpolated in `table`), whole workgroup.
-template <class R, class TT>
-__device__ __forceinline__ void kfield_sites(const TT* __restrict__ table, const qed::TermTable& tt,
-                                             hlbl_geom global, hlbl_geom local, hlbl_coord proc,
-                                             hlbl_coord gsx, hlbl_coord yv, double xunit,
-                                             size_t begin, size_t end, R* __restrict__ coef, R* lds) {
-  SiteGeom<R> g;
-  site_geom(0, global, local, proc, gsx, yv, xunit, g);
-  kfield_constants(table, tt, g, lds);
-  for (size_t site = begin; site < end; ++site) {
-    site_geom(site, global, local, proc, gsx, yv, xunit, g);
-    kfield_site<R, TT, R>(table, tt, g, ChannelRows::none(), lds, coef + (site - begin) * kCoefField);
+// copy the term table into LDS; the whole workgroup calls it, then a barrier
+__device__ inline void stage_terms(const qed::TermTable& src, TermLds* dst) {
+  const unsigned short* s = &src.term[0][0];
+  unsigned short* d = &dst->term[0][0];
+  for (int i = threadIdx.x; i < qed::kNL * qed::kMaxTerms; i += blockDim.x) d[i] = s[i];
+  for (int i = threadIdx.x; i < qed::kNL; i += blockDim.x) dst->nterm[i] = src.nterm[i];
+}
+
+template <class R>
+__device__ inline R l0_lds(const TermLds& tt, int o, const R* t) {
+  R r = R(0);
+  const int n = tt.nterm[o];
+  for (int j = 0; j < n; ++j) {
+    const unsigned short e = tt.term[o][j];
+    r += R(qed::term_coef(e)) * t[qed::term_index(e)];
   }
+  return r;
+}
+
+// channels, T tensors and L0 of evaluation e (0..7) into dst[384]; the
+// evaluations 6, 7 (the pair constants) always interpolate the table
+template <class R, class TT>
+__device__ __forceinline__ void wave_l0(int e, const TT* __restrict__ table, const TermLds& tt,
+                                        const SiteGeom<R>& g, const ChannelRows& rows, R* ch, R* T,
+                                        R* dst) {
+  const int lane = threadIdx.x & 63;
+  if (lane < qed::kNChUsed)
+    ch[lane] = (rows.given && e < 6) ? (R)rows.row(e)[lane] : kp_channel(e, lane, g, table);
+  wave_sync();
+#pragma unroll
+  for (int k = 0; k < qed::kNT / 64; ++k) T[64 * k + lane] = kp_tensor(e, 64 * k + lane, g, ch);
+  wave_sync();
+#pragma unroll 2
+  for (int o = lane; o < qed::kNL; o += 64) dst[o] = l0_lds(tt, o, T);
+  wave_sync();
+}
+
+// K_m (m = 0: L(x,y), 1: L(y,x), 2: L(x,d)) in place over `main`, and its G-column
+// contributions: coef[(mu lam)][(k nu)] = -(K1[k mu nu lam] + K2[k nu mu lam] - K3[k lam nu mu])
+// (ker_4pt_contraction, kernels/cuda_lattice.cu:403-476)
+template <class R>
+__device__ __forceinline__ void wave_kernel(int m, R* main, const R* l00y, const R* l0x0,
+                                            const R* du, const R* dv, bool in, R* C) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll 2
+  for (int o = lane; o < qed::kNL; o += 64) {
+    const R k = in ? qed::lambda_subtract(o, main, l00y, l0x0, du, dv) : R(0);
+    main[o] = k;
+    const int c = o & 3, b = (o >> 2) & 3, a = (o >> 4) & 3, kk = o >> 6;
+    if (m == 0) C[(a * 4 + c) * kCoefUsed + kk * 4 + b] = -k;       // K1[kk][mu=a][nu=b][lam=c]
+    else if (m == 1) C[(b * 4 + c) * kCoefUsed + kk * 4 + a] -= k;  // K2[kk][nu=a][mu=b][lam=c]
+    else C[(c * 4 + a) * kCoefUsed + kk * 4 + b] += k;              // K3[kk][lam=a][nu=b][mu=c]
+  }
+  wave_sync();
+}
+
+// H column (s, nu) of row (mu, lam): sum over pairs k = (rho, sigma) of
+// K3[k][lam][nu][mu] (xz_sigma [rho == s] - xz_rho [sigma == s])
+template <class R>
+__device__ inline R coef_h(int row, int hc, const SiteGeom<R>& g, const R* K3) {
+  const int mu = row >> 2, lam = row & 3, s = hc >> 2, nu = hc & 3;
+  R r = R(0);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int rho = comb_rho(k), sigma = comb_sigma(k);
+    const R b = K3[((k * 4 + lam) * 4 + nu) * 4 + mu];
+    if (sigma == s) r -= b * g.xz[rho];
+    if (rho == s) r += b * g.xz[sigma];
+  }
+  return r;
+}
+
+// The 640 coefficients of one site into out (element type O), computed by the
+// calling wave in w (kKwLen elements of LDS). cst = [L0(0,y) | L0(y,0)] of the
+// pair (768 elements, any address space readable by the wave).
+template <class R, class TT, class O>
+__device__ __forceinline__ void kfield_wave(const TT* __restrict__ table, const TermLds& tt,
+                                            const SiteGeom<R>& g, const ChannelRows rows,
+                                            const R* cst, R* w, O* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  R* ch = w;
+  R* T = ch + 32;
+  R* L = T + qed::kNT;
+  R* C = L + 3 * qed::kNL;
+  R* D = C + kCoefField;
+  R* s0 = L;
+  R* s1 = L + qed::kNL;
+  R* s2 = L + 2 * qed::kNL;
+  if (lane < 16) kp_dmat(lane, g, D);
+  wave_sync();
+  const R* Dx = D;
+  const R* Dy = D + 16;
+  const R* Dd = D + 32;
+  wave_l0(1, table, tt, g, rows, ch, T, s0);                   // (x, 0)
+  wave_l0(0, table, tt, g, rows, ch, T, s2);                   // (x, y)
+  wave_kernel(0, s2, cst, s0, Dx, Dy, g.in12, C);              // K1 = L(x, y)
+  wave_l0(3, table, tt, g, rows, ch, T, s1);                   // (0, x)
+  wave_l0(2, table, tt, g, rows, ch, T, s2);                   // (y, x)
+  wave_kernel(1, s2, s1, cst + qed::kNL, Dy, Dx, g.in12, C);   // K2 = L(y, x)
+  wave_l0(5, table, tt, g, rows, ch, T, s1);                   // (0, d)
+  wave_l0(4, table, tt, g, rows, ch, T, s2);                   // (x, d)
+  wave_kernel(2, s2, s1, s0, Dx, Dd, g.in3, C);                // K3 = L(x, d)
+  for (int t = lane; t < 16 * 16; t += 64) {
+    const int row = t >> 4, hc = t & 15;
+    C[row * kCoefUsed + 24 + hc] = coef_h(row, hc, g, s2);
+  }
+  wave_sync();
+#pragma unroll
+  for (int k = 0; k < kCoefField / 64; ++k) out[64 * k + lane] = (O)C[64 * k + lane];
+  wave_sync();
+}
+
+// The pair constants [L0(0,y) | L0(y,0)] into cst (768 elements), by one wave
+// (w: kKwLen elements of LDS).
+template <class R, class TT>
+__device__ __forceinline__ void kfield_wave_constants(const TT* __restrict__ table, const TermLds& tt,
+                                                      const SiteGeom<R>& g, R* w, R* cst) {
+  R* ch = w;
+  R* T = ch + 32;
+  wave_l0(6, table, tt, g, ChannelRows::none(), ch, T, cst);
+  wave_l0(7, table, tt, g, ChannelRows::none(), ch, T, cst + qed::kNL);
 }
 
 }  // namespace hlbl

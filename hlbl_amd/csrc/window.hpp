@@ -37,6 +37,8 @@ struct WindowPlan {
   DevBuf<double> red1, xi, dz, gdz;
   DevBuf<float> bmat;
   DevBuf<float> coef;      // [n_unique][V][640]
+  DevBuf<double> cst64;    // pair constants L0(0,y), L0(y,0) per distinct yv [n_unique][768]
+  DevBuf<float> cst32;
   DevBuf<float> chbuf;     // surrogate channels [2 + 3 n_unique][V][32]
   DevBuf<double> part2, result;
   PinBuf<double> h_result;

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -36,108 +37,150 @@ size_t div_up(size_t a, size_t b) { return (a + b - 1) / b; }
 }  // namespace
 
 // ------------------------------------------------------------ coefficients --
-// The field F_yv[rel site][16][40] (FP32) for each distinct yv, gsx = 0. A site
-// with |x| or |x - y| below kNearRadius is evaluated in FP64 with the FP64
-// table; the others in FP32 from the FP32 table, or from the surrogate's
-// channels (chbuf, see below) when given.
+// The field F_yv[rel site][16][40] (FP32) for each distinct yv, gsx = 0, one wave
+// per site. Every site is first computed in FP32 (FP32 table, or the surrogate's
+// channels from chbuf); the sites near the origin (|x| or |x - y| below
+// kNearRadius, a list built on the host) are then recomputed in FP64 with the
+// FP64 table, where the Lambda subtraction cancels.
 constexpr int kKmThreads = 256;
 
-__global__ __launch_bounds__(kKmThreads) void kfield_mixed(
-    const float* __restrict__ table32, const double* __restrict__ table64,
-    const hlbl_coord* __restrict__ yvs, int n_unique, hlbl_geom global, double xunit,
-    size_t sites_per_wg, const float* __restrict__ chbuf, float* __restrict__ coef) {
-  __shared__ double lds64[kKfieldLds];
-  __shared__ float lds32[kKfieldLds];
-  const size_t V = geom_volume(global);
-  const size_t begin = (size_t)blockIdx.x * sites_per_wg;
-  const size_t end = begin + sites_per_wg < V ? begin + sites_per_wg : V;
-  if (begin >= end) return;
+// pair constants of each distinct yv: cst64[u][768] and an FP32 copy; one wave per u
+__global__ __launch_bounds__(64) void kfield_consts_win(const double* __restrict__ table64,
+                                                        const hlbl_coord* __restrict__ yvs,
+                                                        hlbl_geom global, double xunit,
+                                                        double* __restrict__ cst64,
+                                                        float* __restrict__ cst32) {
+  __shared__ TermLds tt;
+  __shared__ double w[kKwLen];
+  __shared__ double c[2 * qed::kNL];
+  stage_terms(c_terms32, &tt);
+  __syncthreads();
+  const int u = blockIdx.x;
   const hlbl_coord zero = {0, 0, 0, 0};
-  const double near2 = kNearRadius * kNearRadius;
-  for (int u = 0; u < n_unique; ++u) {
+  SiteGeom<double> g;
+  site_geom(0, global, global, zero, zero, yvs[u], xunit, g);
+  kfield_wave_constants<double, double>(table64, tt, g, w, c);
+  for (int i = threadIdx.x; i < 2 * qed::kNL; i += 64) {
+    cst64[(size_t)u * 2 * qed::kNL + i] = c[i];
+    cst32[(size_t)u * 2 * qed::kNL + i] = (float)c[i];
+  }
+}
+
+__global__ __launch_bounds__(kKmThreads) void kfield_mixed(
+    const float* __restrict__ table32, const hlbl_coord* __restrict__ yvs, int n_unique,
+    hlbl_geom global, double xunit, const float* __restrict__ cst32,
+    const float* __restrict__ chbuf, float* __restrict__ coef) {
+  __shared__ TermLds tt;
+  __shared__ float w[kKmThreads / 64][kKwLen];
+  stage_terms(c_terms32, &tt);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6;
+  const size_t V = geom_volume(global);
+  const size_t total = (size_t)n_unique * V;
+  const size_t nw = (size_t)gridDim.x * (kKmThreads / 64);
+  const hlbl_coord zero = {0, 0, 0, 0};
+  for (size_t i = (size_t)blockIdx.x * (kKmThreads / 64) + wave; i < total; i += nw) {
+    const int u = (int)(i / V);
+    const size_t site = i - (size_t)u * V;
     const hlbl_coord yv = yvs[u];
-    SiteGeom<double> g64;
-    SiteGeom<float> g32;
-    site_geom(0, global, global, zero, zero, yv, xunit, g64);
-    site_geom(0, global, global, zero, zero, yv, xunit, g32);
-    kfield_constants(table64, c_terms32, g64, lds64);
-    // FP32 copies of the pair constants
-    for (int t = threadIdx.x; t < 2 * qed::kNL; t += blockDim.x)
-      lds32[8 * qed::kNCh + 8 * qed::kNT + 6 * qed::kNL + t] = (float)lds64[8 * qed::kNCh + 8 * qed::kNT + 6 * qed::kNL + t];
-    __syncthreads();
-    float* cu = coef + (size_t)u * V * kCoefField;
-    for (size_t site = begin; site < end; ++site) {
-      site_geom(site, global, global, zero, zero, yv, xunit, g64);
-      const bool near = norm2(g64.xm) < near2 || norm2(g64.dm) < near2;
-      if (near) {
-        kfield_site<double, double, float>(table64, c_terms32, g64, ChannelRows::none(), lds64,
-                                           cu + site * kCoefField);
-      } else {
-        site_geom(site, global, global, zero, zero, yv, xunit, g32);
-        if (chbuf) {
-          // surrogate channel rows: blocks 0 (x,0) and 1 (0,x) shared by every yv,
-          // block 2+3u+j the yv-dependent (x,y), (y,x), (x,d); (0,d) is (0,x) at
-          // the site d = x - y
-          int crd[4], dsh[4];
-          site_coords(site, global, crd);
-          for (int r = 0; r < 4; ++r) dsh[r] = mod_pos(crd[r] - coord_dim(yv, r), geom_dim(global, r));
-          const size_t sd = (((size_t)dsh[0] * global.LX + dsh[1]) * global.LY + dsh[2]) * global.LZ + dsh[3];
-          ChannelRows rows;
-          rows.given = true;
-          rows.p[0] = chbuf + ((size_t)(2 + 3 * u + 0) * V + site) * qed::kNCh;
-          rows.p[1] = chbuf + ((size_t)0 * V + site) * qed::kNCh;
-          rows.p[2] = chbuf + ((size_t)(2 + 3 * u + 1) * V + site) * qed::kNCh;
-          rows.p[3] = chbuf + ((size_t)1 * V + site) * qed::kNCh;
-          rows.p[4] = chbuf + ((size_t)(2 + 3 * u + 2) * V + site) * qed::kNCh;
-          rows.p[5] = chbuf + ((size_t)1 * V + sd) * qed::kNCh;
-          kfield_site<float, float, float>(table32, c_terms32, g32, rows, lds32, cu + site * kCoefField);
-        } else {
-          kfield_site<float, float, float>(table32, c_terms32, g32, ChannelRows::none(), lds32,
-                                           cu + site * kCoefField);
-        }
-      }
-      __syncthreads();
+    SiteGeom<float> g;
+    site_geom(site, global, global, zero, zero, yv, xunit, g);
+    ChannelRows rows = ChannelRows::none();
+    if (chbuf) {
+      // surrogate channel rows: blocks 0 (x,0) and 1 (0,x) shared by every yv,
+      // block 2+3u+j the yv-dependent (x,y), (y,x), (x,d); (0,d) is (0,x) at the
+      // site d = x - y
+      int crd[4], dsh[4];
+      site_coords(site, global, crd);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) dsh[r] = mod_pos(crd[r] - coord_dim(yv, r), geom_dim(global, r));
+      const size_t sd = (((size_t)dsh[0] * global.LX + dsh[1]) * global.LY + dsh[2]) * global.LZ + dsh[3];
+      rows.given = true;
+      rows.p[0] = chbuf + ((size_t)(2 + 3 * u + 0) * V + site) * qed::kNCh;
+      rows.p[1] = chbuf + ((size_t)0 * V + site) * qed::kNCh;
+      rows.p[2] = chbuf + ((size_t)(2 + 3 * u + 1) * V + site) * qed::kNCh;
+      rows.p[3] = chbuf + ((size_t)1 * V + site) * qed::kNCh;
+      rows.p[4] = chbuf + ((size_t)(2 + 3 * u + 2) * V + site) * qed::kNCh;
+      rows.p[5] = chbuf + ((size_t)1 * V + sd) * qed::kNCh;
     }
-    __syncthreads();
+    kfield_wave<float, float, float>(table32, tt, g, rows, cst32 + (size_t)u * 2 * qed::kNL, w[wave],
+                                     coef + i * kCoefField);
+  }
+}
+
+// FP64 recomputation of the near sites. Candidates: for each distinct yv u, the
+// sites within box[r] lattice units of the origin (which = 0) and of y (which =
+// 1) in every direction; a candidate with |x| or |x - y| below kNearRadius is
+// recomputed (a site in both boxes twice, with the same result).
+__global__ __launch_bounds__(kKmThreads) void kfield_near(
+    const double* __restrict__ table64, const hlbl_coord* __restrict__ yvs, int n_unique,
+    hlbl_geom global, double xunit, const double* __restrict__ cst64, hlbl_coord box,
+    float* __restrict__ coef) {
+  __shared__ TermLds tt;
+  __shared__ double w[kKmThreads / 64][kKwLen];
+  stage_terms(c_terms32, &tt);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6;
+  const size_t V = geom_volume(global);
+  const int side[4] = {2 * box.t + 1, 2 * box.x + 1, 2 * box.y + 1, 2 * box.z + 1};
+  const size_t per_box = (size_t)side[0] * side[1] * side[2] * side[3];
+  const size_t total = (size_t)n_unique * 2 * per_box;
+  const double near2 = kNearRadius * kNearRadius;
+  const hlbl_coord zero = {0, 0, 0, 0};
+  for (size_t i = (size_t)blockIdx.x * (kKmThreads / 64) + wave; i < total;
+       i += (size_t)gridDim.x * (kKmThreads / 64)) {
+    const int u = (int)(i / (2 * per_box));
+    const int which = (int)((i / per_box) & 1);
+    size_t rem = i % per_box;
+    const hlbl_coord yv = yvs[u];
+    int crd[4];
+    for (int r = 3; r >= 0; --r) {
+      const int off = (int)(rem % side[r]) - coord_dim(box, r);
+      rem /= side[r];
+      crd[r] = mod_pos((which ? coord_dim(yv, r) : 0) + off, geom_dim(global, r));
+    }
+    const size_t site = (((size_t)crd[0] * global.LX + crd[1]) * global.LY + crd[2]) * global.LZ + crd[3];
+    SiteGeom<double> g;
+    site_geom(site, global, global, zero, zero, yv, xunit, g);
+    if (!(norm2(g.xm) < near2 || norm2(g.dm) < near2)) continue;  // wave-uniform
+    kfield_wave<double, double, float>(table64, tt, g, ChannelRows::none(),
+                                       cst64 + (size_t)u * 2 * qed::kNL, w[wave],
+                                       coef + ((size_t)u * V + site) * kCoefField);
   }
 }
 
 // The surrogate's judge: at sample sites, the FP64 plain coefficients against
 // the field in use; err[u][k] = max |c - c_plain| / max |c_plain| for sample k.
-__global__ __launch_bounds__(kKmThreads) void judge_coef(
-    const double* __restrict__ table64, const hlbl_coord* __restrict__ yvs,
-    hlbl_geom global, double xunit, const size_t* __restrict__ samples, int n_samples,
-    const float* __restrict__ coef, double* __restrict__ err) {
-  __shared__ double lds64[kKfieldLds];
+// One wave per (sample, yv).
+__global__ __launch_bounds__(64) void judge_coef(
+    const double* __restrict__ table64, const hlbl_coord* __restrict__ yvs, hlbl_geom global,
+    double xunit, const double* __restrict__ cst64, const size_t* __restrict__ samples,
+    int n_samples, const float* __restrict__ coef, double* __restrict__ err) {
+  __shared__ TermLds tt;
+  __shared__ double w[kKwLen];
   __shared__ double ref[kCoefField];
-  __shared__ double red[2][kKmThreads];
-  const int u = blockIdx.y, k = blockIdx.x;
+  stage_terms(c_terms32, &tt);
+  __syncthreads();
+  const int u = blockIdx.y, k = blockIdx.x, lane = threadIdx.x;
   const size_t V = geom_volume(global);
   const hlbl_coord zero = {0, 0, 0, 0};
-  SiteGeom<double> g;
-  site_geom(0, global, global, zero, zero, yvs[u], xunit, g);
-  kfield_constants(table64, c_terms32, g, lds64);
   const size_t site = samples[k];
+  SiteGeom<double> g;
   site_geom(site, global, global, zero, zero, yvs[u], xunit, g);
-  kfield_site<double, double, double>(table64, c_terms32, g, ChannelRows::none(), lds64, ref);
-  __syncthreads();
+  kfield_wave<double, double, double>(table64, tt, g, ChannelRows::none(),
+                                      cst64 + (size_t)u * 2 * qed::kNL, w, ref);
   const float* c = coef + ((size_t)u * V + site) * kCoefField;
   double dmax = 0.0, rmax = 0.0;
-  for (int t = threadIdx.x; t < kCoefField; t += blockDim.x) {
+  for (int t = lane; t < kCoefField; t += 64) {  // the entries this lane wrote
     dmax = fmax(dmax, fabs((double)c[t] - ref[t]));
     rmax = fmax(rmax, fabs(ref[t]));
   }
-  red[0][threadIdx.x] = dmax;
-  red[1][threadIdx.x] = rmax;
-  __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) {
-      red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + o]);
-      red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + o]);
-    }
-    __syncthreads();
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    dmax = fmax(dmax, __shfl_xor(dmax, off));
+    rmax = fmax(rmax, __shfl_xor(rmax, off));
   }
-  if (threadIdx.x == 0) err[(size_t)u * n_samples + k] = red[1][0] > 0 ? red[0][0] / red[1][0] : red[0][0];
+  if (lane == 0) err[(size_t)u * n_samples + k] = rmax > 0 ? dmax / rmax : dmax;
 }
 
 // ---------------------------------------------------------------- pass 2 --
@@ -342,17 +385,28 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
   }
   // coefficient fields, one per distinct yv
   if (stage == 1) {
-    const int nwg = 4 * c->n_cu;
-    const size_t spw = div_up(V, (size_t)nwg);
-    const int n = (int)div_up(V, spw);
-    hipLaunchKernelGGL(kfield_mixed, dim3(n), dim3(kKmThreads), 0, st, c->qed->table32.p,
-                       c->qed->table64.p, w.d_yvs.p, nu, g, w.xunit, spw,
-                       w.form == 2 ? w.chbuf.p : nullptr, w.coef.p);
+    hipLaunchKernelGGL(kfield_consts_win, dim3(nu), dim3(64), 0, st, c->qed->table64.p, w.d_yvs.p, g,
+                       w.xunit, w.cst64.p, w.cst32.p);
+    check_hip(hipGetLastError(), "kfield_consts_win launch");
+    const int nwg = (int)std::min<size_t>(div_up((size_t)nu * V, kKmThreads / 64), (size_t)4 * c->n_cu);
+    hipLaunchKernelGGL(kfield_mixed, dim3(nwg), dim3(kKmThreads), 0, st, c->qed->table32.p,
+                       w.d_yvs.p, nu, g, w.xunit, w.cst32.p, w.form == 2 ? w.chbuf.p : nullptr,
+                       w.coef.p);
     check_hip(hipGetLastError(), "kfield_mixed launch");
+    // candidate boxes: ceil(kNearRadius / xunit) lattice units, at most the whole extent
+    hlbl_coord box;
+    const int hb = (int)std::ceil(kNearRadius / w.xunit);
+    box.t = std::min(hb, (g.T - 1) / 2);
+    box.x = std::min(hb, (g.LX - 1) / 2);
+    box.y = std::min(hb, (g.LY - 1) / 2);
+    box.z = std::min(hb, (g.LZ - 1) / 2);
+    hipLaunchKernelGGL(kfield_near, dim3((unsigned)c->n_cu), dim3(kKmThreads), 0, st, c->qed->table64.p,
+                       w.d_yvs.p, nu, g, w.xunit, w.cst64.p, box, w.coef.p);
+    check_hip(hipGetLastError(), "kfield_near launch");
     if (w.form == 2) {
-      hipLaunchKernelGGL(judge_coef, dim3(w.n_samples, nu), dim3(kKmThreads), 0, st,
-                         c->qed->table64.p, w.d_yvs.p, g, w.xunit, w.d_samples.p, w.n_samples,
-                         w.coef.p, w.judge.p);
+      hipLaunchKernelGGL(judge_coef, dim3(w.n_samples, nu), dim3(64), 0, st, c->qed->table64.p,
+                         w.d_yvs.p, g, w.xunit, w.cst64.p, w.d_samples.p, w.n_samples, w.coef.p,
+                         w.judge.p);
       check_hip(hipGetLastError(), "judge_coef launch");
     }
   }

@@ -109,6 +109,8 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
     p->gdz.ensure((size_t)n * 2 * 40 * 144 * 2);
     p->bmat.ensure((size_t)n * 2 * 288 * 48);
     p->coef.ensure((size_t)nu * V * 640);
+    p->cst64.ensure((size_t)nu * 2 * qed::kNL);
+    p->cst32.ensure((size_t)nu * 2 * qed::kNL);
     if (form == 2) p->chbuf.ensure((size_t)(2 + 3 * nu) * V * qed::kNCh);
     p->part2.ensure(window_part2_len(c, n));
     p->result.ensure((size_t)2 * n);
