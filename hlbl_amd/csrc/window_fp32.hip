@@ -188,16 +188,18 @@ __global__ __launch_bounds__(64) void judge_coef(
 //   C[(mu lam)][col] = Re sum_{r,i} q^{mu lam}_r[i] X_col[i][r],  q^{mu lam}_r = gamma_lam gamma_5 dxu_mu[r],
 // (cuda_lattice.cu:324-344, 403-476) as a 16 x 48 x 288 product on FP32 MFMA,
 // A = q (from dxu in LDS, gamma_lam gamma_5 as a per-lane signed permutation), B =
-// the pair's X (registers), then the kernel sum += sum coef * C. Waves: (flavour,
-// quarter of K); the sum over K quarters is linear in the coefficients, so every
-// wave dots its own partial C with them. Software pipeline over groups of 4
-// sites, one barrier per group: in step g the waves run the MFMAs of group g,
-// build dxu of group g+1 (second dxu buffer) and stream the propagators of group
-// g+2 into a staging buffer with direct global-to-LDS loads.
-constexpr int kW2Threads = 512;
-constexpr int kW2Waves = 8;
+// the pair's X (registers), then the kernel sum += sum coef * C.
+// Twelve waves with two roles, over groups of 4 sites and one barrier per group:
+//  - waves 0..7 (flavour, quarter of K) run the MFMAs of group g; the sum over K
+//    quarters is linear in the coefficients, so every wave dots its own partial C
+//    with them;
+//  - waves 8..11 build dxu of group g+1 (second dxu buffer) and stream the
+//    propagators of group g+2 into a staging buffer with direct global-to-LDS loads.
+constexpr int kW2Threads = 768;
+constexpr int kW2Waves = 8;               // MFMA waves (partials per workgroup)
+constexpr int kW2Prod = 4;                // producer waves
 constexpr int kW2Sites = 4;
-constexpr int kW2KSteps = 18;             // k-steps (of 4) per wave: a quarter of 288
+constexpr int kW2KSteps = 18;             // k-steps (of 4) per MFMA wave: a quarter of 288
 constexpr int kW2RowLen = kW2Sites * 24;  // doubles of one staged spinor row (4 sites)
 constexpr int kW2RowPad = kW2RowLen + 2;  // row stride in LDS: rows start 4 banks apart
 constexpr int kW2BufLen = 48 * kW2RowPad; // doubles per staging buffer: (S/Y, f, a) rows
@@ -208,17 +210,17 @@ constexpr int kW2DxuLen = kW2Sites * 2 * 4 * 288;  // floats per dxu buffer [sit
 __device__ inline void barrier_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n s_barrier" ::: "memory"); }
 __device__ inline void barrier_all() { asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n s_barrier" ::: "memory"); }
 
-// Issue the direct loads of sites [z0, z0 + 4) of pair wp into buf (FP64, rows
-// (S/Y, f, a) of 96 doubles at stride kW2RowPad): one load of 48 lanes x 16 bytes
-// per row. Sites at or beyond V are read from site V-1; their coefficients are
-// masked.
-__device__ inline void w2_issue(const WinPair& wp, size_t V, size_t z0, double* buf) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// Producer wave p issues the direct loads of sites [z0, z0 + 4) of pair wp into
+// buf (FP64, rows (S/Y, f, a) of 96 doubles at stride kW2RowPad): one load of 48
+// lanes x 16 bytes per row, rows p, p + 4, ... Sites at or beyond V are read from
+// site V-1; their coefficients are masked.
+__device__ inline void w2_issue(int p, const WinPair& wp, size_t V, size_t z0, double* buf) {
+  const int lane = threadIdx.x & 63;
   if (lane >= kW2RowLen / 2) return;
   const int within = 2 * lane;
   size_t z = z0 + within / 24;
   z = z < V ? z : V - 1;
-  for (int row = wave; row < 48; row += kW2Waves) {
+  for (int row = p; row < 48; row += kW2Prod) {
     const int bl = row / 24, ff = (row / 12) & 1, a = row % 12;
     const double* src = (bl ? wp.Y : wp.S) + (((size_t)(ff * 12 + a)) * V + z) * 24 + within % 24;
     __builtin_amdgcn_global_load_lds((const void*)src,
@@ -226,9 +228,9 @@ __device__ inline void w2_issue(const WinPair& wp, size_t V, size_t z0, double* 
   }
 }
 
-// dxu of both flavours of a staged group: task (site, f2, r, a)
-__device__ inline void w2_dxu(const double* buf, float* dxu) {
-  for (int t = threadIdx.x; t < kW2Sites * 288; t += kW2Threads) {
+// dxu of both flavours of a staged group by the producer lanes: task (site, f2, r, a)
+__device__ inline void w2_dxu(int ptid, const double* buf, float* dxu) {
+  for (int t = ptid; t < kW2Sites * 288; t += 64 * kW2Prod) {
     const int zi = t / 288, f2 = (t / 144) & 1, r = (t / 12) % 12, a = t % 12;
     const double* sa = buf + ((1 - f2) * 12 + a) * kW2RowPad + zi * 24;
     const double* ur = buf + (24 + f2 * 12 + r) * kW2RowPad + zi * 24;
@@ -244,7 +246,8 @@ __global__ __launch_bounds__(kW2Threads) void pass2_window(
   __shared__ float sDxu[2][kW2DxuLen];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int f = wave >> 2, kq = wave & 3;
+  const bool producer = wave >= kW2Waves;
+  const int pw = wave - kW2Waves;  // producer index
   const size_t V = geom_volume(global);
   const size_t begin = (size_t)blockIdx.x * sites_per_wg;
   const size_t end = begin + sites_per_wg < V ? begin + sites_per_wg : V;
@@ -254,6 +257,25 @@ __global__ __launch_bounds__(kW2Threads) void pass2_window(
   auto group_pair = [&](int it) { return it / ng; };
   auto group_z0 = [&](int it) { return begin + (size_t)(it - (it / ng) * ng) * kW2Sites; };
 
+  // prologue: group 0 staged and its dxu built, group 1 in flight
+  if (producer) w2_issue(pw, pairs[0], V, begin, sBuf[0]);
+  barrier_all();
+  if (producer) {
+    w2_dxu(tid - 64 * kW2Waves, sBuf[0], sDxu[0]);
+    if (total > 1) w2_issue(pw, pairs[group_pair(1)], V, group_z0(1), sBuf[1]);
+  }
+
+  if (producer) {
+    for (int it = 0; it < total; ++it) {
+      barrier_all();  // dxu of it ready, staging of it+1 landed, buffers of it-1 free
+      if (it + 2 < total) w2_issue(pw, pairs[group_pair(it + 2)], V, group_z0(it + 2), sBuf[it & 1]);
+      if (it + 1 < total) w2_dxu(tid - 64 * kW2Waves, sBuf[(it + 1) & 1], sDxu[(it + 1) & 1]);
+    }
+    return;
+  }
+
+  // MFMA waves
+  const int f = wave >> 2, kq = wave & 3;
   // A fragments: row (mu, lam) = lane%16, K index r*24 + j, j = 4*jq + lane/16
   const int arow = lane & 15, amu = arow >> 2, alam = arow & 3;
   int aoff[6];
@@ -266,13 +288,6 @@ __global__ __launch_bounds__(kW2Threads) void pass2_window(
     asg[jq] = (float)(gamma_sign24(4, pj) * gamma_sign24(alam, j));
   }
   const bool c2on = (lane & 15) < kCoefUsed - 32;
-
-  // prologue: group 0 staged and its dxu built, group 1 in flight
-  w2_issue(pairs[0], V, begin, sBuf[0]);
-  barrier_all();
-  w2_dxu(sBuf[0], sDxu[0]);
-  if (total > 1) w2_issue(pairs[group_pair(1)], V, group_z0(1), sBuf[1]);
-
   float breg[3][kW2KSteps];
   Kahan acc;
   for (int it = 0; it < total; ++it) {
@@ -289,18 +304,14 @@ __global__ __launch_bounds__(kW2Threads) void pass2_window(
                            16 * cb + (lane & 15)];
       acc = Kahan();
     }
-    // dxu of group it ready, staging of group it+1 landed, buffers of it-1 free
     barrier_all();
-    if (it + 2 < total) w2_issue(pairs[group_pair(it + 2)], V, group_z0(it + 2), sBuf[it & 1]);
-    if (it + 1 < total) w2_dxu(sBuf[(it + 1) & 1], sDxu[(it + 1) & 1]);
-    // MFMA over the group's sites
     const float* dg = sDxu[it & 1];
     // coordinates of z0, stepped along the site order; relative to gsx (inside
     // the lattice) for the coefficient field
     int crd[4];
     site_coords(z0, global, crd);
     float v = 0.f;
-#pragma unroll
+#pragma unroll 1
     for (int zi = 0; zi < kW2Sites; ++zi) {
       const size_t z = z0 + zi;
       if (z >= end) break;
