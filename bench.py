@@ -86,6 +86,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 1 GiB device buffer three times (reads and writes 1 GiB "
+                         "each): the reference point for the HBM byte counters, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
@@ -95,6 +98,15 @@ def main():
     import numpy as np
     import torch
     import hlbl_amd as H
+
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 8, "copies": 3}), flush=True)
+        return
 
     lat = H.Lattice(DIMS)
     qed = H.QedKernel(0)
