@@ -123,13 +123,24 @@ __global__ void k_gelu_back(const float* __restrict__ z, const float* __restrict
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dz[i] = da[i] * gelu_grad(z[i]);
 }
-// db[j] = sum_b dz[b][j], fixed order
-__global__ void k_colsum(const float* __restrict__ dz, int B, int N, float* __restrict__ db) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= N) return;
+// db[j] = sum_b dz[b][j] in a fixed order: a workgroup takes 32 columns, its 8
+// row groups sum rows g, g+8, ... (coalesced along j), then the 8 partials are
+// added in order.
+__global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ dz, int B, int N,
+                                                float* __restrict__ db) {
+  __shared__ float part[8][32];
+  const int c = threadIdx.x & 31, g = threadIdx.x >> 5;
+  const int j = blockIdx.x * 32 + c;
   float s = 0.f;
-  for (int b = 0; b < B; ++b) s += dz[(size_t)b * N + j];
-  db[j] = s;
+  if (j < N)
+    for (int b = g; b < B; b += 8) s += dz[(size_t)b * N + j];
+  part[g][c] = s;
+  __syncthreads();
+  if (g == 0 && j < N) {
+    float t = 0.f;
+    for (int q = 0; q < 8; ++q) t += part[q][c];
+    db[j] = t;
+  }
 }
 // dout = 2 (out - t) / (B*C); per-row loss partials (FP64)
 __global__ void k_mse(const float* __restrict__ out, const float* __restrict__ t, int B, int Cn,
@@ -211,7 +222,7 @@ double mlp_backward(const float* wq, const float* x, const float* target, int B,
     // dW[j][k] = sum_b dz[b][j] a_in[b][k]
     gemm(L.out[l], L.in[l], B, wk.dz.p, 1, L.out[l], a_in, L.in[l], 1, grad + L.w[l], L.in[l],
          nullptr, 0, st);
-    hipLaunchKernelGGL(k_colsum, dim3((unsigned)div_up(L.out[l], 128)), dim3(128), 0, st, wk.dz.p, B,
+    hipLaunchKernelGGL(k_colsum, dim3((unsigned)div_up(L.out[l], 32)), dim3(256), 0, st, wk.dz.p, B,
                        L.out[l], grad + L.b[l]);
     if (l > 0) {
       // da_in[b][k] = sum_j dz[b][j] W[j][k]

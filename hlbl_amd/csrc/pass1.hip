@@ -48,10 +48,12 @@ __global__ __launch_bounds__(kPlThreads, 2) void pass1_planes(
     const WinPair* __restrict__ pairs, WinPair single, hlbl_geom global, hlbl_geom local,
     hlbl_coord proc, int flav_mask, T* __restrict__ part) {
   __shared__ T sh[kPlSites][24][24];  // [site][12 rows S_f[a], 12 rows Y_{1-f}[b]][24]
-  const int f = blockIdx.y;
+  // blockIdx.x = pair * 2 + flavour (fastest, so that the pairs of a window that
+  // share a block read its plane close together in time), blockIdx.y = plane
+  const int ip = blockIdx.x >> 1, f = blockIdx.x & 1;
   if (!((flav_mask >> f) & 1)) return;
-  const WinPair wp = pairs ? pairs[blockIdx.z] : single;
-  const int plane = blockIdx.x;
+  const WinPair wp = pairs ? pairs[ip] : single;
+  const int plane = blockIdx.y;
   const int LYZ = local.LY * local.LZ;
   const size_t V = geom_volume(local);
   const size_t base = (size_t)plane * LYZ;
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(kPlThreads, 2) void pass1_planes(
   }
   __syncthreads();
   if (!half) {
-    T* out = part + (((size_t)blockIdx.z * 2 + f) * gridDim.x + plane) * kPlPart;
+    T* out = part + (((size_t)ip * 2 + f) * gridDim.y + plane) * kPlPart;
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
@@ -216,7 +218,7 @@ void launch_pass1_planes(const WinPair* d_pairs, const WinPair& single, int n_pa
                          int flav_mask, T* part, double* red1, double* xi, hipStream_t st) {
   if (n_pairs <= 0) return;
   const int np = n_planes(local), nc = n_chunks(local);
-  hipLaunchKernelGGL(pass1_planes<T>, dim3(np, 2, n_pairs), dim3(kPlThreads), 0, st, d_pairs,
+  hipLaunchKernelGGL(pass1_planes<T>, dim3(2 * n_pairs, np), dim3(kPlThreads), 0, st, d_pairs,
                      single, global, local, proc, flav_mask, part);
   check_hip(hipGetLastError(), "pass1_planes launch");
   hipLaunchKernelGGL(reduce_planes1<T>, dim3((unsigned)div_up(kXiReal, 256), nc, 2 * n_pairs),

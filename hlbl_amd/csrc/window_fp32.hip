@@ -192,49 +192,94 @@ __global__ __launch_bounds__(64) void judge_coef(
 // Twelve waves with two roles, over groups of 4 sites and one barrier per group:
 //  - waves 0..7 (flavour, quarter of K) run the MFMAs of group g; the sum over K
 //    quarters is linear in the coefficients, so every wave dots its own partial C
-//    with them;
-//  - waves 8..11 build dxu of group g+1 (second dxu buffer) and stream the
-//    propagators of group g+2 into a staging buffer with direct global-to-LDS loads.
+//    with them (read from LDS);
+//  - waves 8..11 (producers) store group g+2's spinors (fetched into registers
+//    during the previous step) as FP32, stage group g+1's coefficients into LDS
+//    (direct loads), build dxu of group g+1 from its FP32 spinors, and fetch
+//    group g+3's spinors.
 constexpr int kW2Threads = 768;
 constexpr int kW2Waves = 8;               // MFMA waves (partials per workgroup)
-constexpr int kW2Prod = 4;                // producer waves
+constexpr int kW2Prod = 4;                // producer waves, one per site of a group
 constexpr int kW2Sites = 4;
 constexpr int kW2KSteps = 18;             // k-steps (of 4) per MFMA wave: a quarter of 288
-constexpr int kW2RowLen = kW2Sites * 24;  // doubles of one staged spinor row (4 sites)
-constexpr int kW2RowPad = kW2RowLen + 2;  // row stride in LDS: rows start 4 banks apart
-constexpr int kW2BufLen = 48 * kW2RowPad; // doubles per staging buffer: (S/Y, f, a) rows
-constexpr int kW2DxuLen = kW2Sites * 2 * 4 * 288;  // floats per dxu buffer [site][f][mu][r][24]
+constexpr int kW2RowLen = kW2Sites * 24;  // values of one staged spinor row (4 sites)
+constexpr int kW2RowPad = kW2RowLen + 4;  // FP32 row stride: rows start 36 banks apart
+constexpr int kW2BufLen = 48 * kW2RowPad; // floats per staging buffer: (S/Y, f, a) rows
+constexpr int kW2DxuRow = 304;            // floats per (site, f, mu) row of dxu: 288 + pad, so
+                                          // that an A fragment's 16 rows x 4 k hit distinct banks
+constexpr int kW2DxuLen = kW2Sites * 2 * 4 * kW2DxuRow;  // floats per dxu buffer
+constexpr int kW2CoefRow = 44;            // staged coefficient row (40 used): conflict-free reads
+constexpr int kW2CoefSite = 16 * kW2CoefRow;
+constexpr int kW2Fetch = 48 * kW2RowLen / 2 / (64 * kW2Prod);  // double2 per producer lane: 9
+static_assert(kW2Fetch == 9, "barrier_vm_fetch leaves exactly 9 loads in flight");
 
-// barrier that waits for this wave's LDS operations and, with vm, for its
-// global loads (direct loads into LDS included)
-__device__ inline void barrier_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n s_barrier" ::: "memory"); }
+// barrier that waits for this wave's LDS operations and for its global loads
+// (direct loads into LDS included)
 __device__ inline void barrier_all() { asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n s_barrier" ::: "memory"); }
+// the same, leaving the producer's last kW2Fetch (= 9) global loads in flight
+__device__ inline void barrier_vm_fetch() { asm volatile("s_waitcnt vmcnt(9) lgkmcnt(0)\n s_barrier" ::: "memory"); }
 
-// Producer wave p issues the direct loads of sites [z0, z0 + 4) of pair wp into
-// buf (FP64, rows (S/Y, f, a) of 96 doubles at stride kW2RowPad): one load of 48
-// lanes x 16 bytes per row, rows p, p + 4, ... Sites at or beyond V are read from
-// site V-1; their coefficients are masked.
-__device__ inline void w2_issue(int p, const WinPair& wp, size_t V, size_t z0, double* buf) {
-  const int lane = threadIdx.x & 63;
-  if (lane >= kW2RowLen / 2) return;
-  const int within = 2 * lane;
-  size_t z = z0 + within / 24;
-  z = z < V ? z : V - 1;
-  for (int row = p; row < 48; row += kW2Prod) {
+// Producer lane ptid fetches its 9 double2 of the spinors of sites [z0, z0 + 4)
+// of pair wp: element e2 = ptid + 256 k is double pair e2 % 48 of row e2 / 48,
+// rows (S/Y, f, a). Sites at or beyond V are read from site V-1 (masked later).
+__device__ inline void w2_fetch(int ptid, const WinPair& wp, size_t V, size_t z0, double2 (&r)[kW2Fetch]) {
+#pragma unroll
+  for (int k = 0; k < kW2Fetch; ++k) {
+    const int e2 = ptid + 64 * kW2Prod * k;
+    const int row = e2 / (kW2RowLen / 2), within = 2 * (e2 % (kW2RowLen / 2));
     const int bl = row / 24, ff = (row / 12) & 1, a = row % 12;
+    size_t z = z0 + within / 24;
+    z = z < V ? z : V - 1;
     const double* src = (bl ? wp.Y : wp.S) + (((size_t)(ff * 12 + a)) * V + z) * 24 + within % 24;
-    __builtin_amdgcn_global_load_lds((const void*)src,
-                                     (__attribute__((address_space(3))) void*)(buf + row * kW2RowPad), 16, 0, 0);
+    r[k] = *reinterpret_cast<const double2*>(src);
   }
 }
 
+// ... and stores them as FP32 into buf [48][kW2RowPad]
+__device__ inline void w2_store(int ptid, const double2 (&r)[kW2Fetch], float* buf) {
+#pragma unroll
+  for (int k = 0; k < kW2Fetch; ++k) {
+    const int e2 = ptid + 64 * kW2Prod * k;
+    const int row = e2 / (kW2RowLen / 2), within = 2 * (e2 % (kW2RowLen / 2));
+    float2 v;
+    v.x = (float)r[k].x;
+    v.y = (float)r[k].y;
+    *reinterpret_cast<float2*>(buf + row * kW2RowPad + within) = v;
+  }
+}
+
+// Producer p stages the 16 x 40 coefficients of site z0 + p (relative to gsx)
+// into dst + p * kW2CoefSite, rows kW2CoefRow apart, with direct loads: one load of
+// 10 lanes x 16 bytes per row.
+__device__ inline void w2_coef(int p, const WinPair& wp, hlbl_geom global, size_t V, size_t z0,
+                               size_t end, const float* __restrict__ coef, float* dst) {
+  const int lane = threadIdx.x & 63;
+  const size_t z = z0 + p;
+  if (z >= end) return;
+  int crd[4];
+  site_coords(z, global, crd);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    crd[r] -= coord_dim(wp.gsx, r);
+    crd[r] += crd[r] < 0 ? geom_dim(global, r) : 0;
+  }
+  const size_t rs = (((size_t)crd[0] * global.LX + crd[1]) * global.LY + crd[2]) * global.LZ + crd[3];
+  if (lane >= kCoefUsed / 4) return;
+  const float* src = coef + ((size_t)wp.yv_slot * V + rs) * kCoefField + 4 * lane;
+  for (int row = 0; row < 16; ++row)
+    __builtin_amdgcn_global_load_lds((const void*)(src + row * kCoefUsed),
+                                     (__attribute__((address_space(3))) void*)(dst + p * kW2CoefSite + row * kW2CoefRow),
+                                     16, 0, 0);
+}
+
 // dxu of both flavours of a staged group by the producer lanes: task (site, f2, r, a)
-__device__ inline void w2_dxu(int ptid, const double* buf, float* dxu) {
+__device__ inline void w2_dxu(int ptid, const float* buf, float* dxu) {
   for (int t = ptid; t < kW2Sites * 288; t += 64 * kW2Prod) {
     const int zi = t / 288, f2 = (t / 144) & 1, r = (t / 12) % 12, a = t % 12;
-    const double* sa = buf + ((1 - f2) * 12 + a) * kW2RowPad + zi * 24;
-    const double* ur = buf + (24 + f2 * 12 + r) * kW2RowPad + zi * 24;
-    dxu_site<double, float, float>(sa, ur, dxu + ((zi * 2 + f2) * 4) * 288 + r * 24 + 2 * a, 288);
+    const float* sa = buf + ((1 - f2) * 12 + a) * kW2RowPad + zi * 24;
+    const float* ur = buf + (24 + f2 * 12 + r) * kW2RowPad + zi * 24;
+    dxu_site_lean<float, float, float>(sa, ur, dxu + ((zi * 2 + f2) * 4) * kW2DxuRow + r * 24 + 2 * a,
+                                       kW2DxuRow);
   }
 }
 
@@ -242,12 +287,12 @@ __global__ __launch_bounds__(kW2Threads) void pass2_window(
     const WinPair* __restrict__ pairs, int n_pairs, const float* __restrict__ bmat,
     const float* __restrict__ coef, hlbl_geom global, size_t sites_per_wg, int nwg,
     double* __restrict__ part) {
-  __shared__ __attribute__((aligned(16))) double sBuf[2][kW2BufLen];
+  __shared__ __attribute__((aligned(16))) float sF[2][kW2BufLen];
   __shared__ float sDxu[2][kW2DxuLen];
+  __shared__ __attribute__((aligned(16))) float sCoef[2][kW2Sites * kW2CoefSite];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const bool producer = wave >= kW2Waves;
-  const int pw = wave - kW2Waves;  // producer index
   const size_t V = geom_volume(global);
   const size_t begin = (size_t)blockIdx.x * sites_per_wg;
   const size_t end = begin + sites_per_wg < V ? begin + sites_per_wg : V;
@@ -257,43 +302,59 @@ __global__ __launch_bounds__(kW2Threads) void pass2_window(
   auto group_pair = [&](int it) { return it / ng; };
   auto group_z0 = [&](int it) { return begin + (size_t)(it - (it / ng) * ng) * kW2Sites; };
 
-  // prologue: group 0 staged and its dxu built, group 1 in flight
-  if (producer) w2_issue(pw, pairs[0], V, begin, sBuf[0]);
-  barrier_all();
   if (producer) {
-    w2_dxu(tid - 64 * kW2Waves, sBuf[0], sDxu[0]);
-    if (total > 1) w2_issue(pw, pairs[group_pair(1)], V, group_z0(1), sBuf[1]);
-  }
-
-  if (producer) {
+    const int pw = wave - kW2Waves, ptid = tid - 64 * kW2Waves;
+    double2 r[kW2Fetch];
+    // prologue: spinors of groups 0 and 1 staged, coefficients of group 0 staged,
+    // group 2 in flight, dxu of group 0 built
+    w2_fetch(ptid, pairs[0], V, begin, r);
+    w2_store(ptid, r, sF[0]);
+    if (total > 1) {
+      w2_fetch(ptid, pairs[group_pair(1)], V, group_z0(1), r);
+      w2_store(ptid, r, sF[1]);
+    }
+    w2_coef(pw, pairs[0], global, V, begin, end, coef, sCoef[0]);
+    bool inflight = total > 2;
+    if (inflight) w2_fetch(ptid, pairs[group_pair(2)], V, group_z0(2), r);
+    // the barrier waits for the coefficient loads but lets the later fetch fly
+    if (inflight) barrier_vm_fetch(); else barrier_all();
+    w2_dxu(ptid, sF[0], sDxu[0]);
     for (int it = 0; it < total; ++it) {
-      barrier_all();  // dxu of it ready, staging of it+1 landed, buffers of it-1 free
-      if (it + 2 < total) w2_issue(pw, pairs[group_pair(it + 2)], V, group_z0(it + 2), sBuf[it & 1]);
-      if (it + 1 < total) w2_dxu(tid - 64 * kW2Waves, sBuf[(it + 1) & 1], sDxu[(it + 1) & 1]);
+      if (inflight) barrier_vm_fetch(); else barrier_all();
+      if (it + 2 < total) w2_store(ptid, r, sF[it & 1]);  // fetched during the last step
+      if (it + 1 < total) {
+        w2_coef(pw, pairs[group_pair(it + 1)], global, V, group_z0(it + 1), end, coef, sCoef[(it + 1) & 1]);
+        w2_dxu(ptid, sF[(it + 1) & 1], sDxu[(it + 1) & 1]);
+      }
+      inflight = it + 3 < total;
+      if (inflight) w2_fetch(ptid, pairs[group_pair(it + 3)], V, group_z0(it + 3), r);
     }
     return;
   }
 
   // MFMA waves
   const int f = wave >> 2, kq = wave & 3;
-  // A fragments: row (mu, lam) = lane%16, K index r*24 + j, j = 4*jq + lane/16
+  // A fragments: row (mu, lam) = lane%16, K index r*24 + j, j = 4*jq + lane/16,
+  // read at dxu offset perm(j) with a sign: packed 5 bits per jq (offsets) and one
+  // bit per jq (negate)
   const int arow = lane & 15, amu = arow >> 2, alam = arow & 3;
-  int aoff[6];
-  float asg[6];
+  unsigned aoffp = 0, asgp = 0;
 #pragma unroll
   for (int jq = 0; jq < 6; ++jq) {
     const int j = 4 * jq + (lane >> 4);
     const int pj = gamma_perm24(alam, j);
-    aoff[jq] = pj;
-    asg[jq] = (float)(gamma_sign24(4, pj) * gamma_sign24(alam, j));
+    aoffp |= (unsigned)pj << (5 * jq);
+    if (gamma_sign24(4, pj) * gamma_sign24(alam, j) < 0) asgp |= 1u << jq;
   }
   const bool c2on = (lane & 15) < kCoefUsed - 32;
+  // FP32 16x16x4 result: element i of lane l is row 4*(l/16) + i, column l%16
+  const int coff = (4 * (lane >> 4)) * kW2CoefRow + (lane & 15);
   float breg[3][kW2KSteps];
   Kahan acc;
+  barrier_all();  // the producers' prologue
   for (int it = 0; it < total; ++it) {
     const int ip = group_pair(it);
     const size_t z0 = group_z0(it);
-    const WinPair wp = pairs[ip];
     if (z0 == begin) {
       const float* bm = bmat + (size_t)ip * 2 * 288 * kCoefCols;
 #pragma unroll
@@ -306,50 +367,37 @@ __global__ __launch_bounds__(kW2Threads) void pass2_window(
     }
     barrier_all();
     const float* dg = sDxu[it & 1];
-    // coordinates of z0, stepped along the site order; relative to gsx (inside
-    // the lattice) for the coefficient field
-    int crd[4];
-    site_coords(z0, global, crd);
+    const float* cg = sCoef[it & 1];
     float v = 0.f;
 #pragma unroll 1
     for (int zi = 0; zi < kW2Sites; ++zi) {
-      const size_t z = z0 + zi;
-      if (z >= end) break;
-      if (zi > 0 && ++crd[3] == global.LZ) {
-        crd[3] = 0;
-        if (++crd[2] == global.LY) {
-          crd[2] = 0;
-          if (++crd[1] == global.LX) {
-            crd[1] = 0;
-            ++crd[0];
-          }
-        }
-      }
-      int rel[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        rel[r] = crd[r] - coord_dim(wp.gsx, r);
-        rel[r] += rel[r] < 0 ? geom_dim(global, r) : 0;
-      }
-      const size_t rs = (((size_t)rel[0] * global.LX + rel[1]) * global.LY + rel[2]) * global.LZ + rel[3];
-      // FP32 16x16x4 result: element i of lane l is row 4*(l/16) + i, column l%16
-      const float* cs = coef + ((size_t)wp.yv_slot * V + rs) * kCoefField + (4 * (lane >> 4)) * kCoefUsed + (lane & 15);
+      if (z0 + zi >= end) break;
+      // this site's coefficients (FP32 16x16x4 result: element i of lane l is row
+      // 4*(l/16) + i, column l%16)
+      const float* cs = cg + zi * kW2CoefSite + coff;
       float cv[4][3];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        cv[i][0] = cs[i * kCoefUsed];
-        cv[i][1] = cs[i * kCoefUsed + 16];
-        cv[i][2] = c2on ? cs[i * kCoefUsed + 32] : 0.f;
+        cv[i][0] = cs[i * kW2CoefRow];
+        cv[i][1] = cs[i * kW2CoefRow + 16];
+        cv[i][2] = c2on ? cs[i * kW2CoefRow + 32] : 0.f;
       }
-      const float* dx = dg + ((zi * 2 + f) * 4 + amu) * 288;
+      // A values, read six at a time (one row r of dxu) ahead of their MFMAs
+      const float* dx = dg + ((zi * 2 + f) * 4 + amu) * kW2DxuRow + (kq * kW2KSteps / 6) * 24;
       f4v c0 = {0.f, 0.f, 0.f, 0.f}, c1 = c0, c2 = c0;
 #pragma unroll
-      for (int i = 0; i < kW2KSteps; ++i) {
-        const int st = kq * kW2KSteps + i;  // kq*18 is a multiple of 6: jq = i % 6
-        const float a = asg[i % 6] * dx[(st / 6) * 24 + aoff[i % 6]];
-        c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[0][i], c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[1][i], c1, 0, 0, 0);
-        c2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[2][i], c2, 0, 0, 0);
+      for (int rb = 0; rb < kW2KSteps / 6; ++rb) {  // kq*18 is a multiple of 6: jq = i % 6
+        float av[6];
+#pragma unroll
+        for (int jq = 0; jq < 6; ++jq) av[jq] = dx[rb * 24 + ((aoffp >> (5 * jq)) & 31)];
+#pragma unroll
+        for (int jq = 0; jq < 6; ++jq) {
+          const int i = 6 * rb + jq;
+          const float a = __uint_as_float(__float_as_uint(av[jq]) ^ (((asgp >> jq) & 1u) << 31));
+          c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[0][i], c0, 0, 0, 0);
+          c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[1][i], c1, 0, 0, 0);
+          c2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[2][i], c2, 0, 0, 0);
+        }
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i) v += c0[i] * cv[i][0] + c1[i] * cv[i][1] + c2[i] * cv[i][2];
