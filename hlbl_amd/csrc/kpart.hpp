@@ -144,6 +144,7 @@ constexpr int kKwLen = 32 + qed::kNT + 3 * qed::kNL + kCoefField + 48;
 // L0 sparse terms, staged in LDS by the workgroup
 struct TermLds {
   unsigned short term[qed::kNL][qed::kMaxTerms];
+  unsigned short order[qed::kNL];  // outputs by decreasing term count
   unsigned char nterm[qed::kNL];
 };
 
@@ -158,7 +159,10 @@ __device__ inline void stage_terms(const qed::TermTable& src, TermLds* dst) {
   const unsigned short* s = &src.term[0][0];
   unsigned short* d = &dst->term[0][0];
   for (int i = threadIdx.x; i < qed::kNL * qed::kMaxTerms; i += blockDim.x) d[i] = s[i];
-  for (int i = threadIdx.x; i < qed::kNL; i += blockDim.x) dst->nterm[i] = src.nterm[i];
+  for (int i = threadIdx.x; i < qed::kNL; i += blockDim.x) {
+    dst->nterm[i] = src.nterm[i];
+    dst->order[i] = src.order[i];
+  }
 }
 
 template <class R>
@@ -185,8 +189,12 @@ __device__ __forceinline__ void wave_l0(int e, const TT* __restrict__ table, con
 #pragma unroll
   for (int k = 0; k < qed::kNT / 64; ++k) T[64 * k + lane] = kp_tensor(e, 64 * k + lane, g, ch);
   wave_sync();
+  // outputs in order of decreasing term count, so the lanes of a round loop alike
 #pragma unroll 2
-  for (int o = lane; o < qed::kNL; o += 64) dst[o] = l0_lds(tt, o, T);
+  for (int t = lane; t < qed::kNL; t += 64) {
+    const int o = tt.order[t];
+    dst[o] = l0_lds(tt, o, T);
+  }
   wave_sync();
 }
 

@@ -66,6 +66,7 @@ constexpr int kMaxTerms = 9;
 struct TermTable {
   unsigned short term[kNL][kMaxTerms];
   unsigned char nterm[kNL];
+  unsigned short order[kNL];  // outputs sorted by decreasing nterm
 };
 // built on the host from gamma-matrix traces (qed_tables.cpp)
 void build_term_table(TermTable* out);

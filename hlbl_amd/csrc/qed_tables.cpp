@@ -2,6 +2,7 @@
 // and the weight-function table.
 #include <cmath>
 #include <complex>
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 #include <vector>
@@ -112,6 +113,11 @@ void build_term_table(TermTable* out) {
                 }
         }
   }
+  // outputs by decreasing term count (stable): the lanes of a wave that evaluate
+  // consecutive entries of `order` loop over similar term counts
+  for (int o = 0; o < kNL; ++o) out->order[o] = (unsigned short)o;
+  std::stable_sort(out->order, out->order + kNL,
+                   [&](unsigned short x, unsigned short y) { return out->nterm[x] > out->nterm[y]; });
 }
 
 // ---- weight functions ------------------------------------------------------
