@@ -132,14 +132,14 @@ struct ChannelRows {
 // A wave computes the 16 x 40 coefficients of one site in its own LDS area, with
 // no workgroup barrier: a wave's LDS operations complete in order, and
 // wave_sync() keeps the compiler from moving accesses across phases.
-//   ch[32]  channels of the current evaluation
+//   ch[6][32]  channels of the six evaluations, all fetched first (one latency)
 //   T[192]  its T tensors
 //   L[3][384]  L0 slots: 0 = (x, 0), 1 = the partner (0, x) / (0, d), 2 = the main
 //           evaluation, turned in place into K_m (each lane updates its own entries)
 //   C[640]  coefficients; the K_m scatter into the 24 G columns as they complete
 //           (one coefficient per K element), the H columns gather from K3 at the end
 //   D[48]   subtraction matrices of xm, ym, dm
-constexpr int kKwLen = 32 + qed::kNT + 3 * qed::kNL + kCoefField + 48;
+constexpr int kKwLen = 6 * 32 + qed::kNT + 3 * qed::kNL + kCoefField + 48;
 
 // L0 sparse terms, staged in LDS by the workgroup
 struct TermLds {
@@ -176,16 +176,28 @@ __device__ inline R l0_lds(const TermLds& tt, int o, const R* t) {
   return r;
 }
 
-// channels, T tensors and L0 of evaluation e (0..7) into dst[384]; the
-// evaluations 6, 7 (the pair constants) always interpolate the table
+// the channels of the six site evaluations into ch[6][32]: from the rows when
+// given, else interpolated in the table
 template <class R, class TT>
-__device__ __forceinline__ void wave_l0(int e, const TT* __restrict__ table, const TermLds& tt,
-                                        const SiteGeom<R>& g, const ChannelRows& rows, R* ch, R* T,
-                                        R* dst) {
+__device__ __forceinline__ void wave_channels(const TT* __restrict__ table, const SiteGeom<R>& g,
+                                              const ChannelRows& rows, R* ch) {
   const int lane = threadIdx.x & 63;
-  if (lane < qed::kNChUsed)
-    ch[lane] = (rows.given && e < 6) ? (R)rows.row(e)[lane] : kp_channel(e, lane, g, table);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int t = lane + 64 * k;
+    if (t < 6 * qed::kNChUsed) {
+      const int e = t / qed::kNChUsed, c = t - e * qed::kNChUsed;
+      ch[e * 32 + c] = rows.given ? (R)rows.row(e)[c] : kp_channel(e, c, g, table);
+    }
+  }
   wave_sync();
+}
+
+// T tensors and L0 of evaluation e (0..7) into dst[384] from its channels ch[32]
+template <class R>
+__device__ __forceinline__ void wave_l0_from(int e, const TermLds& tt, const SiteGeom<R>& g,
+                                             const R* ch, R* T, R* dst) {
+  const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int k = 0; k < qed::kNT / 64; ++k) T[64 * k + lane] = kp_tensor(e, 64 * k + lane, g, ch);
   wave_sync();
@@ -196,6 +208,16 @@ __device__ __forceinline__ void wave_l0(int e, const TT* __restrict__ table, con
     dst[o] = l0_lds(tt, o, T);
   }
   wave_sync();
+}
+
+// channels (interpolated), T tensors and L0 of evaluation e into dst[384]
+template <class R, class TT>
+__device__ __forceinline__ void wave_l0(int e, const TT* __restrict__ table, const TermLds& tt,
+                                        const SiteGeom<R>& g, R* ch, R* T, R* dst) {
+  const int lane = threadIdx.x & 63;
+  if (lane < qed::kNChUsed) ch[lane] = kp_channel(e, lane, g, table);
+  wave_sync();
+  wave_l0_from(e, tt, g, ch, T, dst);
 }
 
 // K_m (m = 0: L(x,y), 1: L(y,x), 2: L(x,d)) in place over `main`, and its G-column
@@ -242,7 +264,7 @@ __device__ __forceinline__ void kfield_wave(const TT* __restrict__ table, const 
                                             const R* cst, R* w, O* __restrict__ out) {
   const int lane = threadIdx.x & 63;
   R* ch = w;
-  R* T = ch + 32;
+  R* T = ch + 6 * 32;
   R* L = T + qed::kNT;
   R* C = L + 3 * qed::kNL;
   R* D = C + kCoefField;
@@ -250,18 +272,18 @@ __device__ __forceinline__ void kfield_wave(const TT* __restrict__ table, const 
   R* s1 = L + qed::kNL;
   R* s2 = L + 2 * qed::kNL;
   if (lane < 16) kp_dmat(lane, g, D);
-  wave_sync();
+  wave_channels(table, g, rows, ch);
   const R* Dx = D;
   const R* Dy = D + 16;
   const R* Dd = D + 32;
-  wave_l0(1, table, tt, g, rows, ch, T, s0);                   // (x, 0)
-  wave_l0(0, table, tt, g, rows, ch, T, s2);                   // (x, y)
+  wave_l0_from(1, tt, g, ch + 1 * 32, T, s0);                  // (x, 0)
+  wave_l0_from(0, tt, g, ch + 0 * 32, T, s2);                  // (x, y)
   wave_kernel(0, s2, cst, s0, Dx, Dy, g.in12, C);              // K1 = L(x, y)
-  wave_l0(3, table, tt, g, rows, ch, T, s1);                   // (0, x)
-  wave_l0(2, table, tt, g, rows, ch, T, s2);                   // (y, x)
+  wave_l0_from(3, tt, g, ch + 3 * 32, T, s1);                  // (0, x)
+  wave_l0_from(2, tt, g, ch + 2 * 32, T, s2);                  // (y, x)
   wave_kernel(1, s2, s1, cst + qed::kNL, Dy, Dx, g.in12, C);   // K2 = L(y, x)
-  wave_l0(5, table, tt, g, rows, ch, T, s1);                   // (0, d)
-  wave_l0(4, table, tt, g, rows, ch, T, s2);                   // (x, d)
+  wave_l0_from(5, tt, g, ch + 5 * 32, T, s1);                  // (0, d)
+  wave_l0_from(4, tt, g, ch + 4 * 32, T, s2);                  // (x, d)
   wave_kernel(2, s2, s1, s0, Dx, Dd, g.in3, C);                // K3 = L(x, d)
   for (int t = lane; t < 16 * 16; t += 64) {
     const int row = t >> 4, hc = t & 15;
@@ -279,9 +301,9 @@ template <class R, class TT>
 __device__ __forceinline__ void kfield_wave_constants(const TT* __restrict__ table, const TermLds& tt,
                                                       const SiteGeom<R>& g, R* w, R* cst) {
   R* ch = w;
-  R* T = ch + 32;
-  wave_l0(6, table, tt, g, ChannelRows::none(), ch, T, cst);
-  wave_l0(7, table, tt, g, ChannelRows::none(), ch, T, cst + qed::kNL);
+  R* T = ch + 6 * 32;
+  wave_l0(6, table, tt, g, ch, T, cst);
+  wave_l0(7, table, tt, g, ch, T, cst + qed::kNL);
 }
 
 }  // namespace hlbl

@@ -195,3 +195,47 @@ def test_run_chain_tiers_and_direct(qed):
     win_rows, _ = H.run_chain(con, sch, seed, xunit, residency=0, form=0, batch=1)
     assert win_rows == dev_rows
     con.close()
+
+
+def test_config_16x32_fp64_and_window(qed, oracle):
+    """The configured plain-path workload: 16^3x32, chain K=4, n=2, h=(+1,-1,+1,-1),
+    RandomProvider seed 1, a = 0.1 fm. The FP64 path against the oracle to 1e-10 on
+    the first, middle and last pairs of the chain; the FP32 window form (all 65 pairs
+    of window 1 in one call) against the FP64 path per pair."""
+    dims = (32, 16, 16, 16)
+    xunit = 105.658 / 197.3269804 * 0.1
+    lat = H.Lattice(dims)
+    con = H.Contractor(lat, qed)
+    sch = H.Schedule((1, -1, 1, -1), 2, 4)
+    pairs = sch.pairs()
+    assert len(pairs) == 275
+    prov = H.RandomProvider(lat, 1)
+    wrap = lambda v: tuple(a % L for a, L in zip(v, dims))
+    cache = {}
+
+    def block(pos):
+        if pos not in cache:
+            d = torch.empty(H.block_length(lat), dtype=torch.float64, device="cuda")
+            prov.compute(wrap(pos), d.data_ptr())
+            cache[pos] = d
+        return cache[pos]
+
+    for i in (0, len(pairs) // 2, len(pairs) - 1):
+        q, p = pairs[i]
+        S, Y = block(q), block(p)
+        yv = lat.site_map_diff(p, q)
+        got = np.array(con.kernel_sum(S.data_ptr(), Y.data_ptr(), wrap(q), yv, xunit))
+        want = oracle_pair(oracle, S.cpu().numpy(), Y.cpu().numpy(), dims, wrap(q), yv, xunit)
+        log("pair", i, q, p, "gpu", got, "oracle", want)
+        assert np.all(np.abs(got - want) <= 1e-10 * np.abs(want)), (i, got, want)
+    # window 1 in FP32 against FP64 per pair
+    src = sch.sources()
+    pw = [(q, p) for w, p, parts in src if w == 1 for q in parts]
+    assert len(pw) == 65
+    descs = [(block(q).data_ptr(), block(p).data_ptr(), wrap(q), lat.site_map_diff(p, q)) for q, p in pw]
+    f1 = np.array(con.window_kernel_sum(descs, xunit, form=1))
+    ref = np.array([con.kernel_sum(*d, xunit) for d in descs])
+    err = np.abs(f1 - ref) / np.abs(ref)
+    log("16^3x32 window 1, form 1: max rel err", err.max(), "median", np.median(err))
+    assert err.max() < 1e-3
+    con.close()
