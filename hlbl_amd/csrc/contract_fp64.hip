@@ -93,9 +93,7 @@ __global__ void form_gdz(const double* __restrict__ dz, double* __restrict__ gdz
   gdz[(size_t)i * 2 + 1] = im;
 }
 
-// B operand of the pass-2 traces: bmat[f][r*24 + j][col],
-//   j = 2i: X_re[i][r], j = 2i+1: -X_im[i][r], X = G[k][nu] (col = k*4+nu < 24) or
-//   H[s][nu] (col = 24 + s*4+nu < 40), 0 for the pad columns.
+// B operand of the pass-2 traces: bmat[f][r*24 + j][col] = bmat_value (pad columns 0)
 template <class B>
 __global__ void form_bmat(const double* __restrict__ gdz, B* __restrict__ bmat) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (f, kidx, col)
@@ -103,16 +101,7 @@ __global__ void form_bmat(const double* __restrict__ gdz, B* __restrict__ bmat) 
   gdz += (size_t)blockIdx.y * kGdzLen;
   bmat += (size_t)blockIdx.y * kBmatLen;
   const int col = i % kCoefCols, kidx = (i / kCoefCols) % 288, f = i / (288 * kCoefCols);
-  const int r = kidx / 24, jj = kidx % 24, ii = jj >> 1, part = jj & 1;
-  double v = 0.0;
-  if (col < 40) {
-    // gdz row j: 0..5 dzu k, 6..9 dzsu s
-    const int jrow = col < 24 ? (col >> 2) : 6 + ((col - 24) >> 2);
-    const int nu = col & 3;
-    const double* x = gdz + (((((size_t)f * 10 + jrow) * 4 + nu) * 12 + ii) * 12 + r) * 2;
-    v = part ? -x[1] : x[0];
-  }
-  bmat[i] = (B)v;
+  bmat[i] = (B)(col < 40 ? bmat_value(gdz, f, kidx, col) : 0.0);
 }
 
 // ---------------------------------------------------------------- pass 2 --

@@ -102,6 +102,52 @@ __device__ inline void dxu_site_lean(const R* __restrict__ sa, const R* __restri
   }
 }
 
+// dxu_mu[r][a] of dxu_site, mu = 0..3, returned in (re[mu], im[mu]); the sink
+// spins unrolled, so the phases i^q are constants.
+template <class R, class A>
+__device__ inline void dxu_vals(const R* __restrict__ sa, const R* __restrict__ ur, A (&re)[4],
+                                A (&im)[4]) {
+#pragma unroll
+  for (int mu = 0; mu < 4; ++mu) re[mu] = im[mu] = A(0);
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int s0 = s ^ 2, s1 = 3 - s;
+    A x0r = 0, x0i = 0, x1r = 0, x1i = 0;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+      const A vr = ur[2 * (3 * s + cc)], vi = ur[2 * (3 * s + cc) + 1];
+      const A t0r = sa[2 * (3 * s0 + cc)], t0i = sa[2 * (3 * s0 + cc) + 1];
+      const A t1r = sa[2 * (3 * s1 + cc)], t1i = sa[2 * (3 * s1 + cc) + 1];
+      x0r += t0r * vr + t0i * vi;
+      x0i += t0r * vi - t0i * vr;
+      x1r += t1r * vr + t1i * vi;
+      x1i += t1r * vi - t1i * vr;
+    }
+    // dxu_mu = -sum_s eps_s i^{-ph_mu(s)} Xi[cls(mu)][s]: the sign -eps_s folded in
+    const bool neg = s < 2;
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+      const bool c0 = (mu == 0 || mu == 3);
+      A ar = c0 ? x0r : x1r, ai = c0 ? x0i : x1i;
+      mul_ipow(ar, ai, 4 - gamma_phase(mu, s));
+      re[mu] += neg ? -ar : ar;
+      im[mu] += neg ? -ai : ai;
+    }
+  }
+}
+
+// Element (kidx = r*24 + j, col < 40) of flavour f of the pass-2 B operand, from
+// gdz[f][10][4][12][12] (complex): j = 2i: X_re[i][r], j = 2i+1: -X_im[i][r],
+// X = G[k][nu] (col = k*4+nu < 24, gdz rows 0..5: dzu) or H[s][nu]
+// (col = 24 + s*4+nu, gdz rows 6..9: dzsu).
+HLBL_HD inline double bmat_value(const double* gdz, int f, int kidx, int col) {
+  const int r = kidx / 24, jj = kidx % 24, ii = jj >> 1, part = jj & 1;
+  const int jrow = col < 24 ? (col >> 2) : 6 + ((col - 24) >> 2);
+  const int nu = col & 3;
+  const double* x = gdz + (((((size_t)f * 10 + jrow) * 4 + nu) * 12 + ii) * 12 + r) * 2;
+  return part ? -x[1] : x[0];
+}
+
 // Kahan-compensated accumulation
 struct Kahan {
   double s = 0.0, c = 0.0;

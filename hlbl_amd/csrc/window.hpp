@@ -35,7 +35,8 @@ struct WindowPlan {
   PinBuf<hlbl_coord> h_yvs;
   DevBuf<float> part1;     // pass-1 plane partials
   DevBuf<double> red1, xi, dz, gdz;
-  DevBuf<float> bmat;
+  DevBuf<unsigned short> bsplit;  // pass-2 B operands, FP16 hi/lo in MFMA fragment order
+  DevBuf<int> bexp;               // their column exponents [pair][f][160]
   DevBuf<float> coef;      // [n_unique][V][640]
   DevBuf<double> cst64;    // pair constants L0(0,y), L0(y,0) per distinct yv [n_unique][768]
   DevBuf<float> cst32;
@@ -58,6 +59,8 @@ struct WindowPlan {
 void window_record(hlbl_contractor* c, WindowPlan& w, hipStream_t st);
 void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st);
 size_t window_part2_len(const hlbl_contractor* c, int n_pairs);
+size_t window_bsplit_len(int n_pairs);  // FP16 elements of the pass-2 B operands
+size_t window_bexp_len(int n_pairs);    // their column exponents
 
 // dz, gdz (FP64) and the B operands (FP64 into bmat64 or FP32 into bmat32) of
 // n_pairs reduced pass-1 sums xi[pair][...]

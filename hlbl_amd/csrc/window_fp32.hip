@@ -1,8 +1,8 @@
 // Window-batched mixed-precision contraction: all pairs of a window and both
 // flavours per call. Pass 1 (pass1.hip) is one launch over (plane, flavour,
 // pair), pass 2 one persistent launch over the pairs. Propagator products run in
-// FP32 (pass-2 traces on FP32 MFMA), the lattice sums and the per-pair sums are
-// FP64, combined with Kahan compensation. The coefficient field of the QED kernel depends only on
+// FP32 (pass-2 traces as scaled FP16 hi/lo products on MFMA, about 21 bits), the
+// lattice sums and the per-pair sums are FP64, combined with Kahan compensation. The coefficient field of the QED kernel depends only on
 // (x - gsx, yv), so it is built once per distinct yv of the window (at gsx = 0)
 // and read at the shifted site; sites near the origin (|x| or |x - y| below
 // kNearRadius) evaluate the kernel in FP64.
@@ -185,73 +185,186 @@ __global__ __launch_bounds__(64) void judge_coef(
 
 // ---------------------------------------------------------------- pass 2 --
 // Per site and flavour f the traces
-//   C[(mu lam)][col] = Re sum_{r,i} q^{mu lam}_r[i] X_col[i][r],  q^{mu lam}_r = gamma_lam gamma_5 dxu_mu[r],
-// (cuda_lattice.cu:324-344, 403-476) as a 16 x 48 x 288 product on FP32 MFMA,
-// A = q (from dxu in LDS, gamma_lam gamma_5 as a per-lane signed permutation), B =
-// the pair's X (registers), then the kernel sum += sum coef * C.
-// Twelve waves with two roles, over groups of 4 sites and one barrier per group:
-//  - waves 0..7 (flavour, quarter of K) run the MFMAs of group g; the sum over K
-//    quarters is linear in the coefficients, so every wave dots its own partial C
-//    with them (read from LDS);
-//  - waves 8..11 (producers) store group g+2's spinors (fetched into registers
-//    during the previous step) as FP32, stage group g+1's coefficients into LDS
-//    (direct loads), build dxu of group g+1 from its FP32 spinors, and fetch
-//    group g+3's spinors.
-constexpr int kW2Threads = 768;
-constexpr int kW2Waves = 8;               // MFMA waves (partials per workgroup)
-constexpr int kW2Prod = 4;                // producer waves, one per site of a group
-constexpr int kW2Sites = 4;
-constexpr int kW2KSteps = 18;             // k-steps (of 4) per MFMA wave: a quarter of 288
-constexpr int kW2RowLen = kW2Sites * 24;  // values of one staged spinor row (4 sites)
-constexpr int kW2RowPad = kW2RowLen + 4;  // FP32 row stride: rows start 36 banks apart
-constexpr int kW2BufLen = 48 * kW2RowPad; // floats per staging buffer: (S/Y, f, a) rows
-constexpr int kW2DxuRow = 304;            // floats per (site, f, mu) row of dxu: 288 + pad, so
-                                          // that an A fragment's 16 rows x 4 k hit distinct banks
-constexpr int kW2DxuLen = kW2Sites * 2 * 4 * kW2DxuRow;  // floats per dxu buffer
-constexpr int kW2CoefRow = 44;            // staged coefficient row (40 used): conflict-free reads
-constexpr int kW2CoefSite = 16 * kW2CoefRow;
-constexpr int kW2Fetch = 48 * kW2RowLen / 2 / (64 * kW2Prod);  // double2 per producer lane: 9
-static_assert(kW2Fetch == 9, "barrier_vm_fetch leaves exactly 9 loads in flight");
+//   C[(mu lam)][col] = Re sum_{r,j} q^{mu lam}[r*24 + j] B_f[r*24 + j][col],
+//   q^{mu lam} = gamma_lam gamma_5 dxu_mu    (cuda_lattice.cu:324-344, 403-476),
+// and the kernel sum += sum coef[(mu lam)][col] C. gamma_lam gamma_5 acts on the 24
+// reals of a colour-spinor as a signed permutation j -> perm_lam(j), so it moves
+// from the rows of A onto the columns of B:
+//   C[(zi, mu)][(lam, col)] = sum_k dxu[zi][mu][k] Bg_f[k][(lam, col)],
+//   Bg_f[r*24 + perm_lam(j)][(lam, col)] = sign_lam(j) B_f[r*24 + j][col],
+// a 16 x 160 x 288 product per group of 4 sites (rows (site, mu)) and flavour.
+//
+// Precision: A and Bg are scaled by powers of two into FP16's range (A per site,
+// from the site's spinor maxima: |dxu| <= 24 max|S| max|Y|; Bg per column, from
+// the column maximum), split x = hi + lo in FP16 and multiplied as
+// C = hi.hi + hi.lo + lo.hi on v_mfma_f32_16x16x32_f16 (FP32 accumulation), then
+// unscaled exactly: every product keeps about 21 significant bits. The sum is FP32
+// per group of 4 sites and FP64 (Kahan) across groups.
+//
+// Work unit = (pair, flavour): the unit's Bg (hi and lo, 180 KB) lives in the
+// registers of 10 MFMA waves, one 16-column block each (72 VGPRs). Sixteen waves:
+//  - waves 0..9 run the 27 MFMAs of their column block for group g and dot the
+//    result with the coefficients;
+//  - waves 10..15 (producers) store group g+2's spinors (fetched into registers
+//    during the previous step) as FP32 with their per-site maxima, stage group
+//    g+1's coefficients (direct loads into LDS), build dxu of group g+1 (scaled,
+//    split hi/lo), fetch group g+3.
+constexpr int kS2Mfma = 10;                // MFMA waves = 16-column blocks of the 160
+constexpr int kS2Prod = 6;                 // producer waves
+constexpr int kS2Threads = 64 * (kS2Mfma + kS2Prod);
+constexpr int kS2Sites = 4;
+constexpr int kS2Chunks = 9;               // K = 288 in steps of 32
+constexpr int kS2RowLen = kS2Sites * 24;   // values of one staged spinor row (4 sites)
+constexpr int kS2RowPad = kS2RowLen + 4;   // FP32 row stride
+constexpr int kS2BufLen = 24 * kS2RowPad;  // rows S[1-f][a] (0..11), Y[f][r] (12..23)
+constexpr int kS2ARow = 296;               // FP16 per A row (288 + 8): 148 dwords, so that the
+                                           // 16-byte reads of 16 rows hit distinct banks
+constexpr int kS2ALen = 16 * kS2ARow;      // one of hi / lo
+constexpr int kS2CoefSite = 656;           // floats per staged site (640 + 16): 16 banks apart
+constexpr int kS2KeyLanes = 48;            // producer lanes per (block, site) key
+constexpr int kS2Fetch = 144 / kS2KeyLanes;  // double2 per producer lane: 3
+constexpr int kS2Tasks = kS2Sites * 144;   // dxu tasks (site, r, a) per group
+constexpr int kS2AMax = 14;                // |scaled value| < 2^14 (FP16 max 65504)
+constexpr int kWinGdzLen = 2 * 40 * 144 * 2;
+static_assert(kS2Fetch == 3, "barrier_vm3 leaves exactly 3 loads in flight");
+static_assert(8 * kS2KeyLanes == 64 * kS2Prod, "one key per 48 producer lanes");
+// Bg per (pair, flavour) in fragment order [block 10][chunk 9][hi, lo][lane 64][8]
+constexpr int kBsplitLen = kS2Mfma * kS2Chunks * 2 * 64 * 8;
+
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 
 // barrier that waits for this wave's LDS operations and for its global loads
 // (direct loads into LDS included)
 __device__ inline void barrier_all() { asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n s_barrier" ::: "memory"); }
-// the same, leaving the producer's last kW2Fetch (= 9) global loads in flight
-__device__ inline void barrier_vm_fetch() { asm volatile("s_waitcnt vmcnt(9) lgkmcnt(0)\n s_barrier" ::: "memory"); }
+// the same, leaving the producer's last kS2Fetch (= 3) global loads in flight
+__device__ inline void barrier_vm3() { asm volatile("s_waitcnt vmcnt(3) lgkmcnt(0)\n s_barrier" ::: "memory"); }
 
-// Producer lane ptid fetches its 9 double2 of the spinors of sites [z0, z0 + 4)
-// of pair wp: element e2 = ptid + 256 k is double pair e2 % 48 of row e2 / 48,
-// rows (S/Y, f, a). Sites at or beyond V are read from site V-1 (masked later).
-__device__ inline void w2_fetch(int ptid, const WinPair& wp, size_t V, size_t z0, double2 (&r)[kW2Fetch]) {
+__device__ inline unsigned short f16_bits(float x) {
+  const _Float16 h = (_Float16)x;  // nearest, ties to even
+  unsigned short u;
+  __builtin_memcpy(&u, &h, 2);
+  return u;
+}
+__device__ inline float f16_float(unsigned short u) {
+  _Float16 h;
+  __builtin_memcpy(&h, &u, 2);
+  return (float)h;
+}
+// x = hi + lo in FP16, packed (hi bits, lo bits)
+__device__ inline void f16_split(float x, unsigned short& hi, unsigned short& lo) {
+  hi = f16_bits(x);
+  lo = f16_bits(x - f16_float(hi));
+}
+
+// Column exponents of Bg: per (pair, flavour, col < 40), te = kS2AMax - E with
+// max_k |B_f[k][col]| = m 2^E, m in [0.5, 1) (0 for an all-zero column); the
+// permutation does not change a column's maximum, so the four lam share it:
+// bexp[pair][f][lam*40 + col].
+__global__ void bsplit_exp(const double* __restrict__ gdz, int* __restrict__ bexp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (f, col) of pair blockIdx.y
+  if (i >= 2 * 40) return;
+  const int f = i / 40, col = i % 40;
+  gdz += (size_t)blockIdx.y * kWinGdzLen;
+  double m = 0.0;
+  for (int k = 0; k < 288; ++k) m = fmax(m, fabs(bmat_value(gdz, f, k, col)));
+  int e = 0;
+  if (m > 0.0) frexp(m, &e);
+  int te = m > 0.0 ? kS2AMax - e : 0;
+  te = te < -120 ? -120 : (te > 120 ? 120 : te);
+  for (int lam = 0; lam < 4; ++lam) bexp[((size_t)blockIdx.y * 2 + f) * 160 + lam * 40 + col] = te;
+}
+
+// Bg of every (pair, flavour) of the window, scaled by 2^te of its column and split
+// hi/lo in FP16, in fragment order: element (block b, chunk c, hl, lane l, e) is
+// Bg[k = 32c + 8(l/16) + e][g = 16b + l%16], g = lam*40 + col (the B layout of
+// v_mfma_f32_16x16x32_f16).
+__global__ void form_bsplit(const double* __restrict__ gdz, const int* __restrict__ bexp,
+                            unsigned short* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (f, b, c, l, e) of pair blockIdx.y
+  if (i >= kBsplitLen) return;  // 2 flavours x kBsplitLen / 2 (hi and lo per thread)
+  const int e = i & 7, l = (i >> 3) & 63, c = (i >> 9) % kS2Chunks, b = (i / (512 * kS2Chunks)) % kS2Mfma,
+            f = i / (512 * kS2Chunks * kS2Mfma);
+  gdz += (size_t)blockIdx.y * kWinGdzLen;
+  const int k = 32 * c + 8 * (l >> 4) + e, g = 16 * b + (l & 15);
+  const int lam = g / 40, col = g % 40, r = k / 24, jp = k % 24;
+  int j = 0;
+  for (int jj = 0; jj < 24; ++jj) j = gamma_perm24(lam, jj) == jp ? jj : j;
+  const int te = bexp[((size_t)blockIdx.y * 2 + f) * 160 + g];
+  const double v = ldexp((gamma_sign24(4, jp) * gamma_sign24(lam, j)) * bmat_value(gdz, f, r * 24 + j, col), te);
+  const unsigned short hi = f16_bits((float)v);
+  const unsigned short lo = f16_bits((float)(v - (double)f16_float(hi)));
+  unsigned short* o = out + ((size_t)blockIdx.y * 2 + f) * kBsplitLen + (((size_t)b * kS2Chunks + c) * 2 * 64 + l) * 8 + e;
+  o[0] = hi;
+  o[64 * 8] = lo;
+}
+
+// a 64-bit value the wave holds in common, as a scalar (both halves zero-extended)
+__device__ inline size_t uniform_u64(size_t v) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v & 0xffffffffu));
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+  return ((size_t)hi << 32) | (size_t)lo;
+}
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const d2v gd2v;
+
+// Producer lane ptid serves key ptid / 48 = (isy, site) of the group [z0, z0 + 4):
+// isy = 0 the rows S[1-f][a], isy = 1 the rows Y[f][r] of site z0 + site. Its k-th
+// double pair is d = j + 48 k (j = ptid % 48) of the key's 144: row d / 12,
+// doubles 2 (d % 12) of the site. Sites at or beyond V are read from site V-1
+// (their coefficients are masked).
+__device__ inline void s2_fetch(int ptid, const WinPair* wp, int f, size_t V, size_t z0,
+                                d2v (&r)[kS2Fetch]) {
+  // both block pointers as wave-uniform values in the global address space (plain
+  // global loads: counted by vmcnt only)
+  const size_t ps = uniform_u64((size_t)wp->S), py = uniform_u64((size_t)wp->Y);
+  const int key = ptid / kS2KeyLanes, j = ptid % kS2KeyLanes;
+  const bool isy = key >= 4;
+  const int site = key & 3, ff = isy ? f : 1 - f;
+  size_t z = z0 + site;
+  z = z < V ? z : V - 1;
 #pragma unroll
-  for (int k = 0; k < kW2Fetch; ++k) {
-    const int e2 = ptid + 64 * kW2Prod * k;
-    const int row = e2 / (kW2RowLen / 2), within = 2 * (e2 % (kW2RowLen / 2));
-    const int bl = row / 24, ff = (row / 12) & 1, a = row % 12;
-    size_t z = z0 + within / 24;
-    z = z < V ? z : V - 1;
-    const double* src = (bl ? wp.Y : wp.S) + (((size_t)(ff * 12 + a)) * V + z) * 24 + within % 24;
-    r[k] = *reinterpret_cast<const double2*>(src);
+  for (int k = 0; k < kS2Fetch; ++k) {
+    const int d = j + kS2KeyLanes * k, row = d / 12, w2 = 2 * (d % 12);
+    const size_t off = ((((size_t)(ff * 12 + row)) * V + z) * 24 + w2) * sizeof(double);
+    r[k] = *reinterpret_cast<gd2v*>((isy ? py : ps) + off);
   }
 }
 
-// ... and stores them as FP32 into buf [48][kW2RowPad]
-__device__ inline void w2_store(int ptid, const double2 (&r)[kW2Fetch], float* buf) {
+// ... stores them as FP32 into buf [24][kS2RowPad] and folds their largest
+// magnitude into smax[key] (a wave spans two keys: one masked wave maximum each,
+// one LDS atomic per key and wave; non-negative floats order as their bits)
+__device__ inline void s2_store(int ptid, const d2v (&r)[kS2Fetch], float* buf, unsigned* smax) {
+  const int key = ptid / kS2KeyLanes, j = ptid % kS2KeyLanes;
+  const bool isy = key >= 4;
+  const int site = key & 3;
+  float m = 0.f;
 #pragma unroll
-  for (int k = 0; k < kW2Fetch; ++k) {
-    const int e2 = ptid + 64 * kW2Prod * k;
-    const int row = e2 / (kW2RowLen / 2), within = 2 * (e2 % (kW2RowLen / 2));
+  for (int k = 0; k < kS2Fetch; ++k) {
+    const int d = j + kS2KeyLanes * k, row = d / 12, w2 = 2 * (d % 12);
     float2 v;
     v.x = (float)r[k].x;
     v.y = (float)r[k].y;
-    *reinterpret_cast<float2*>(buf + row * kW2RowPad + within) = v;
+    m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+    *reinterpret_cast<float2*>(buf + ((isy ? 12 : 0) + row) * kS2RowPad + site * 24 + w2) = v;
+  }
+  const int w = ptid >> 6;
+  const int key0 = (64 * w) / kS2KeyLanes, key1 = (64 * w + 63) / kS2KeyLanes;
+  float m0 = key == key0 ? m : 0.f, m1 = key == key1 ? m : 0.f;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    m0 = fmaxf(m0, __shfl_xor(m0, off));
+    m1 = fmaxf(m1, __shfl_xor(m1, off));
+  }
+  if ((ptid & 63) == 0) {
+    atomicMax(smax + key0, __float_as_uint(m0));
+    if (key1 != key0) atomicMax(smax + key1, __float_as_uint(m1));
   }
 }
 
 // Producer p stages the 16 x 40 coefficients of site z0 + p (relative to gsx)
-// into dst + p * kW2CoefSite, rows kW2CoefRow apart, with direct loads: one load of
-// 10 lanes x 16 bytes per row.
-__device__ inline void w2_coef(int p, const WinPair& wp, hlbl_geom global, size_t V, size_t z0,
+// into dst + p * kS2CoefSite with direct loads (lane l: 16 bytes at 16 l).
+__device__ inline void s2_coef(int p, const WinPair& wp, hlbl_geom global, size_t V, size_t z0,
                                size_t end, const float* __restrict__ coef, float* dst) {
   const int lane = threadIdx.x & 63;
   const size_t z = z0 + p;
@@ -264,164 +377,177 @@ __device__ inline void w2_coef(int p, const WinPair& wp, hlbl_geom global, size_
     crd[r] += crd[r] < 0 ? geom_dim(global, r) : 0;
   }
   const size_t rs = (((size_t)crd[0] * global.LX + crd[1]) * global.LY + crd[2]) * global.LZ + crd[3];
-  if (lane >= kCoefUsed / 4) return;
   const float* src = coef + ((size_t)wp.yv_slot * V + rs) * kCoefField + 4 * lane;
-  for (int row = 0; row < 16; ++row)
-    __builtin_amdgcn_global_load_lds((const void*)(src + row * kCoefUsed),
-                                     (__attribute__((address_space(3))) void*)(dst + p * kW2CoefSite + row * kW2CoefRow),
-                                     16, 0, 0);
+  float* d = dst + p * kS2CoefSite;
+  // 640 floats = 2.5 loads of 64 lanes x 16 bytes
+  __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)d, 16, 0, 0);
+  __builtin_amdgcn_global_load_lds((const void*)(src + 256), (__attribute__((address_space(3))) void*)(d + 256),
+                                   16, 0, 0);
+  if (lane < 32)
+    __builtin_amdgcn_global_load_lds((const void*)(src + 512),
+                                     (__attribute__((address_space(3))) void*)(d + 512), 16, 0, 0);
 }
 
-// dxu of both flavours of a staged group by the producer lanes: task (site, f2, r, a)
-__device__ inline void w2_dxu(int ptid, const float* buf, float* dxu) {
-  for (int t = ptid; t < kW2Sites * 288; t += 64 * kW2Prod) {
-    const int zi = t / 288, f2 = (t / 144) & 1, r = (t / 12) % 12, a = t % 12;
-    const float* sa = buf + ((1 - f2) * 12 + a) * kW2RowPad + zi * 24;
-    const float* ur = buf + (24 + f2 * 12 + r) * kW2RowPad + zi * 24;
-    dxu_site_lean<float, float, float>(sa, ur, dxu + ((zi * 2 + f2) * 4) * kW2DxuRow + r * 24 + 2 * a,
-                                       kW2DxuRow);
+// Site exponent: with max|S| < 2^eS and max|Y| < 2^eY (eS, eY from the float bits of
+// the maxima), |dxu| < 24 2^(eS + eY), so dxu 2^se with se = 9 - eS - eY stays
+// below 24 2^9 < 2^kS2AMax.
+__device__ inline int site_exp(unsigned ms, unsigned my) {
+  const int es = (int)((ms >> 23) & 0xffu) - 126, ey = (int)((my >> 23) & 0xffu) - 126;
+  const int se = (kS2AMax - 5) - es - ey;
+  return se < -120 ? -120 : (se > 120 ? 120 : se);
+}
+
+// dxu of a staged group by the producer lanes, task (site zi, r, a): scaled by
+// 2^se of its site, split into FP16 (re, im) pairs into ah / al [(zi, mu)][kS2ARow]
+// at k = r*24 + 2a; the site exponents go to sse[zi]
+__device__ inline void s2_dxu(int ptid, const float* buf, const unsigned* smax, unsigned* ah, unsigned* al,
+                              int* sse) {
+  for (int t = ptid; t < kS2Tasks; t += 64 * kS2Prod) {
+    const int zi = t / 144, r = (t / 12) % 12, a = t % 12;
+    const int se = site_exp(smax[zi], smax[4 + zi]);
+    if (r == 0 && a == 0) sse[zi] = se;
+    float re[4], im[4];
+    dxu_vals<float, float>(buf + a * kS2RowPad + zi * 24, buf + (12 + r) * kS2RowPad + zi * 24, re, im);
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+      unsigned short hr, lr, hi, li;
+      f16_split(ldexpf(re[mu], se), hr, lr);
+      f16_split(ldexpf(im[mu], se), hi, li);
+      const int o = ((zi * 4 + mu) * kS2ARow + r * 24 + 2 * a) / 2;
+      ah[o] = (unsigned)hr | ((unsigned)hi << 16);
+      al[o] = (unsigned)lr | ((unsigned)li << 16);
+    }
   }
 }
 
-__global__ __launch_bounds__(kW2Threads) void pass2_window(
-    const WinPair* __restrict__ pairs, int n_pairs, const float* __restrict__ bmat,
-    const float* __restrict__ coef, hlbl_geom global, size_t sites_per_wg, int nwg,
-    double* __restrict__ part) {
-  __shared__ __attribute__((aligned(16))) float sF[2][kW2BufLen];
-  __shared__ float sDxu[2][kW2DxuLen];
-  __shared__ __attribute__((aligned(16))) float sCoef[2][kW2Sites * kW2CoefSite];
+__global__ __launch_bounds__(kS2Threads) void pass2_window(
+    const WinPair* __restrict__ pairs, int n_pairs, const unsigned short* __restrict__ bsplit,
+    const int* __restrict__ bexp, const float* __restrict__ coef, hlbl_geom global,
+    size_t sites_per_wg, int nwg, double* __restrict__ part) {
+  __shared__ __attribute__((aligned(16))) float sF[2][kS2BufLen];
+  __shared__ __attribute__((aligned(16))) unsigned short sA[2][2][kS2ALen];
+  __shared__ __attribute__((aligned(16))) float sCoef[2][kS2Sites * kS2CoefSite];
+  __shared__ unsigned sMax[3][8];  // per group g % 3: spinor maxima per (isy, site)
+  __shared__ int sSe[2][kS2Sites];  // per group g % 2: site exponents
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const bool producer = wave >= kW2Waves;
+  const bool producer = wave >= kS2Mfma;
   const size_t V = geom_volume(global);
   const size_t begin = (size_t)blockIdx.x * sites_per_wg;
   const size_t end = begin + sites_per_wg < V ? begin + sites_per_wg : V;
   if (begin >= end) return;
-  const int ng = (int)((end - begin + kW2Sites - 1) / kW2Sites);
-  const int total = ng * n_pairs;
-  auto group_pair = [&](int it) { return it / ng; };
-  auto group_z0 = [&](int it) { return begin + (size_t)(it - (it / ng) * ng) * kW2Sites; };
+  const int ng = (int)((end - begin + kS2Sites - 1) / kS2Sites);
+  const int total = ng * 2 * n_pairs;  // steps: (unit = (pair, flavour), group)
+  auto step_unit = [&](int it) { return it / ng; };
+  auto step_z0 = [&](int it) { return begin + (size_t)(it - (it / ng) * ng) * kS2Sites; };
+  if (tid < 24) (&sMax[0][0])[tid] = 0u;
+  __syncthreads();
 
   if (producer) {
-    const int pw = wave - kW2Waves, ptid = tid - 64 * kW2Waves;
-    double2 r[kW2Fetch];
-    // prologue: spinors of groups 0 and 1 staged, coefficients of group 0 staged,
-    // group 2 in flight, dxu of group 0 built
-    w2_fetch(ptid, pairs[0], V, begin, r);
-    w2_store(ptid, r, sF[0]);
+    const int pw = wave - kS2Mfma, ptid = tid - 64 * kS2Mfma;
+    auto fetch = [&](int it, d2v (&r)[kS2Fetch]) {
+      const int u = step_unit(it);
+      s2_fetch(ptid, pairs + (u >> 1), u & 1, V, step_z0(it), r);
+    };
+    auto stage_coef = [&](int it, float* dst) {
+      if (pw >= kS2Prod - kS2Sites) s2_coef(pw - (kS2Prod - kS2Sites), pairs[step_unit(it) >> 1], global, V,
+                                            step_z0(it), end, coef, dst);
+    };
+    auto dxu = [&](int it) {
+      s2_dxu(ptid, sF[it & 1], sMax[it % 3], reinterpret_cast<unsigned*>(sA[it & 1][0]),
+             reinterpret_cast<unsigned*>(sA[it & 1][1]), sSe[it & 1]);
+    };
+    d2v r[kS2Fetch];
+    // prologue: spinors of steps 0 and 1 staged, coefficients of step 0 staged,
+    // step 2 in flight, dxu of step 0 built
+    fetch(0, r);
+    s2_store(ptid, r, sF[0], sMax[0]);
     if (total > 1) {
-      w2_fetch(ptid, pairs[group_pair(1)], V, group_z0(1), r);
-      w2_store(ptid, r, sF[1]);
+      fetch(1, r);
+      s2_store(ptid, r, sF[1], sMax[1]);
     }
-    w2_coef(pw, pairs[0], global, V, begin, end, coef, sCoef[0]);
+    stage_coef(0, sCoef[0]);
     bool inflight = total > 2;
-    if (inflight) w2_fetch(ptid, pairs[group_pair(2)], V, group_z0(2), r);
+    if (inflight) fetch(2, r);
     // the barrier waits for the coefficient loads but lets the later fetch fly
-    if (inflight) barrier_vm_fetch(); else barrier_all();
-    w2_dxu(ptid, sF[0], sDxu[0]);
+    if (inflight) barrier_vm3(); else barrier_all();
+    dxu(0);
     for (int it = 0; it < total; ++it) {
-      if (inflight) barrier_vm_fetch(); else barrier_all();
-      if (it + 2 < total) w2_store(ptid, r, sF[it & 1]);  // fetched during the last step
+      if (inflight) barrier_vm3(); else barrier_all();
+      // maxima of group it were read by its dxu (last step): clear them for group it+3
+      if (ptid < 8) sMax[it % 3][ptid] = 0u;
+      if (it + 2 < total) s2_store(ptid, r, sF[it & 1], sMax[(it + 2) % 3]);  // fetched last step
       if (it + 1 < total) {
-        w2_coef(pw, pairs[group_pair(it + 1)], global, V, group_z0(it + 1), end, coef, sCoef[(it + 1) & 1]);
-        w2_dxu(ptid, sF[(it + 1) & 1], sDxu[(it + 1) & 1]);
+        stage_coef(it + 1, sCoef[(it + 1) & 1]);
+        dxu(it + 1);
       }
       inflight = it + 3 < total;
-      if (inflight) w2_fetch(ptid, pairs[group_pair(it + 3)], V, group_z0(it + 3), r);
+      if (inflight) fetch(it + 3, r);
     }
     return;
   }
 
-  // MFMA waves
-  const int f = wave >> 2, kq = wave & 3;
-  // A fragments: row (mu, lam) = lane%16, K index r*24 + j, j = 4*jq + lane/16,
-  // read at dxu offset perm(j) with a sign: packed 5 bits per jq (offsets) and one
-  // bit per jq (negate)
-  const int arow = lane & 15, amu = arow >> 2, alam = arow & 3;
-  unsigned aoffp = 0, asgp = 0;
-#pragma unroll
-  for (int jq = 0; jq < 6; ++jq) {
-    const int j = 4 * jq + (lane >> 4);
-    const int pj = gamma_perm24(alam, j);
-    aoffp |= (unsigned)pj << (5 * jq);
-    if (gamma_sign24(4, pj) * gamma_sign24(alam, j) < 0) asgp |= 1u << jq;
-  }
-  const bool c2on = (lane & 15) < kCoefUsed - 32;
-  // FP32 16x16x4 result: element i of lane l is row 4*(l/16) + i, column l%16
-  const int coff = (4 * (lane >> 4)) * kW2CoefRow + (lane & 15);
-  float breg[3][kW2KSteps];
+  // MFMA waves: column block b = wave. A fragment of lane l: row (zi, mu) = l%16,
+  // k = 32 c + 8 (l/16) + e; FP32 16x16 result: element i of lane l is row
+  // 4 (l/16) + i = (zi = l/16, mu = i), column g = 16 b + l%16.
+  const int b = wave, lrow = lane & 15, lq = lane >> 4;
+  h8v bh[kS2Chunks], bl[kS2Chunks];
+  int te = 0;
   Kahan acc;
   barrier_all();  // the producers' prologue
   for (int it = 0; it < total; ++it) {
-    const int ip = group_pair(it);
-    const size_t z0 = group_z0(it);
+    const int u = step_unit(it);
+    const size_t z0 = step_z0(it);
     if (z0 == begin) {
-      const float* bm = bmat + (size_t)ip * 2 * 288 * kCoefCols;
+      const h8v* bs = reinterpret_cast<const h8v*>(bsplit + (size_t)u * kBsplitLen) + (size_t)b * kS2Chunks * 128 + lane;
 #pragma unroll
-      for (int cb = 0; cb < 3; ++cb)
-#pragma unroll
-        for (int i = 0; i < kW2KSteps; ++i)
-          breg[cb][i] = bm[((size_t)f * 288 + 4 * (kq * kW2KSteps + i) + (lane >> 4)) * kCoefCols +
-                           16 * cb + (lane & 15)];
+      for (int c = 0; c < kS2Chunks; ++c) {
+        bh[c] = bs[c * 128];
+        bl[c] = bs[c * 128 + 64];
+      }
+      te = bexp[(size_t)u * 160 + 16 * b + lrow];
       acc = Kahan();
     }
     barrier_all();
-    const float* dg = sDxu[it & 1];
-    const float* cg = sCoef[it & 1];
-    float v = 0.f;
-#pragma unroll 1
-    for (int zi = 0; zi < kW2Sites; ++zi) {
-      if (z0 + zi >= end) break;
-      // this site's coefficients (FP32 16x16x4 result: element i of lane l is row
-      // 4*(l/16) + i, column l%16)
-      const float* cs = cg + zi * kW2CoefSite + coff;
-      float cv[4][3];
+    const unsigned short* ah = sA[it & 1][0] + lrow * kS2ARow + 8 * lq;
+    const unsigned short* al = sA[it & 1][1] + lrow * kS2ARow + 8 * lq;
+    f4v c0 = {0.f, 0.f, 0.f, 0.f}, c1 = c0;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        cv[i][0] = cs[i * kW2CoefRow];
-        cv[i][1] = cs[i * kW2CoefRow + 16];
-        cv[i][2] = c2on ? cs[i * kW2CoefRow + 32] : 0.f;
-      }
-      // A values, read six at a time (one row r of dxu) ahead of their MFMAs
-      const float* dx = dg + ((zi * 2 + f) * 4 + amu) * kW2DxuRow + (kq * kW2KSteps / 6) * 24;
-      f4v c0 = {0.f, 0.f, 0.f, 0.f}, c1 = c0, c2 = c0;
-#pragma unroll
-      for (int rb = 0; rb < kW2KSteps / 6; ++rb) {  // kq*18 is a multiple of 6: jq = i % 6
-        float av[6];
-#pragma unroll
-        for (int jq = 0; jq < 6; ++jq) av[jq] = dx[rb * 24 + ((aoffp >> (5 * jq)) & 31)];
-#pragma unroll
-        for (int jq = 0; jq < 6; ++jq) {
-          const int i = 6 * rb + jq;
-          const float a = __uint_as_float(__float_as_uint(av[jq]) ^ (((asgp >> jq) & 1u) << 31));
-          c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[0][i], c0, 0, 0, 0);
-          c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[1][i], c1, 0, 0, 0);
-          c2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, breg[2][i], c2, 0, 0, 0);
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) v += c0[i] * cv[i][0] + c1[i] * cv[i][1] + c2[i] * cv[i][2];
+    for (int c = 0; c < kS2Chunks; ++c) {
+      const h8v xh = *reinterpret_cast<const h8v*>(ah + 32 * c);
+      const h8v xl = *reinterpret_cast<const h8v*>(al + 32 * c);
+      c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bh[c], c0, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bl[c], c1, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl, bh[c], c1, 0, 0, 0);
     }
-    acc.add((double)v);
-    if (z0 + kW2Sites >= end) {
+    // coefficient (mu, lam)(col) of column g = lam*40 + col: index mu*160 + g
+    const float* cs = sCoef[it & 1] + lq * kS2CoefSite + 16 * b + lrow;
+    const bool live = z0 + lq < end;
+    float v = 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float cf = cs[i * 160];  // in bounds; stale for sites past the end, so selected away
+      v += (c0[i] + c1[i]) * (live ? cf : 0.f);
+    }
+    acc.add((double)ldexpf(v, -(sSe[it & 1][lq] + te)));
+    if (z0 + kS2Sites >= end) {
       double s = acc.s - acc.c;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-      if (lane == 0) part[((size_t)ip * nwg + blockIdx.x) * kW2Waves + wave] = s;
+      if (lane == 0) part[((size_t)u * nwg + blockIdx.x) * kS2Mfma + b] = s;
     }
   }
 }
 
-// out[pair][f] = Kahan sum over workgroups and waves of the pair's partials
+// out[pair][f] = Kahan sum over workgroups and column blocks of the unit's partials
 __global__ void reduce_window2(const double* __restrict__ part, int nwg, int n_pairs,
                                double* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (pair, f)
-  if (i >= 2 * n_pairs) return;
-  const int ip = i / 2, f = i % 2;
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;  // (pair, f)
+  if (u >= 2 * n_pairs) return;
   Kahan k;
   for (int w = 0; w < nwg; ++w)
-    for (int j = 0; j < kW2Waves / 2; ++j) k.add(part[((size_t)ip * nwg + w) * kW2Waves + f * (kW2Waves / 2) + j]);
-  out[i] = k.s;
+    for (int j = 0; j < kS2Mfma; ++j) k.add(part[((size_t)u * nwg + w) * kS2Mfma + j]);
+  out[u] = k.s;
 }
 
 // ------------------------------------------------------------- launchers --
@@ -440,7 +566,11 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
   if (stage == 0) {
     launch_pass1_planes<float>(w.d_pairs.p, WinPair{}, np, g, g, c->proc, 3, w.part1.p, w.red1.p,
                                w.xi.p, st);
-    launch_forms_batched(w.xi.p, w.dz.p, w.gdz.p, nullptr, w.bmat.p, np, st);
+    launch_forms_batched(w.xi.p, w.dz.p, w.gdz.p, nullptr, nullptr, np, st);
+    hipLaunchKernelGGL(bsplit_exp, dim3(1, np), dim3(128), 0, st, w.gdz.p, w.bexp.p);
+    hipLaunchKernelGGL(form_bsplit, dim3((unsigned)div_up(kBsplitLen, 256), np), dim3(256), 0, st, w.gdz.p,
+                       w.bexp.p, w.bsplit.p);
+    check_hip(hipGetLastError(), "form_bsplit launch");
   }
   // coefficient fields, one per distinct yv
   if (stage == 1) {
@@ -472,9 +602,9 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
   // pass 2
   if (stage == 2) {
     const int nwg = c->n_cu;
-    const size_t spw = div_up(div_up(V, (size_t)nwg), kW2Sites) * kW2Sites;
+    const size_t spw = div_up(div_up(V, (size_t)nwg), kS2Sites) * kS2Sites;
     const int n = (int)div_up(V, spw);
-    hipLaunchKernelGGL(pass2_window, dim3(n), dim3(kW2Threads), 0, st, w.d_pairs.p, np, w.bmat.p,
+    hipLaunchKernelGGL(pass2_window, dim3(n), dim3(kS2Threads), 0, st, w.d_pairs.p, np, w.bsplit.p, w.bexp.p,
                        w.coef.p, g, spw, n, w.part2.p);
     check_hip(hipGetLastError(), "pass2_window launch");
     hipLaunchKernelGGL(reduce_window2, dim3((unsigned)div_up(2 * np, 64)), dim3(64), 0, st,
@@ -484,8 +614,11 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
 }
 
 size_t window_part2_len(const hlbl_contractor* c, int n_pairs) {
-  return (size_t)n_pairs * c->n_cu * kW2Waves;
+  return (size_t)n_pairs * 2 * c->n_cu * kS2Mfma;
 }
+
+size_t window_bsplit_len(int n_pairs) { return (size_t)n_pairs * 2 * kBsplitLen; }
+size_t window_bexp_len(int n_pairs) { return (size_t)n_pairs * 2 * 160; }
 
 
 }  // namespace hlbl

@@ -107,7 +107,8 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
     p->xi.ensure((size_t)n * 2 * 8 * 5 * 144 * 2);
     p->dz.ensure((size_t)n * 2 * 10 * 144 * 2);
     p->gdz.ensure((size_t)n * 2 * 40 * 144 * 2);
-    p->bmat.ensure((size_t)n * 2 * 288 * 48);
+    p->bsplit.ensure(window_bsplit_len(n));
+    p->bexp.ensure(window_bexp_len(n));
     p->coef.ensure((size_t)nu * V * 640);
     p->cst64.ensure((size_t)nu * 2 * qed::kNL);
     p->cst32.ensure((size_t)nu * 2 * qed::kNL);
