@@ -118,10 +118,15 @@ __device__ inline void dxu_vals(const R* __restrict__ sa, const R* __restrict__ 
       const A vr = ur[2 * (3 * s + cc)], vi = ur[2 * (3 * s + cc) + 1];
       const A t0r = sa[2 * (3 * s0 + cc)], t0i = sa[2 * (3 * s0 + cc) + 1];
       const A t1r = sa[2 * (3 * s1 + cc)], t1i = sa[2 * (3 * s1 + cc) + 1];
-      x0r += t0r * vr + t0i * vi;
-      x0i += t0r * vi - t0i * vr;
-      x1r += t1r * vr + t1i * vi;
-      x1i += t1r * vi - t1i * vr;
+      // conj(t) v as four fused multiply-adds each
+      x0r = fma(t0r, vr, x0r);
+      x0r = fma(t0i, vi, x0r);
+      x0i = fma(t0r, vi, x0i);
+      x0i = fma(-t0i, vr, x0i);
+      x1r = fma(t1r, vr, x1r);
+      x1r = fma(t1i, vi, x1r);
+      x1i = fma(t1r, vi, x1i);
+      x1i = fma(-t1i, vr, x1i);
     }
     // dxu_mu = -sum_s eps_s i^{-ph_mu(s)} Xi[cls(mu)][s]: the sign -eps_s folded in
     const bool neg = s < 2;
@@ -132,6 +137,36 @@ __device__ inline void dxu_vals(const R* __restrict__ sa, const R* __restrict__ 
       mul_ipow(ar, ai, 4 - gamma_phase(mu, s));
       re[mu] += neg ? -ar : ar;
       im[mu] += neg ? -ai : ai;
+    }
+  }
+}
+
+// The half of dxu_vals of one gamma class: CLS 0 gives mu = (0, 3), CLS 1 gives
+// mu = (1, 2), in (re[m], im[m]) for the m-th of the two; Xi[CLS][s] only.
+template <int CLS, class R, class A>
+__device__ inline void dxu_half(const R* __restrict__ sa, const R* __restrict__ ur, A (&re)[2], A (&im)[2]) {
+  re[0] = re[1] = im[0] = im[1] = A(0);
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int sp = CLS == 0 ? (s ^ 2) : (3 - s);
+    A xr = 0, xi = 0;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+      const A vr = ur[2 * (3 * s + cc)], vi = ur[2 * (3 * s + cc) + 1];
+      const A tr = sa[2 * (3 * sp + cc)], ti = sa[2 * (3 * sp + cc) + 1];
+      xr = fma(tr, vr, xr);
+      xr = fma(ti, vi, xr);
+      xi = fma(tr, vi, xi);
+      xi = fma(-ti, vr, xi);
+    }
+    const bool neg = s < 2;
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const int mu = CLS == 0 ? (m ? 3 : 0) : (m ? 2 : 1);
+      A ar = xr, ai = xi;
+      mul_ipow(ar, ai, 4 - gamma_phase(mu, s));
+      re[m] += neg ? -ar : ar;
+      im[m] += neg ? -ai : ai;
     }
   }
 }

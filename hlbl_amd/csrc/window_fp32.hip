@@ -192,52 +192,58 @@ __global__ __launch_bounds__(64) void judge_coef(
 // from the rows of A onto the columns of B:
 //   C[(zi, mu)][(lam, col)] = sum_k dxu[zi][mu][k] Bg_f[k][(lam, col)],
 //   Bg_f[r*24 + perm_lam(j)][(lam, col)] = sign_lam(j) B_f[r*24 + j][col],
-// a 16 x 160 x 288 product per group of 4 sites (rows (site, mu)) and flavour.
+// a 32 x 160 x 288 product per group of 8 sites (rows (site, mu)) and flavour.
 //
 // Precision: A and Bg are scaled by powers of two into FP16's range (A per site,
 // from the site's spinor maxima: |dxu| <= 24 max|S| max|Y|; Bg per column, from
 // the column maximum), split x = hi + lo in FP16 and multiplied as
-// C = hi.hi + hi.lo + lo.hi on v_mfma_f32_16x16x32_f16 (FP32 accumulation), then
+// C = hi.hi + hi.lo + lo.hi on v_mfma_f32_32x32x16_f16 (FP32 accumulation), then
 // unscaled exactly: every product keeps about 21 significant bits. The sum is FP32
-// per group of 4 sites and FP64 (Kahan) across groups.
+// per group of 8 sites and FP64 (Kahan) across groups.
 //
 // Work unit = (pair, flavour): the unit's Bg (hi and lo, 180 KB) lives in the
-// registers of 10 MFMA waves, one 16-column block each (72 VGPRs). Sixteen waves:
-//  - waves 0..9 run the 27 MFMAs of their column block for group g and dot the
-//    result with the coefficients;
+// registers of 10 MFMA waves, one (32-column block, half of K) each (72 VGPRs).
+// Sixteen waves, one barrier per group:
+//  - waves 0..9 run the 27 MFMAs of their block for group g and dot the result
+//    with the coefficients (the sum is linear, so the K halves dot separately);
 //  - waves 10..15 (producers) store group g+2's spinors (fetched into registers
 //    during the previous step) as FP32 with their per-site maxima, stage group
 //    g+1's coefficients (direct loads into LDS), build dxu of group g+1 (scaled,
 //    split hi/lo), fetch group g+3.
-constexpr int kS2Mfma = 10;                // MFMA waves = 16-column blocks of the 160
+// Every MFMA wave reads the whole A tile of a group from LDS; at 8 sites per
+// group that is half the LDS traffic per site of a 16-row tile.
+constexpr int kS2Mfma = 10;                // MFMA waves: 5 column blocks of 32 x 2 halves of K
 constexpr int kS2Prod = 6;                 // producer waves
 constexpr int kS2Threads = 64 * (kS2Mfma + kS2Prod);
-constexpr int kS2Sites = 4;
-constexpr int kS2Chunks = 9;               // K = 288 in steps of 32
-constexpr int kS2RowLen = kS2Sites * 24;   // values of one staged spinor row (4 sites)
+constexpr int kS2Sites = 8;
+constexpr int kS2Chunks = 18;              // K = 288 in steps of 16
+constexpr int kS2HalfChunks = kS2Chunks / 2;
+constexpr int kS2RowLen = kS2Sites * 24;   // values of one staged spinor row (8 sites)
 constexpr int kS2RowPad = kS2RowLen + 4;   // FP32 row stride
 constexpr int kS2BufLen = 24 * kS2RowPad;  // rows S[1-f][a] (0..11), Y[f][r] (12..23)
 constexpr int kS2ARow = 296;               // FP16 per A row (288 + 8): 148 dwords, so that the
-                                           // 16-byte reads of 16 rows hit distinct banks
-constexpr int kS2ALen = 16 * kS2ARow;      // one of hi / lo
+                                           // 16-byte reads of 8 consecutive rows hit distinct banks
+constexpr int kS2ALen = 32 * kS2ARow;      // one of hi / lo
 constexpr int kS2CoefSite = 656;           // floats per staged site (640 + 16): 16 banks apart
-constexpr int kS2KeyLanes = 48;            // producer lanes per (block, site) key
-constexpr int kS2Fetch = 144 / kS2KeyLanes;  // double2 per producer lane: 3
+constexpr int kS2Keys = 2 * kS2Sites;      // (isy, site) keys of the spinor maxima
+constexpr int kS2KeyLanes = 64 * kS2Prod / kS2Keys;  // producer lanes per key: 24
+constexpr int kS2Fetch = 144 / kS2KeyLanes;          // double2 per producer lane: 6
 constexpr int kS2Tasks = kS2Sites * 144;   // dxu tasks (site, r, a) per group
 constexpr int kS2AMax = 14;                // |scaled value| < 2^14 (FP16 max 65504)
 constexpr int kWinGdzLen = 2 * 40 * 144 * 2;
-static_assert(kS2Fetch == 3, "barrier_vm3 leaves exactly 3 loads in flight");
-static_assert(8 * kS2KeyLanes == 64 * kS2Prod, "one key per 48 producer lanes");
-// Bg per (pair, flavour) in fragment order [block 10][chunk 9][hi, lo][lane 64][8]
-constexpr int kBsplitLen = kS2Mfma * kS2Chunks * 2 * 64 * 8;
+static_assert(kS2Fetch == 6, "barrier_vm6 leaves exactly 6 loads in flight");
+static_assert(kS2KeyLanes % 8 == 0, "8-lane groups never straddle keys");
+// Bg per (pair, flavour) in fragment order [block 5][chunk 18][hi, lo][lane 64][8]
+constexpr int kBsplitLen = 5 * kS2Chunks * 2 * 64 * 8;
 
 typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+typedef float f16acc __attribute__((ext_vector_type(16)));
 
 // barrier that waits for this wave's LDS operations and for its global loads
 // (direct loads into LDS included)
 __device__ inline void barrier_all() { asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n s_barrier" ::: "memory"); }
-// the same, leaving the producer's last kS2Fetch (= 3) global loads in flight
-__device__ inline void barrier_vm3() { asm volatile("s_waitcnt vmcnt(3) lgkmcnt(0)\n s_barrier" ::: "memory"); }
+// the same, leaving the producer's last kS2Fetch (= 6) global loads in flight
+__device__ inline void barrier_vm6() { asm volatile("s_waitcnt vmcnt(6) lgkmcnt(0)\n s_barrier" ::: "memory"); }
 
 __device__ inline unsigned short f16_bits(float x) {
   const _Float16 h = (_Float16)x;  // nearest, ties to even
@@ -250,7 +256,7 @@ __device__ inline float f16_float(unsigned short u) {
   __builtin_memcpy(&h, &u, 2);
   return (float)h;
 }
-// x = hi + lo in FP16, packed (hi bits, lo bits)
+// x = hi + lo in FP16
 __device__ inline void f16_split(float x, unsigned short& hi, unsigned short& lo) {
   hi = f16_bits(x);
   lo = f16_bits(x - f16_float(hi));
@@ -275,17 +281,17 @@ __global__ void bsplit_exp(const double* __restrict__ gdz, int* __restrict__ bex
 }
 
 // Bg of every (pair, flavour) of the window, scaled by 2^te of its column and split
-// hi/lo in FP16, in fragment order: element (block b, chunk c, hl, lane l, e) is
-// Bg[k = 32c + 8(l/16) + e][g = 16b + l%16], g = lam*40 + col (the B layout of
-// v_mfma_f32_16x16x32_f16).
+// hi/lo in FP16, in fragment order: element (block cb, chunk c, hl, lane l, e) is
+// Bg[k = 16c + 8(l/32) + e][g = 32cb + l%32], g = lam*40 + col (the B layout of
+// v_mfma_f32_32x32x16_f16).
 __global__ void form_bsplit(const double* __restrict__ gdz, const int* __restrict__ bexp,
                             unsigned short* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (f, b, c, l, e) of pair blockIdx.y
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (f, cb, c, l, e) of pair blockIdx.y
   if (i >= kBsplitLen) return;  // 2 flavours x kBsplitLen / 2 (hi and lo per thread)
-  const int e = i & 7, l = (i >> 3) & 63, c = (i >> 9) % kS2Chunks, b = (i / (512 * kS2Chunks)) % kS2Mfma,
-            f = i / (512 * kS2Chunks * kS2Mfma);
+  const int e = i & 7, l = (i >> 3) & 63, c = (i >> 9) % kS2Chunks, cb = (i / (512 * kS2Chunks)) % 5,
+            f = i / (512 * kS2Chunks * 5);
   gdz += (size_t)blockIdx.y * kWinGdzLen;
-  const int k = 32 * c + 8 * (l >> 4) + e, g = 16 * b + (l & 15);
+  const int k = 16 * c + 8 * (l >> 5) + e, g = 32 * cb + (l & 31);
   const int lam = g / 40, col = g % 40, r = k / 24, jp = k % 24;
   int j = 0;
   for (int jj = 0; jj < 24; ++jj) j = gamma_perm24(lam, jj) == jp ? jj : j;
@@ -293,7 +299,7 @@ __global__ void form_bsplit(const double* __restrict__ gdz, const int* __restric
   const double v = ldexp((gamma_sign24(4, jp) * gamma_sign24(lam, j)) * bmat_value(gdz, f, r * 24 + j, col), te);
   const unsigned short hi = f16_bits((float)v);
   const unsigned short lo = f16_bits((float)(v - (double)f16_float(hi)));
-  unsigned short* o = out + ((size_t)blockIdx.y * 2 + f) * kBsplitLen + (((size_t)b * kS2Chunks + c) * 2 * 64 + l) * 8 + e;
+  unsigned short* o = out + ((size_t)blockIdx.y * 2 + f) * kBsplitLen + (((size_t)cb * kS2Chunks + c) * 2 * 64 + l) * 8 + e;
   o[0] = hi;
   o[64 * 8] = lo;
 }
@@ -308,36 +314,70 @@ __device__ inline size_t uniform_u64(size_t v) {
 typedef double d2v __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) const d2v gd2v;
 
-// Producer lane ptid serves key ptid / 48 = (isy, site) of the group [z0, z0 + 4):
+// Position of a step (unit = (pair, flavour), group) along the workgroup's walk,
+// advanced one step at a time (no divisions).
+struct S2Cursor {
+  int u = 0, g = 0;
+  __device__ inline void next(int ng) {
+    if (++g == ng) {
+      g = 0;
+      ++u;
+    }
+  }
+};
+
+// Producer lane ptid serves key ptid / 24 = (isy, site) of a group [z0, z0 + 8):
 // isy = 0 the rows S[1-f][a], isy = 1 the rows Y[f][r] of site z0 + site. Its k-th
-// double pair is d = j + 48 k (j = ptid % 48) of the key's 144: row d / 12,
-// doubles 2 (d % 12) of the site. Sites at or beyond V are read from site V-1
-// (their coefficients are masked).
-__device__ inline void s2_fetch(int ptid, const WinPair* wp, int f, size_t V, size_t z0,
+// double pair is d = j + 24 k (j = ptid % 24) of the key's 144: row d / 12,
+// doubles 2 (d % 12) of the site. S2Lane holds the lane's fixed byte offsets.
+struct S2Lane {
+  size_t row_off[kS2Fetch];  // (row * V * 24 + 2 (d % 12)) * 8
+  int key, isy, site;
+};
+
+__device__ inline S2Lane s2_lane(int ptid, size_t V) {
+  S2Lane L;
+  L.key = ptid / kS2KeyLanes;
+  L.isy = L.key >= kS2Sites;
+  L.site = L.key % kS2Sites;
+  const int j = ptid % kS2KeyLanes;
+#pragma unroll
+  for (int k = 0; k < kS2Fetch; ++k) {
+    const int d = j + kS2KeyLanes * k;
+    L.row_off[k] = ((size_t)(d / 12) * V * 24 + 2 * (d % 12)) * sizeof(double);
+  }
+  return L;
+}
+
+// fetch of flavour f's rows of the group at z0 of pair wp; sites at or beyond V
+// are read from site V-1 (their coefficients are masked)
+__device__ inline void s2_fetch(const S2Lane& L, const WinPair* wp, int f, size_t V, size_t z0,
                                 d2v (&r)[kS2Fetch]) {
   // both block pointers as wave-uniform values in the global address space (plain
   // global loads: counted by vmcnt only)
   const size_t ps = uniform_u64((size_t)wp->S), py = uniform_u64((size_t)wp->Y);
-  const int key = ptid / kS2KeyLanes, j = ptid % kS2KeyLanes;
-  const bool isy = key >= 4;
-  const int site = key & 3, ff = isy ? f : 1 - f;
-  size_t z = z0 + site;
+  size_t z = z0 + L.site;
   z = z < V ? z : V - 1;
+  const int ff = L.isy ? f : 1 - f;
+  const size_t base = (L.isy ? py : ps) + ((size_t)ff * 12 * V + z) * 24 * sizeof(double);
 #pragma unroll
-  for (int k = 0; k < kS2Fetch; ++k) {
-    const int d = j + kS2KeyLanes * k, row = d / 12, w2 = 2 * (d % 12);
-    const size_t off = ((((size_t)(ff * 12 + row)) * V + z) * 24 + w2) * sizeof(double);
-    r[k] = *reinterpret_cast<gd2v*>((isy ? py : ps) + off);
-  }
+  for (int k = 0; k < kS2Fetch; ++k) r[k] = *reinterpret_cast<gd2v*>(base + L.row_off[k]);
+}
+
+// maximum over each group of 8 lanes (all lanes active), by DPP
+__device__ inline int max8(int v) {
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  return v;
 }
 
 // ... stores them as FP32 into buf [24][kS2RowPad] and folds their largest
-// magnitude into smax[key] (a wave spans two keys: one masked wave maximum each,
-// one LDS atomic per key and wave; non-negative floats order as their bits)
-__device__ inline void s2_store(int ptid, const d2v (&r)[kS2Fetch], float* buf, unsigned* smax) {
-  const int key = ptid / kS2KeyLanes, j = ptid % kS2KeyLanes;
-  const bool isy = key >= 4;
-  const int site = key & 3;
+// magnitude into smax[key] (8-lane groups never straddle keys: one DPP maximum per
+// group, one LDS atomic per group; non-negative floats order as their bits)
+__device__ inline void s2_store(int ptid, const S2Lane& L, const d2v (&r)[kS2Fetch], float* buf,
+                                unsigned* smax) {
+  const int j = ptid % kS2KeyLanes;
   float m = 0.f;
 #pragma unroll
   for (int k = 0; k < kS2Fetch; ++k) {
@@ -346,37 +386,35 @@ __device__ inline void s2_store(int ptid, const d2v (&r)[kS2Fetch], float* buf, 
     v.x = (float)r[k].x;
     v.y = (float)r[k].y;
     m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
-    *reinterpret_cast<float2*>(buf + ((isy ? 12 : 0) + row) * kS2RowPad + site * 24 + w2) = v;
+    *reinterpret_cast<float2*>(buf + ((L.isy ? 12 : 0) + row) * kS2RowPad + L.site * 24 + w2) = v;
   }
-  const int w = ptid >> 6;
-  const int key0 = (64 * w) / kS2KeyLanes, key1 = (64 * w + 63) / kS2KeyLanes;
-  float m0 = key == key0 ? m : 0.f, m1 = key == key1 ? m : 0.f;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    m0 = fmaxf(m0, __shfl_xor(m0, off));
-    m1 = fmaxf(m1, __shfl_xor(m1, off));
-  }
-  if ((ptid & 63) == 0) {
-    atomicMax(smax + key0, __float_as_uint(m0));
-    if (key1 != key0) atomicMax(smax + key1, __float_as_uint(m1));
-  }
+  const int mg = max8(__float_as_int(m));
+  if ((ptid & 7) == 0) atomicMax(smax + L.key, (unsigned)mg);
 }
 
-// Producer p stages the 16 x 40 coefficients of site z0 + p (relative to gsx)
-// into dst + p * kS2CoefSite with direct loads (lane l: 16 bytes at 16 l).
+// A producer stages the 16 x 40 coefficients of site z0 + p (relative to gsx) into
+// dst + p * kS2CoefSite with direct loads (lane l: 16 bytes at 16 l). p, z0 and the
+// pair are wave-uniform, so the site arithmetic runs on the scalar unit (32-bit:
+// the volume is below 2^31).
 __device__ inline void s2_coef(int p, const WinPair& wp, hlbl_geom global, size_t V, size_t z0,
                                size_t end, const float* __restrict__ coef, float* dst) {
   const int lane = threadIdx.x & 63;
   const size_t z = z0 + p;
   if (z >= end) return;
+  unsigned q = (unsigned)z;
   int crd[4];
-  site_coords(z, global, crd);
+  crd[3] = (int)(q % (unsigned)global.LZ);
+  q /= (unsigned)global.LZ;
+  crd[2] = (int)(q % (unsigned)global.LY);
+  q /= (unsigned)global.LY;
+  crd[1] = (int)(q % (unsigned)global.LX);
+  crd[0] = (int)(q / (unsigned)global.LX);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     crd[r] -= coord_dim(wp.gsx, r);
     crd[r] += crd[r] < 0 ? geom_dim(global, r) : 0;
   }
-  const size_t rs = (((size_t)crd[0] * global.LX + crd[1]) * global.LY + crd[2]) * global.LZ + crd[3];
+  const unsigned rs = (((unsigned)crd[0] * global.LX + crd[1]) * global.LY + crd[2]) * global.LZ + crd[3];
   const float* src = coef + ((size_t)wp.yv_slot * V + rs) * kCoefField + 4 * lane;
   float* d = dst + p * kS2CoefSite;
   // 640 floats = 2.5 loads of 64 lanes x 16 bytes
@@ -397,14 +435,18 @@ __device__ inline int site_exp(unsigned ms, unsigned my) {
   return se < -120 ? -120 : (se > 120 ? 120 : se);
 }
 
-// dxu of a staged group by the producer lanes, task (site zi, r, a): scaled by
-// 2^se of its site, split into FP16 (re, im) pairs into ah / al [(zi, mu)][kS2ARow]
-// at k = r*24 + 2a; the site exponents go to sse[zi]
-__device__ inline void s2_dxu(int ptid, const float* buf, const unsigned* smax, unsigned* ah, unsigned* al,
-                              int* sse) {
-  for (int t = ptid; t < kS2Tasks; t += 64 * kS2Prod) {
+// dxu tasks t = t0 + 64 i + lane (i < N) of a staged group, task t = (site zi, r, a):
+// dxu_mu[r][a] of the four mu, scaled by 2^se of the site and split into FP16
+// (re, im) pairs into ah / al [(zi, mu)][kS2ARow] at k = r*24 + 2a; the site
+// exponents go to sse[zi] (from task (zi, 0, 0)).
+template <int N>
+__device__ inline void s2_dxu(int t0, int lane, const float* buf, const unsigned* smax, unsigned* ah,
+                              unsigned* al, int* sse) {
+#pragma unroll 1
+  for (int i = 0; i < N; ++i) {
+    const int t = t0 + 64 * i + lane;
     const int zi = t / 144, r = (t / 12) % 12, a = t % 12;
-    const int se = site_exp(smax[zi], smax[4 + zi]);
+    const int se = site_exp(smax[zi], smax[kS2Sites + zi]);
     if (r == 0 && a == 0) sse[zi] = se;
     float re[4], im[4];
     dxu_vals<float, float>(buf + a * kS2RowPad + zi * 24, buf + (12 + r) * kS2RowPad + zi * 24, re, im);
@@ -420,6 +462,14 @@ __device__ inline void s2_dxu(int ptid, const float* buf, const unsigned* smax, 
   }
 }
 
+// Producer roles. Waves sit on SIMD (wave % 4): producer waves 12 and 13 are alone
+// on SIMDs 0 and 1 and take 5 dxu tasks per lane; 10, 14 (SIMD 2) and 11, 15
+// (SIMD 3) take 2 each and stage the coefficients of two sites each. Task ranges:
+// pw 2: [0, 320), pw 3: [320, 640), pw 0, 1, 4, 5: 128 each from 640.
+__device__ inline int s2_task0(int pw) { return pw == 2 ? 0 : pw == 3 ? 320 : pw == 0 ? 640 : pw == 1 ? 768 : pw == 4 ? 896 : 1024; }
+__device__ inline int s2_coef_site0(int pw) { return pw == 0 ? 0 : pw == 1 ? 2 : pw == 4 ? 4 : pw == 5 ? 6 : -1; }
+static_assert(2 * 320 + 4 * 128 == kS2Tasks, "the producer task ranges cover a group");
+
 __global__ __launch_bounds__(kS2Threads) void pass2_window(
     const WinPair* __restrict__ pairs, int n_pairs, const unsigned short* __restrict__ bsplit,
     const int* __restrict__ bexp, const float* __restrict__ coef, hlbl_geom global,
@@ -427,8 +477,8 @@ __global__ __launch_bounds__(kS2Threads) void pass2_window(
   __shared__ __attribute__((aligned(16))) float sF[2][kS2BufLen];
   __shared__ __attribute__((aligned(16))) unsigned short sA[2][2][kS2ALen];
   __shared__ __attribute__((aligned(16))) float sCoef[2][kS2Sites * kS2CoefSite];
-  __shared__ unsigned sMax[3][8];  // per group g % 3: spinor maxima per (isy, site)
-  __shared__ int sSe[2][kS2Sites];  // per group g % 2: site exponents
+  __shared__ unsigned sMax[3][kS2Keys];  // per group g % 3: spinor maxima per (isy, site)
+  __shared__ int sSe[2][kS2Sites];        // per group g % 2: site exponents
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const bool producer = wave >= kS2Mfma;
@@ -438,108 +488,122 @@ __global__ __launch_bounds__(kS2Threads) void pass2_window(
   if (begin >= end) return;
   const int ng = (int)((end - begin + kS2Sites - 1) / kS2Sites);
   const int total = ng * 2 * n_pairs;  // steps: (unit = (pair, flavour), group)
-  auto step_unit = [&](int it) { return it / ng; };
-  auto step_z0 = [&](int it) { return begin + (size_t)(it - (it / ng) * ng) * kS2Sites; };
-  if (tid < 24) (&sMax[0][0])[tid] = 0u;
+  if (tid < 3 * kS2Keys) (&sMax[0][0])[tid] = 0u;
   __syncthreads();
 
   if (producer) {
-    const int pw = wave - kS2Mfma, ptid = tid - 64 * kS2Mfma;
-    auto fetch = [&](int it, d2v (&r)[kS2Fetch]) {
-      const int u = step_unit(it);
-      s2_fetch(ptid, pairs + (u >> 1), u & 1, V, step_z0(it), r);
-    };
-    auto stage_coef = [&](int it, float* dst) {
-      if (pw >= kS2Prod - kS2Sites) s2_coef(pw - (kS2Prod - kS2Sites), pairs[step_unit(it) >> 1], global, V,
-                                            step_z0(it), end, coef, dst);
-    };
-    auto dxu = [&](int it) {
-      s2_dxu(ptid, sF[it & 1], sMax[it % 3], reinterpret_cast<unsigned*>(sA[it & 1][0]),
-             reinterpret_cast<unsigned*>(sA[it & 1][1]), sSe[it & 1]);
-    };
+    const int pw = __builtin_amdgcn_readfirstlane(wave - kS2Mfma), ptid = tid - 64 * kS2Mfma;
+    const S2Lane L = s2_lane(ptid, V);
+    const int t0 = s2_task0(pw), cs0 = s2_coef_site0(pw);
+    S2Cursor fc, nc;  // the step being fetched, the step being prepared
     d2v r[kS2Fetch];
-    // prologue: spinors of steps 0 and 1 staged, coefficients of step 0 staged,
-    // step 2 in flight, dxu of step 0 built
-    fetch(0, r);
-    s2_store(ptid, r, sF[0], sMax[0]);
-    if (total > 1) {
-      fetch(1, r);
-      s2_store(ptid, r, sF[1], sMax[1]);
-    }
-    stage_coef(0, sCoef[0]);
-    bool inflight = total > 2;
-    if (inflight) fetch(2, r);
-    // the barrier waits for the coefficient loads but lets the later fetch fly
-    if (inflight) barrier_vm3(); else barrier_all();
-    dxu(0);
-    for (int it = 0; it < total; ++it) {
-      if (inflight) barrier_vm3(); else barrier_all();
-      // maxima of group it were read by its dxu (last step): clear them for group it+3
-      if (ptid < 8) sMax[it % 3][ptid] = 0u;
-      if (it + 2 < total) s2_store(ptid, r, sF[it & 1], sMax[(it + 2) % 3]);  // fetched last step
-      if (it + 1 < total) {
-        stage_coef(it + 1, sCoef[(it + 1) & 1]);
-        dxu(it + 1);
+    auto fetch = [&]() {
+      s2_fetch(L, pairs + (fc.u >> 1), fc.u & 1, V, begin + (size_t)fc.g * kS2Sites, r);
+      fc.next(ng);
+    };
+    auto prepare = [&](int it) {  // coefficients and dxu of step it (= nc)
+      if (cs0 >= 0) {
+        const WinPair& wp = pairs[nc.u >> 1];
+        const size_t z0 = begin + (size_t)nc.g * kS2Sites;
+        s2_coef(cs0, wp, global, V, z0, end, coef, sCoef[it & 1]);
+        s2_coef(cs0 + 1, wp, global, V, z0, end, coef, sCoef[it & 1]);
       }
+      unsigned* ah = reinterpret_cast<unsigned*>(sA[it & 1][0]);
+      unsigned* al = reinterpret_cast<unsigned*>(sA[it & 1][1]);
+      if (pw == 2 || pw == 3) s2_dxu<5>(t0, lane, sF[it & 1], sMax[it % 3], ah, al, sSe[it & 1]);
+      else s2_dxu<2>(t0, lane, sF[it & 1], sMax[it % 3], ah, al, sSe[it & 1]);
+      nc.next(ng);
+    };
+    // prologue: spinors of steps 0 and 1 staged, step 2 in flight; coefficients and
+    // dxu of step 0 built after the barrier
+    fetch();
+    s2_store(ptid, L, r, sF[0], sMax[0]);
+    if (total > 1) {
+      fetch();
+      s2_store(ptid, L, r, sF[1], sMax[1]);
+    }
+    bool inflight = total > 2;
+    if (inflight) fetch();
+    if (inflight) barrier_vm6(); else barrier_all();
+    prepare(0);
+    int m3 = 0;  // it % 3
+    for (int it = 0; it < total; ++it) {
+      // waits for this step's coefficient loads but lets the later fetch fly
+      if (inflight) barrier_vm6(); else barrier_all();
+      // maxima of group it were read by its dxu (last step): clear them for group it+3
+      if (ptid < kS2Keys) sMax[m3][ptid] = 0u;
+      const int m3n = m3 == 2 ? 0 : m3 + 1, m3nn = m3n == 2 ? 0 : m3n + 1;
+      if (it + 2 < total) s2_store(ptid, L, r, sF[it & 1], sMax[m3nn]);  // fetched last step
+      if (it + 1 < total) prepare(it + 1);
       inflight = it + 3 < total;
-      if (inflight) fetch(it + 3, r);
+      if (inflight) fetch();
+      m3 = m3n;
     }
     return;
   }
 
-  // MFMA waves: column block b = wave. A fragment of lane l: row (zi, mu) = l%16,
-  // k = 32 c + 8 (l/16) + e; FP32 16x16 result: element i of lane l is row
-  // 4 (l/16) + i = (zi = l/16, mu = i), column g = 16 b + l%16.
-  const int b = wave, lrow = lane & 15, lq = lane >> 4;
-  h8v bh[kS2Chunks], bl[kS2Chunks];
+  // MFMA waves: column block cb = wave / 2, K half kh = wave % 2 (chunks 9 kh ..
+  // 9 kh + 8). A fragment of lane l: row (zi, mu) = l%32, k = 16 c + 8 (l/32) + e;
+  // FP32 32x32 result: element e of lane l is row 8 (e/4) + 4 (l/32) + e%4 =
+  // (zi = 2 (e/4) + l/32, mu = e%4), column g = 32 cb + l%32.
+  const int cb = wave >> 1, kh = wave & 1, lrow = lane & 31, lh = lane >> 5;
+  h8v bh[kS2HalfChunks], bl[kS2HalfChunks];
   int te = 0;
   Kahan acc;
+  S2Cursor cur;
   barrier_all();  // the producers' prologue
-  for (int it = 0; it < total; ++it) {
-    const int u = step_unit(it);
-    const size_t z0 = step_z0(it);
-    if (z0 == begin) {
-      const h8v* bs = reinterpret_cast<const h8v*>(bsplit + (size_t)u * kBsplitLen) + (size_t)b * kS2Chunks * 128 + lane;
+  for (int it = 0; it < total; ++it, cur.next(ng)) {
+    const int u = cur.u;
+    const size_t z0 = begin + (size_t)cur.g * kS2Sites;
+    if (cur.g == 0) {
+      const h8v* bs = reinterpret_cast<const h8v*>(bsplit + (size_t)u * kBsplitLen) +
+                      ((size_t)cb * kS2Chunks + kS2HalfChunks * kh) * 128 + lane;
 #pragma unroll
-      for (int c = 0; c < kS2Chunks; ++c) {
+      for (int c = 0; c < kS2HalfChunks; ++c) {
         bh[c] = bs[c * 128];
         bl[c] = bs[c * 128 + 64];
       }
-      te = bexp[(size_t)u * 160 + 16 * b + lrow];
+      te = bexp[(size_t)u * 160 + 32 * cb + lrow];
       acc = Kahan();
     }
     barrier_all();
-    const unsigned short* ah = sA[it & 1][0] + lrow * kS2ARow + 8 * lq;
-    const unsigned short* al = sA[it & 1][1] + lrow * kS2ARow + 8 * lq;
-    f4v c0 = {0.f, 0.f, 0.f, 0.f}, c1 = c0;
+    const unsigned short* ah = sA[it & 1][0] + lrow * kS2ARow + 8 * lh + 16 * kS2HalfChunks * kh;
+    const unsigned short* al = sA[it & 1][1] + lrow * kS2ARow + 8 * lh + 16 * kS2HalfChunks * kh;
+    f16acc c;
 #pragma unroll
-    for (int c = 0; c < kS2Chunks; ++c) {
-      const h8v xh = *reinterpret_cast<const h8v*>(ah + 32 * c);
-      const h8v xl = *reinterpret_cast<const h8v*>(al + 32 * c);
-      c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bh[c], c0, 0, 0, 0);
-      c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bl[c], c1, 0, 0, 0);
-      c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl, bh[c], c1, 0, 0, 0);
-    }
-    // coefficient (mu, lam)(col) of column g = lam*40 + col: index mu*160 + g
-    const float* cs = sCoef[it & 1] + lq * kS2CoefSite + 16 * b + lrow;
-    const bool live = z0 + lq < end;
-    float v = 0.f;
+    for (int e = 0; e < 16; ++e) c[e] = 0.f;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const float cf = cs[i * 160];  // in bounds; stale for sites past the end, so selected away
-      v += (c0[i] + c1[i]) * (live ? cf : 0.f);
+    for (int i = 0; i < kS2HalfChunks; ++i) {
+      const h8v xh = *reinterpret_cast<const h8v*>(ah + 16 * i);
+      const h8v xl = *reinterpret_cast<const h8v*>(al + 16 * i);
+      c = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bh[i], c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bl[i], c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_32x32x16_f16(xl, bh[i], c, 0, 0, 0);
     }
-    acc.add((double)ldexpf(v, -(sSe[it & 1][lq] + te)));
-    if (z0 + kS2Sites >= end) {
+    // coefficient (mu, lam)(col) of column g = lam*40 + col: index mu*160 + g; the
+    // four sites zi = 2q + l/32 of this lane unscaled by 2^-(se + te) each
+    const float* cs = sCoef[it & 1] + lh * kS2CoefSite + 32 * cb + lrow;
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int zi = 2 * q + lh;
+      float vq = 0.f;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) vq += c[4 * q + m] * cs[2 * q * kS2CoefSite + m * 160];
+      // sites past the end: stale coefficients (in bounds), selected away
+      v += z0 + zi < end ? (double)ldexpf(vq, -(sSe[it & 1][zi] + te)) : 0.0;
+    }
+    acc.add(v);
+    if (cur.g == ng - 1) {
       double s = acc.s - acc.c;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-      if (lane == 0) part[((size_t)u * nwg + blockIdx.x) * kS2Mfma + b] = s;
+      if (lane == 0) part[((size_t)u * nwg + blockIdx.x) * kS2Mfma + wave] = s;
     }
   }
 }
 
-// out[pair][f] = Kahan sum over workgroups and column blocks of the unit's partials
+// out[pair][f] = Kahan sum over workgroups and MFMA waves of the unit's partials
 __global__ void reduce_window2(const double* __restrict__ part, int nwg, int n_pairs,
                                double* __restrict__ out) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;  // (pair, f)
