@@ -295,6 +295,96 @@ __device__ __forceinline__ void kfield_wave(const TT* __restrict__ table, const 
   wave_sync();
 }
 
+// The evaluations of L0 that depend on the site alone, L0(x, 0) and L0(0, x), into
+// x0[384] and ox[384] (one wave per site; w: kKwLen elements of LDS). The same
+// functions as in kfield_wave, so the rows equal what it computes in place.
+template <class R, class TT, class O>
+__device__ __forceinline__ void kfield_wave_axis_rows(const TT* __restrict__ table, const TermLds& tt,
+                                                      const SiteGeom<R>& g, const ChannelRows rows, R* w,
+                                                      O* __restrict__ x0, O* __restrict__ ox) {
+  const int lane = threadIdx.x & 63;
+  R* ch = w;
+  R* T = ch + 6 * 32;
+  R* s0 = T + qed::kNT;
+  if (lane < qed::kNChUsed) {
+    ch[1 * 32 + lane] = rows.given ? (R)rows.row(1)[lane] : kp_channel(1, lane, g, table);
+    ch[3 * 32 + lane] = rows.given ? (R)rows.row(3)[lane] : kp_channel(3, lane, g, table);
+  }
+  wave_sync();
+  wave_l0_from(1, tt, g, ch + 1 * 32, T, s0);  // (x, 0)
+#pragma unroll
+  for (int k = 0; k < qed::kNL / 64; ++k) x0[64 * k + lane] = (O)s0[64 * k + lane];
+  wave_sync();
+  wave_l0_from(3, tt, g, ch + 3 * 32, T, s0);  // (0, x)
+#pragma unroll
+  for (int k = 0; k < qed::kNL / 64; ++k) ox[64 * k + lane] = (O)s0[64 * k + lane];
+  wave_sync();
+}
+
+// kfield_wave with the site-only evaluations read from rows computed by
+// kfield_wave_axis_rows: x0 = L0(x, 0) and ox = L0(0, x) of this site, od =
+// L0(0, d), the (0, x) row of the site d = x - y. Only (x, y), (y, x) and (x, d)
+// are evaluated here.
+template <class R, class TT, class O>
+__device__ __forceinline__ void kfield_wave_axes(const TT* __restrict__ table, const TermLds& tt,
+                                                 const SiteGeom<R>& g, const ChannelRows rows,
+                                                 const R* cst, const float* __restrict__ x0,
+                                                 const float* __restrict__ ox, const float* __restrict__ od,
+                                                 R* w, O* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  R* ch = w;
+  R* T = ch + 6 * 32;
+  R* L = T + qed::kNT;
+  R* C = L + 3 * qed::kNL;
+  R* D = C + kCoefField;
+  R* s0 = L;
+  R* s1 = L + qed::kNL;
+  R* s2 = L + 2 * qed::kNL;
+  // the three rows, loaded first (one latency)
+  float rx0[qed::kNL / 64], rox[qed::kNL / 64], rod[qed::kNL / 64];
+#pragma unroll
+  for (int k = 0; k < qed::kNL / 64; ++k) {
+    rx0[k] = x0[64 * k + lane];
+    rox[k] = ox[64 * k + lane];
+    rod[k] = od[64 * k + lane];
+  }
+  if (lane < 16) kp_dmat(lane, g, D);
+  // channels of the evaluations 0 (x, y), 2 (y, x), 4 (x, d)
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int t = lane + 64 * k;
+    if (t < 3 * qed::kNChUsed) {
+      const int j = t / qed::kNChUsed, c = t - j * qed::kNChUsed, e = 2 * j;
+      ch[e * 32 + c] = rows.given ? (R)rows.row(e)[c] : kp_channel(e, c, g, table);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < qed::kNL / 64; ++k) {
+    s0[64 * k + lane] = (R)rx0[k];
+    s1[64 * k + lane] = (R)rox[k];
+  }
+  wave_sync();
+  const R* Dx = D;
+  const R* Dy = D + 16;
+  const R* Dd = D + 32;
+  wave_l0_from(0, tt, g, ch + 0 * 32, T, s2);                  // (x, y)
+  wave_kernel(0, s2, cst, s0, Dx, Dy, g.in12, C);              // K1 = L(x, y)
+  wave_l0_from(2, tt, g, ch + 2 * 32, T, s2);                  // (y, x)
+  wave_kernel(1, s2, s1, cst + qed::kNL, Dy, Dx, g.in12, C);   // K2 = L(y, x)
+#pragma unroll
+  for (int k = 0; k < qed::kNL / 64; ++k) s1[64 * k + lane] = (R)rod[k];  // (0, d)
+  wave_l0_from(4, tt, g, ch + 4 * 32, T, s2);                  // (x, d)
+  wave_kernel(2, s2, s1, s0, Dx, Dd, g.in3, C);                // K3 = L(x, d)
+  for (int t = lane; t < 16 * 16; t += 64) {
+    const int row = t >> 4, hc = t & 15;
+    C[row * kCoefUsed + 24 + hc] = coef_h(row, hc, g, s2);
+  }
+  wave_sync();
+#pragma unroll
+  for (int k = 0; k < kCoefField / 64; ++k) out[64 * k + lane] = (O)C[64 * k + lane];
+  wave_sync();
+}
+
 // The pair constants [L0(0,y) | L0(y,0)] into cst (768 elements), by one wave
 // (w: kKwLen elements of LDS).
 template <class R, class TT>

@@ -38,6 +38,7 @@ struct WindowPlan {
   DevBuf<unsigned short> bsplit;  // pass-2 B operands, FP16 hi/lo in MFMA fragment order
   DevBuf<int> bexp;               // their column exponents [pair][f][160]
   DevBuf<float> coef;      // [n_unique][V][640]
+  DevBuf<float> ax;        // L0(x, 0) and L0(0, x) of every site [2][V][384]
   DevBuf<double> cst64;    // pair constants L0(0,y), L0(y,0) per distinct yv [n_unique][768]
   DevBuf<float> cst32;
   DevBuf<float> chbuf;     // surrogate channels [2 + 3 n_unique][V][32]

@@ -66,10 +66,41 @@ __global__ __launch_bounds__(64) void kfield_consts_win(const double* __restrict
   }
 }
 
+// The site-only evaluations L0(x, 0), L0(0, x) of every site (gsx = 0): ax[0][V][384]
+// and ax[1][V][384], FP32; channels from the FP32 table, or from chbuf blocks 0
+// ((x, 0)) and 1 ((0, x)) for the surrogate. One wave per site.
+__global__ __launch_bounds__(kKmThreads) void kfield_axes(const float* __restrict__ table32, hlbl_geom global,
+                                                          double xunit, const float* __restrict__ chbuf,
+                                                          float* __restrict__ ax) {
+  __shared__ TermLds tt;
+  __shared__ float w[kKmThreads / 64][kKwLen];
+  stage_terms(c_terms32, &tt);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6;
+  const size_t V = geom_volume(global);
+  const size_t nw = (size_t)gridDim.x * (kKmThreads / 64);
+  const hlbl_coord zero = {0, 0, 0, 0};
+  for (size_t site = (size_t)blockIdx.x * (kKmThreads / 64) + wave; site < V; site += nw) {
+    SiteGeom<float> g;
+    site_geom(site, global, global, zero, zero, zero, xunit, g);
+    ChannelRows rows = ChannelRows::none();
+    if (chbuf) {
+      rows.given = true;
+      rows.p[1] = chbuf + ((size_t)0 * V + site) * qed::kNCh;
+      rows.p[3] = chbuf + ((size_t)1 * V + site) * qed::kNCh;
+    }
+    kfield_wave_axis_rows<float, float, float>(table32, tt, g, rows, w[wave], ax + site * qed::kNL,
+                                               ax + (V + site) * qed::kNL);
+  }
+}
+
+// The field of each distinct yv in FP32 from the axis rows: per (yv, site) only
+// (x, y), (y, x) and (x, d) are evaluated; (x, 0) and (0, x) are this site's rows
+// and (0, d) is the (0, x) row of the site d = x - y.
 __global__ __launch_bounds__(kKmThreads) void kfield_mixed(
     const float* __restrict__ table32, const hlbl_coord* __restrict__ yvs, int n_unique,
     hlbl_geom global, double xunit, const float* __restrict__ cst32,
-    const float* __restrict__ chbuf, float* __restrict__ coef) {
+    const float* __restrict__ chbuf, const float* __restrict__ ax, float* __restrict__ coef) {
   __shared__ TermLds tt;
   __shared__ float w[kKmThreads / 64][kKwLen];
   stage_terms(c_terms32, &tt);
@@ -85,26 +116,23 @@ __global__ __launch_bounds__(kKmThreads) void kfield_mixed(
     const hlbl_coord yv = yvs[u];
     SiteGeom<float> g;
     site_geom(site, global, global, zero, zero, yv, xunit, g);
+    // the site d = x - y (the argument of (0, d))
+    int crd[4], dsh[4];
+    site_coords(site, global, crd);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) dsh[r] = mod_pos(crd[r] - coord_dim(yv, r), geom_dim(global, r));
+    const size_t sd = (((size_t)dsh[0] * global.LX + dsh[1]) * global.LY + dsh[2]) * global.LZ + dsh[3];
     ChannelRows rows = ChannelRows::none();
     if (chbuf) {
-      // surrogate channel rows: blocks 0 (x,0) and 1 (0,x) shared by every yv,
-      // block 2+3u+j the yv-dependent (x,y), (y,x), (x,d); (0,d) is (0,x) at the
-      // site d = x - y
-      int crd[4], dsh[4];
-      site_coords(site, global, crd);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) dsh[r] = mod_pos(crd[r] - coord_dim(yv, r), geom_dim(global, r));
-      const size_t sd = (((size_t)dsh[0] * global.LX + dsh[1]) * global.LY + dsh[2]) * global.LZ + dsh[3];
+      // surrogate channel rows of the yv-dependent evaluations (x,y), (y,x), (x,d)
       rows.given = true;
       rows.p[0] = chbuf + ((size_t)(2 + 3 * u + 0) * V + site) * qed::kNCh;
-      rows.p[1] = chbuf + ((size_t)0 * V + site) * qed::kNCh;
       rows.p[2] = chbuf + ((size_t)(2 + 3 * u + 1) * V + site) * qed::kNCh;
-      rows.p[3] = chbuf + ((size_t)1 * V + site) * qed::kNCh;
       rows.p[4] = chbuf + ((size_t)(2 + 3 * u + 2) * V + site) * qed::kNCh;
-      rows.p[5] = chbuf + ((size_t)1 * V + sd) * qed::kNCh;
     }
-    kfield_wave<float, float, float>(table32, tt, g, rows, cst32 + (size_t)u * 2 * qed::kNL, w[wave],
-                                     coef + i * kCoefField);
+    kfield_wave_axes<float, float, float>(table32, tt, g, rows, cst32 + (size_t)u * 2 * qed::kNL,
+                                          ax + site * qed::kNL, ax + (V + site) * qed::kNL,
+                                          ax + (V + sd) * qed::kNL, w[wave], coef + i * kCoefField);
   }
 }
 
@@ -641,10 +669,14 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
     hipLaunchKernelGGL(kfield_consts_win, dim3(nu), dim3(64), 0, st, c->qed->table64.p, w.d_yvs.p, g,
                        w.xunit, w.cst64.p, w.cst32.p);
     check_hip(hipGetLastError(), "kfield_consts_win launch");
+    const int nwa = (int)std::min<size_t>(div_up(V, kKmThreads / 64), (size_t)4 * c->n_cu);
+    hipLaunchKernelGGL(kfield_axes, dim3(nwa), dim3(kKmThreads), 0, st, c->qed->table32.p, g, w.xunit,
+                       w.form == 2 ? w.chbuf.p : nullptr, w.ax.p);
+    check_hip(hipGetLastError(), "kfield_axes launch");
     const int nwg = (int)std::min<size_t>(div_up((size_t)nu * V, kKmThreads / 64), (size_t)4 * c->n_cu);
     hipLaunchKernelGGL(kfield_mixed, dim3(nwg), dim3(kKmThreads), 0, st, c->qed->table32.p,
                        w.d_yvs.p, nu, g, w.xunit, w.cst32.p, w.form == 2 ? w.chbuf.p : nullptr,
-                       w.coef.p);
+                       w.ax.p, w.coef.p);
     check_hip(hipGetLastError(), "kfield_mixed launch");
     // candidate boxes: ceil(kNearRadius / xunit) lattice units, at most the whole extent
     hlbl_coord box;

@@ -110,6 +110,7 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
     p->bsplit.ensure(window_bsplit_len(n));
     p->bexp.ensure(window_bexp_len(n));
     p->coef.ensure((size_t)nu * V * 640);
+    p->ax.ensure((size_t)2 * V * qed::kNL);
     p->cst64.ensure((size_t)nu * 2 * qed::kNL);
     p->cst32.ensure((size_t)nu * 2 * qed::kNL);
     if (form == 2) p->chbuf.ensure((size_t)(2 + 3 * nu) * V * qed::kNCh);
