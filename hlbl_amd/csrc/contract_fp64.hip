@@ -112,7 +112,7 @@ constexpr int kKfThreads = 256;  // 4 waves, one site each at a time
 // the pair constants [L0(0,y) | L0(y,0)] into cst[768], one wave
 __global__ __launch_bounds__(64) void kfield_consts64(const double* __restrict__ table, hlbl_geom global,
                                                       hlbl_coord yv, double xunit, double* __restrict__ cst) {
-  __shared__ TermLds tt;
+  __shared__ TermLds<double> tt;
   __shared__ double w[kKwLen];
   __shared__ double c[2 * qed::kNL];
   stage_terms(c_terms64, &tt);
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(kKfThreads) void kfield_fp64(
     const double* __restrict__ table, hlbl_geom global, hlbl_geom local, hlbl_coord proc,
     hlbl_coord gsx, hlbl_coord yv, double xunit, const double* __restrict__ cst,
     double* __restrict__ coef) {
-  __shared__ TermLds tt;
+  __shared__ TermLds<double> tt;
   __shared__ double w[kKfThreads / 64][kKwLen];
   stage_terms(c_terms64, &tt);
   __syncthreads();

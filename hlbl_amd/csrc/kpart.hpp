@@ -141,8 +141,22 @@ struct ChannelRows {
 //   D[48]   subtraction matrices of xm, ym, dm
 constexpr int kKwLen = 6 * 32 + qed::kNT + 3 * qed::kNL + kCoefField + 48;
 
-// L0 sparse terms, staged in LDS by the workgroup
-struct TermLds {
+// L0 sparse terms, staged in LDS by the workgroup, one layout per precision.
+// FP32: each output's terms padded to kTermPad with zero-coefficient terms
+// (coefficient 0, index 0): an output is a fixed kMaxTerms-term sum read in two
+// 16-byte loads (no dependent reads). FP64, where the arithmetic costs more than
+// the latency: the compact rows, each output looping over its nterm terms, outputs
+// taken in order of decreasing term count.
+constexpr int kTermPad = 16;
+static_assert(qed::kMaxTerms <= 10, "terms 0..7 in one 16-byte load, 8..9 in one 4-byte load");
+template <class R>
+struct TermLds;
+template <>
+struct TermLds<float> {
+  __attribute__((aligned(16))) unsigned short term[qed::kNL][kTermPad];
+};
+template <>
+struct TermLds<double> {
   unsigned short term[qed::kNL][qed::kMaxTerms];
   unsigned short order[qed::kNL];  // outputs by decreasing term count
   unsigned char nterm[qed::kNL];
@@ -155,7 +169,13 @@ __device__ inline void wave_sync() {
 }
 
 // copy the term table into LDS; the whole workgroup calls it, then a barrier
-__device__ inline void stage_terms(const qed::TermTable& src, TermLds* dst) {
+__device__ inline void stage_terms(const qed::TermTable& src, TermLds<float>* dst) {
+  for (int i = threadIdx.x; i < qed::kNL * kTermPad; i += blockDim.x) {
+    const int o = i / kTermPad, j = i % kTermPad;
+    dst->term[o][j] = j < src.nterm[o] ? src.term[o][j] : (unsigned short)0;
+  }
+}
+__device__ inline void stage_terms(const qed::TermTable& src, TermLds<double>* dst) {
   const unsigned short* s = &src.term[0][0];
   unsigned short* d = &dst->term[0][0];
   for (int i = threadIdx.x; i < qed::kNL * qed::kMaxTerms; i += blockDim.x) d[i] = s[i];
@@ -165,16 +185,33 @@ __device__ inline void stage_terms(const qed::TermTable& src, TermLds* dst) {
   }
 }
 
-template <class R>
-__device__ inline R l0_lds(const TermLds& tt, int o, const R* t) {
-  R r = R(0);
-  const int n = tt.nterm[o];
-  for (int j = 0; j < n; ++j) {
-    const unsigned short e = tt.term[o][j];
-    r += R(qed::term_coef(e)) * t[qed::term_index(e)];
+// L0 output o: FP32, the padded terms in term order, all term reads and all T
+// reads independent of each other
+__device__ inline float l0_lds(const TermLds<float>& tt, int o, const float* t) {
+  const uint4 w = *reinterpret_cast<const uint4*>(&tt.term[o][0]);
+  const unsigned w8 = *reinterpret_cast<const unsigned*>(&tt.term[o][8]);
+  const unsigned words[5] = {w.x, w.y, w.z, w.w, w8};
+  float r = 0.f;
+#pragma unroll
+  for (int j = 0; j < qed::kMaxTerms; ++j) {
+    const unsigned short e = (unsigned short)(words[j >> 1] >> (16 * (j & 1)));
+    r += float(qed::term_coef(e)) * t[qed::term_index(e)];
   }
   return r;
 }
+// FP64: its nterm terms
+__device__ inline double l0_lds(const TermLds<double>& tt, int o, const double* t) {
+  double r = 0.0;
+  const int n = tt.nterm[o];
+  for (int j = 0; j < n; ++j) {
+    const unsigned short e = tt.term[o][j];
+    r += double(qed::term_coef(e)) * t[qed::term_index(e)];
+  }
+  return r;
+}
+// the output of round slot t
+__device__ inline int l0_output(const TermLds<float>&, int t) { return t; }
+__device__ inline int l0_output(const TermLds<double>& tt, int t) { return tt.order[t]; }
 
 // the channels of the six site evaluations into ch[6][32]: from the rows when
 // given, else interpolated in the table
@@ -195,16 +232,15 @@ __device__ __forceinline__ void wave_channels(const TT* __restrict__ table, cons
 
 // T tensors and L0 of evaluation e (0..7) into dst[384] from its channels ch[32]
 template <class R>
-__device__ __forceinline__ void wave_l0_from(int e, const TermLds& tt, const SiteGeom<R>& g,
+__device__ __forceinline__ void wave_l0_from(int e, const TermLds<R>& tt, const SiteGeom<R>& g,
                                              const R* ch, R* T, R* dst) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int k = 0; k < qed::kNT / 64; ++k) T[64 * k + lane] = kp_tensor(e, 64 * k + lane, g, ch);
   wave_sync();
-  // outputs in order of decreasing term count, so the lanes of a round loop alike
 #pragma unroll 2
   for (int t = lane; t < qed::kNL; t += 64) {
-    const int o = tt.order[t];
+    const int o = l0_output(tt, t);
     dst[o] = l0_lds(tt, o, T);
   }
   wave_sync();
@@ -212,7 +248,7 @@ __device__ __forceinline__ void wave_l0_from(int e, const TermLds& tt, const Sit
 
 // channels (interpolated), T tensors and L0 of evaluation e into dst[384]
 template <class R, class TT>
-__device__ __forceinline__ void wave_l0(int e, const TT* __restrict__ table, const TermLds& tt,
+__device__ __forceinline__ void wave_l0(int e, const TT* __restrict__ table, const TermLds<R>& tt,
                                         const SiteGeom<R>& g, R* ch, R* T, R* dst) {
   const int lane = threadIdx.x & 63;
   if (lane < qed::kNChUsed) ch[lane] = kp_channel(e, lane, g, table);
@@ -259,7 +295,7 @@ __device__ inline R coef_h(int row, int hc, const SiteGeom<R>& g, const R* K3) {
 // calling wave in w (kKwLen elements of LDS). cst = [L0(0,y) | L0(y,0)] of the
 // pair (768 elements, any address space readable by the wave).
 template <class R, class TT, class O>
-__device__ __forceinline__ void kfield_wave(const TT* __restrict__ table, const TermLds& tt,
+__device__ __forceinline__ void kfield_wave(const TT* __restrict__ table, const TermLds<R>& tt,
                                             const SiteGeom<R>& g, const ChannelRows rows,
                                             const R* cst, R* w, O* __restrict__ out) {
   const int lane = threadIdx.x & 63;
@@ -299,7 +335,7 @@ __device__ __forceinline__ void kfield_wave(const TT* __restrict__ table, const 
 // x0[384] and ox[384] (one wave per site; w: kKwLen elements of LDS). The same
 // functions as in kfield_wave, so the rows equal what it computes in place.
 template <class R, class TT, class O>
-__device__ __forceinline__ void kfield_wave_axis_rows(const TT* __restrict__ table, const TermLds& tt,
+__device__ __forceinline__ void kfield_wave_axis_rows(const TT* __restrict__ table, const TermLds<R>& tt,
                                                       const SiteGeom<R>& g, const ChannelRows rows, R* w,
                                                       O* __restrict__ x0, O* __restrict__ ox) {
   const int lane = threadIdx.x & 63;
@@ -326,7 +362,7 @@ __device__ __forceinline__ void kfield_wave_axis_rows(const TT* __restrict__ tab
 // L0(0, d), the (0, x) row of the site d = x - y. Only (x, y), (y, x) and (x, d)
 // are evaluated here.
 template <class R, class TT, class O>
-__device__ __forceinline__ void kfield_wave_axes(const TT* __restrict__ table, const TermLds& tt,
+__device__ __forceinline__ void kfield_wave_axes(const TT* __restrict__ table, const TermLds<R>& tt,
                                                  const SiteGeom<R>& g, const ChannelRows rows,
                                                  const R* cst, const float* __restrict__ x0,
                                                  const float* __restrict__ ox, const float* __restrict__ od,
@@ -388,7 +424,7 @@ __device__ __forceinline__ void kfield_wave_axes(const TT* __restrict__ table, c
 // The pair constants [L0(0,y) | L0(y,0)] into cst (768 elements), by one wave
 // (w: kKwLen elements of LDS).
 template <class R, class TT>
-__device__ __forceinline__ void kfield_wave_constants(const TT* __restrict__ table, const TermLds& tt,
+__device__ __forceinline__ void kfield_wave_constants(const TT* __restrict__ table, const TermLds<R>& tt,
                                                       const SiteGeom<R>& g, R* w, R* cst) {
   R* ch = w;
   R* T = ch + 6 * 32;

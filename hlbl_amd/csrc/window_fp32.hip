@@ -50,7 +50,7 @@ __global__ __launch_bounds__(64) void kfield_consts_win(const double* __restrict
                                                         hlbl_geom global, double xunit,
                                                         double* __restrict__ cst64,
                                                         float* __restrict__ cst32) {
-  __shared__ TermLds tt;
+  __shared__ TermLds<double> tt;
   __shared__ double w[kKwLen];
   __shared__ double c[2 * qed::kNL];
   stage_terms(c_terms32, &tt);
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(64) void kfield_consts_win(const double* __restrict
 __global__ __launch_bounds__(kKmThreads) void kfield_axes(const float* __restrict__ table32, hlbl_geom global,
                                                           double xunit, const float* __restrict__ chbuf,
                                                           float* __restrict__ ax) {
-  __shared__ TermLds tt;
+  __shared__ TermLds<float> tt;
   __shared__ float w[kKmThreads / 64][kKwLen];
   stage_terms(c_terms32, &tt);
   __syncthreads();
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kKmThreads) void kfield_mixed(
     const float* __restrict__ table32, const hlbl_coord* __restrict__ yvs, int n_unique,
     hlbl_geom global, double xunit, const float* __restrict__ cst32,
     const float* __restrict__ chbuf, const float* __restrict__ ax, float* __restrict__ coef) {
-  __shared__ TermLds tt;
+  __shared__ TermLds<float> tt;
   __shared__ float w[kKmThreads / 64][kKwLen];
   stage_terms(c_terms32, &tt);
   __syncthreads();
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kKmThreads) void kfield_near(
     const double* __restrict__ table64, const hlbl_coord* __restrict__ yvs, int n_unique,
     hlbl_geom global, double xunit, const double* __restrict__ cst64, hlbl_coord box,
     float* __restrict__ coef) {
-  __shared__ TermLds tt;
+  __shared__ TermLds<double> tt;
   __shared__ double w[kKmThreads / 64][kKwLen];
   stage_terms(c_terms32, &tt);
   __syncthreads();
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(64) void judge_coef(
     const double* __restrict__ table64, const hlbl_coord* __restrict__ yvs, hlbl_geom global,
     double xunit, const double* __restrict__ cst64, const size_t* __restrict__ samples,
     int n_samples, const float* __restrict__ coef, double* __restrict__ err) {
-  __shared__ TermLds tt;
+  __shared__ TermLds<double> tt;
   __shared__ double w[kKwLen];
   __shared__ double ref[kCoefField];
   stage_terms(c_terms32, &tt);
