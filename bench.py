@@ -32,6 +32,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by a 1 GiB copy's reading), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_24x48_window.txt")
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes" and f[1] == kernel:
+                return float(f[2])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(seconds_target=12.0):
     """The oracle (CPU restatement, FP64, OpenMP) on a bounded sample: whole pairs
     on an 8^4 lattice, repeated for about `seconds_target`, scaled per site to 24^3x48."""
@@ -174,6 +188,7 @@ def main():
     V = lat.volume()
     bytes_pass2 = 65 * (2 * 2 * 12 * 24 * 8 + 640 * 4) * V
     achieved = bytes_pass2 / (ms[3] * 1e-3)
+    traffic = measured_traffic("pass2_window")
     cpu = None if args.no_cpu else cpu_baseline()
     names = {1: "FP32 window-batched (pass-2 products as scaled FP16 hi/lo x3), plain QED kernel",
              2: "FP32 window-batched (pass-2 products as scaled FP16 hi/lo x3), MLP surrogate channels"}
@@ -186,7 +201,7 @@ def main():
                                f"30 resident FP64 blocks, seed 2; {names[head]}",
                    "lattice": list(DIMS), "pairs_per_step": 65, "xunit": XUNIT, "form": head},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": MI355X_HBM / 1e9,
-                     "unit": "GB/s", "frac": achieved / MI355X_HBM, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / MI355X_HBM, "traffic": traffic,
                      "kernel": "pass2_window", "ms": ms[3]},
         "max_pair_rel_err": float(err),
         "plain_fp64_pairs_per_s": 1.0 / dt64,
