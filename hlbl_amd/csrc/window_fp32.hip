@@ -284,10 +284,16 @@ __device__ inline float f16_float(unsigned short u) {
   __builtin_memcpy(&h, &u, 2);
   return (float)h;
 }
-// x = hi + lo in FP16
-__device__ inline void f16_split(float x, unsigned short& hi, unsigned short& lo) {
-  hi = f16_bits(x);
-  lo = f16_bits(x - f16_float(hi));
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+// (x, y) = hi + lo in FP16, each packed as (x bits, y bits) (v_cvt_pk_f16_f32: two
+// values per conversion, nearest, ties to even)
+__device__ inline void f16_split2(float x, float y, unsigned& hi, unsigned& lo) {
+  const f2v v = {x, y};
+  const h2v h = __builtin_convertvector(v, h2v);
+  const h2v l = __builtin_convertvector(v - __builtin_convertvector(h, f2v), h2v);
+  __builtin_memcpy(&hi, &h, 4);
+  __builtin_memcpy(&lo, &l, 4);
 }
 
 // Column exponents of Bg: per (pair, flavour, col < 40), te = kS2AMax - E with
@@ -470,7 +476,7 @@ __device__ inline int site_exp(unsigned ms, unsigned my) {
 template <int N>
 __device__ inline void s2_dxu(int t0, int lane, const float* buf, const unsigned* smax, unsigned* ah,
                               unsigned* al, int* sse) {
-#pragma unroll 1
+#pragma unroll 2
   for (int i = 0; i < N; ++i) {
     const int t = t0 + 64 * i + lane;
     const int zi = t / 144, r = (t / 12) % 12, a = t % 12;
@@ -480,12 +486,11 @@ __device__ inline void s2_dxu(int t0, int lane, const float* buf, const unsigned
     dxu_vals<float, float>(buf + a * kS2RowPad + zi * 24, buf + (12 + r) * kS2RowPad + zi * 24, re, im);
 #pragma unroll
     for (int mu = 0; mu < 4; ++mu) {
-      unsigned short hr, lr, hi, li;
-      f16_split(ldexpf(re[mu], se), hr, lr);
-      f16_split(ldexpf(im[mu], se), hi, li);
+      unsigned hw, lw;
+      f16_split2(ldexpf(re[mu], se), ldexpf(im[mu], se), hw, lw);
       const int o = ((zi * 4 + mu) * kS2ARow + r * 24 + 2 * a) / 2;
-      ah[o] = (unsigned)hr | ((unsigned)hi << 16);
-      al[o] = (unsigned)lr | ((unsigned)li << 16);
+      ah[o] = hw;
+      al[o] = lw;
     }
   }
 }
