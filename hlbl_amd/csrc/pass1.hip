@@ -48,6 +48,7 @@ __global__ __launch_bounds__(kPlThreads, 2) void pass1_planes(
     const WinPair* __restrict__ pairs, WinPair single, hlbl_geom global, hlbl_geom local,
     hlbl_coord proc, int flav_mask, T* __restrict__ part) {
   __shared__ T sh[kPlSites][24][24];  // [site][12 rows S_f[a], 12 rows Y_{1-f}[b]][24]
+  __shared__ T sw[kPlSites][2];       // the staged sites' weights xz_2, xz_3
   // blockIdx.x = pair * 2 + flavour (fastest, so that the pairs of a window that
   // share a block read its plane close together in time), blockIdx.y = plane
   const int ip = blockIdx.x >> 1, f = blockIdx.x & 1;
@@ -86,12 +87,15 @@ __global__ __launch_bounds__(kPlThreads, 2) void pass1_planes(
       sh[zi][row][2 * k] = (T)v.x;
       sh[zi][row][2 * k + 1] = (T)v.y;
     }
+    if (tid < kPlSites) {
+      const int yz = z0 + tid;
+      const int yl = yz / local.LZ, zl = yz - yl * local.LZ;
+      sw[tid][0] = (T)coord_map_zerohalf(mod_pos(yl + oy, global.LY), global.LY);
+      sw[tid][1] = (T)coord_map_zerohalf(mod_pos(zl + oz, global.LZ), global.LZ);
+    }
     __syncthreads();
     for (int zi = half; zi < ns; zi += 2) {
-      const int yz = z0 + zi;
-      const int yl = yz / local.LZ, zl = yz - yl * local.LZ;
-      const T wy = (T)coord_map_zerohalf(mod_pos(yl + oy, global.LY), global.LY);
-      const T wz = (T)coord_map_zerohalf(mod_pos(zl + oz, global.LZ), global.LZ);
+      const T wy = sw[zi][0], wz = sw[zi][1];
       T sr[3][3], si[3][3], yr[3][3], yi[3][3];
 #pragma unroll
       for (int i = 0; i < 3; ++i)
@@ -106,20 +110,25 @@ __global__ __launch_bounds__(kPlThreads, 2) void pass1_planes(
       for (int ia = 0; ia < 3; ++ia)
 #pragma unroll
         for (int ib = 0; ib < 3; ++ib) {
-          T re = sr[ia][0] * yr[ib][0] + si[ia][0] * yi[ib][0];
-          T im = si[ia][0] * yr[ib][0] - sr[ia][0] * yi[ib][0];
+          // S conj(Y) over the three colours as fused multiply-add chains
+          T re = sr[ia][0] * yr[ib][0];
+          T im = si[ia][0] * yr[ib][0];
+          re = fma(si[ia][0], yi[ib][0], re);
+          im = fma(-sr[ia][0], yi[ib][0], im);
 #pragma unroll
           for (int col = 1; col < 3; ++col) {
-            re += sr[ia][col] * yr[ib][col] + si[ia][col] * yi[ib][col];
-            im += si[ia][col] * yr[ib][col] - sr[ia][col] * yi[ib][col];
+            re = fma(sr[ia][col], yr[ib][col], re);
+            re = fma(si[ia][col], yi[ib][col], re);
+            im = fma(si[ia][col], yr[ib][col], im);
+            im = fma(-sr[ia][col], yi[ib][col], im);
           }
           const int o = ia * 3 + ib;
           acc[0][o][0] += re;
           acc[0][o][1] += im;
-          acc[1][o][0] += wy * re;
-          acc[1][o][1] += wy * im;
-          acc[2][o][0] += wz * re;
-          acc[2][o][1] += wz * im;
+          acc[1][o][0] = fma(wy, re, acc[1][o][0]);
+          acc[1][o][1] = fma(wy, im, acc[1][o][1]);
+          acc[2][o][0] = fma(wz, re, acc[2][o][0]);
+          acc[2][o][1] = fma(wz, im, acc[2][o][1]);
         }
     }
   }
