@@ -262,15 +262,68 @@ __device__ __forceinline__ void wave_l0(int e, const TT* __restrict__ table, con
 template <class R>
 __device__ __forceinline__ void wave_kernel(int m, R* main, const R* l00y, const R* l0x0,
                                             const R* du, const R* dv, bool in, R* C) {
+  // the lane's outputs o = lane + 64 kk share (a, b, c) = ((o >> 4) & 3, (o >> 2) & 3, o & 3),
+  // so their rows of the subtraction matrices are loaded once (qed::lambda_subtract:
+  // the same terms in the same order)
   const int lane = threadIdx.x & 63;
+  const int c = lane & 3, b = (lane >> 2) & 3, a = (lane >> 4) & 3;  // of every o = lane + 64 kk
+  R dm[4], dn[4];
+#pragma unroll
+  for (int al = 0; al < 4; ++al) {
+    dm[al] = du[a * 4 + al];
+    dn[al] = dv[b * 4 + al];
+  }
 #pragma unroll 2
-  for (int o = lane; o < qed::kNL; o += 64) {
-    const R k = in ? qed::lambda_subtract(o, main, l00y, l0x0, du, dv) : R(0);
+  for (int kk = 0; kk < qed::kNL / 64; ++kk) {
+    const int o = lane + 64 * kk;
+    R k = main[o];
+#pragma unroll
+    for (int al = 0; al < 4; ++al) {
+      k -= dm[al] * l00y[((kk * 4 + al) * 4 + b) * 4 + c];
+      k -= dn[al] * l0x0[((kk * 4 + a) * 4 + al) * 4 + c];
+    }
+    k = in ? k : R(0);
     main[o] = k;
-    const int c = o & 3, b = (o >> 2) & 3, a = (o >> 4) & 3, kk = o >> 6;
     if (m == 0) C[(a * 4 + c) * kCoefUsed + kk * 4 + b] = -k;       // K1[kk][mu=a][nu=b][lam=c]
     else if (m == 1) C[(b * 4 + c) * kCoefUsed + kk * 4 + a] -= k;  // K2[kk][nu=a][mu=b][lam=c]
     else C[(c * 4 + a) * kCoefUsed + kk * 4 + b] += k;              // K3[kk][lam=a][nu=b][mu=c]
+  }
+  wave_sync();
+}
+
+// FP32: L0 of evaluation e and K_m from it in one pass. With the padded term table a
+// lane's L0 outputs are o = lane + 64 kk, the same outputs it turns into K_m, so the
+// L0 value stays in a register (wave_l0_from followed by wave_kernel, without the
+// round trip of `main` through LDS and one wave_sync).
+__device__ __forceinline__ void wave_l0_kernel(int e, int m, const TermLds<float>& tt, const SiteGeom<float>& g,
+                                               const float* ch, float* T, float* main, const float* l00y,
+                                               const float* l0x0, const float* du, const float* dv, bool in,
+                                               float* C) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < qed::kNT / 64; ++k) T[64 * k + lane] = kp_tensor(e, 64 * k + lane, g, ch);
+  wave_sync();
+  const int c = lane & 3, b = (lane >> 2) & 3, a = (lane >> 4) & 3;
+  float dm[4], dn[4];
+#pragma unroll
+  for (int al = 0; al < 4; ++al) {
+    dm[al] = du[a * 4 + al];
+    dn[al] = dv[b * 4 + al];
+  }
+#pragma unroll 2
+  for (int kk = 0; kk < qed::kNL / 64; ++kk) {
+    const int o = lane + 64 * kk;
+    float k = l0_lds(tt, o, T);
+#pragma unroll
+    for (int al = 0; al < 4; ++al) {
+      k -= dm[al] * l00y[((kk * 4 + al) * 4 + b) * 4 + c];
+      k -= dn[al] * l0x0[((kk * 4 + a) * 4 + al) * 4 + c];
+    }
+    k = in ? k : 0.f;
+    main[o] = k;
+    if (m == 0) C[(a * 4 + c) * kCoefUsed + kk * 4 + b] = -k;
+    else if (m == 1) C[(b * 4 + c) * kCoefUsed + kk * 4 + a] -= k;
+    else C[(c * 4 + a) * kCoefUsed + kk * 4 + b] += k;
   }
   wave_sync();
 }
@@ -403,14 +456,12 @@ __device__ __forceinline__ void kfield_wave_axes(const TT* __restrict__ table, c
   const R* Dx = D;
   const R* Dy = D + 16;
   const R* Dd = D + 32;
-  wave_l0_from(0, tt, g, ch + 0 * 32, T, s2);                  // (x, y)
-  wave_kernel(0, s2, cst, s0, Dx, Dy, g.in12, C);              // K1 = L(x, y)
-  wave_l0_from(2, tt, g, ch + 2 * 32, T, s2);                  // (y, x)
-  wave_kernel(1, s2, s1, cst + qed::kNL, Dy, Dx, g.in12, C);   // K2 = L(y, x)
+  wave_l0_kernel(0, 0, tt, g, ch + 0 * 32, T, s2, cst, s0, Dx, Dy, g.in12, C);             // K1 = L(x, y)
+  wave_l0_kernel(2, 1, tt, g, ch + 2 * 32, T, s2, s1, cst + qed::kNL, Dy, Dx, g.in12, C);  // K2 = L(y, x)
 #pragma unroll
   for (int k = 0; k < qed::kNL / 64; ++k) s1[64 * k + lane] = (R)rod[k];  // (0, d)
-  wave_l0_from(4, tt, g, ch + 4 * 32, T, s2);                  // (x, d)
-  wave_kernel(2, s2, s1, s0, Dx, Dd, g.in3, C);                // K3 = L(x, d)
+  wave_sync();
+  wave_l0_kernel(4, 2, tt, g, ch + 4 * 32, T, s2, s1, s0, Dx, Dd, g.in3, C);               // K3 = L(x, d)
   for (int t = lane; t < 16 * 16; t += 64) {
     const int row = t >> 4, hc = t & 15;
     C[row * kCoefUsed + 24 + hc] = coef_h(row, hc, g, s2);
