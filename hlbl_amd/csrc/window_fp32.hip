@@ -495,13 +495,17 @@ __device__ inline void s2_dxu(int t0, int lane, const float* buf, const unsigned
   }
 }
 
-// Producer roles. Waves sit on SIMD (wave % 4): producer waves 12 and 13 are alone
-// on SIMDs 0 and 1 and take 5 dxu tasks per lane; 10, 14 (SIMD 2) and 11, 15
-// (SIMD 3) take 2 each and stage the coefficients of two sites each. Task ranges:
-// pw 2: [0, 320), pw 3: [320, 640), pw 0, 1, 4, 5: 128 each from 640.
-__device__ inline int s2_task0(int pw) { return pw == 2 ? 0 : pw == 3 ? 320 : pw == 0 ? 640 : pw == 1 ? 768 : pw == 4 ? 896 : 1024; }
+// Producer roles. Waves sit on SIMD (wave % 4), and on a SIMD the matrix and vector
+// work of its waves add up (they hardly co-issue), so the dxu tasks are spread to
+// even out MFMA + vector time per SIMD: SIMDs 0 and 1 carry three MFMA waves and
+// producer 12 or 13 with 4 tasks per lane; SIMDs 2 and 3 carry two MFMA waves and
+// producers 10 + 14 or 11 + 15 with 3 + 2 tasks, and these four stage the coefficients
+// of two sites each. Task ranges: pw 2: [0, 256), pw 3: [256, 512), pw 0: [512, 704),
+// pw 1: [704, 896), pw 4: [896, 1024), pw 5: [1024, 1152).
+__device__ inline int s2_task0(int pw) { return pw == 2 ? 0 : pw == 3 ? 256 : pw == 0 ? 512 : pw == 1 ? 704 : pw == 4 ? 896 : 1024; }
+__device__ inline int s2_n_tasks(int pw) { return (pw == 2 || pw == 3) ? 4 : (pw == 0 || pw == 1) ? 3 : 2; }
 __device__ inline int s2_coef_site0(int pw) { return pw == 0 ? 0 : pw == 1 ? 2 : pw == 4 ? 4 : pw == 5 ? 6 : -1; }
-static_assert(2 * 320 + 4 * 128 == kS2Tasks, "the producer task ranges cover a group");
+static_assert(2 * 256 + 2 * 192 + 2 * 128 == kS2Tasks, "the producer task ranges cover a group");
 
 __global__ __launch_bounds__(kS2Threads) void pass2_window(
     const WinPair* __restrict__ pairs, int n_pairs, const unsigned short* __restrict__ bsplit,
@@ -543,7 +547,9 @@ __global__ __launch_bounds__(kS2Threads) void pass2_window(
       }
       unsigned* ah = reinterpret_cast<unsigned*>(sA[it & 1][0]);
       unsigned* al = reinterpret_cast<unsigned*>(sA[it & 1][1]);
-      if (pw == 2 || pw == 3) s2_dxu<5>(t0, lane, sF[it & 1], sMax[it % 3], ah, al, sSe[it & 1]);
+      const int nt = s2_n_tasks(pw);
+      if (nt == 4) s2_dxu<4>(t0, lane, sF[it & 1], sMax[it % 3], ah, al, sSe[it & 1]);
+      else if (nt == 3) s2_dxu<3>(t0, lane, sF[it & 1], sMax[it % 3], ah, al, sSe[it & 1]);
       else s2_dxu<2>(t0, lane, sF[it & 1], sMax[it % 3], ah, al, sSe[it & 1]);
       nc.next(ng);
     };
