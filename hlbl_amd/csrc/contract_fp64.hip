@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kKfThreads) void kfield_fp64(
   __shared__ double w[kKfThreads / 64][kKwLen];
   stage_terms(c_terms64, &tt);
   __syncthreads();
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: site math on the scalar unit
   const size_t V = geom_volume(local);
   const size_t nw = (size_t)gridDim.x * (kKfThreads / 64);
   for (size_t site = (size_t)blockIdx.x * (kKfThreads / 64) + wave; site < V; site += nw) {

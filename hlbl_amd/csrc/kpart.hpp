@@ -140,20 +140,23 @@ struct ChannelRows {
 //           (one coefficient per K element), the H columns gather from K3 at the end
 //   D[48]   subtraction matrices of xm, ym, dm
 constexpr int kKwLen = 6 * 32 + qed::kNT + 3 * qed::kNL + kCoefField + 48;
+// the same for the FP32 window field (kfield_wave_axes, kfield_wave_axis_rows), which
+// keeps the channels of three evaluations: ch[3][32]
+constexpr int kKwLen3 = 3 * 32 + qed::kNT + 3 * qed::kNL + kCoefField + 48;
 
 // L0 sparse terms, staged in LDS by the workgroup, one layout per precision.
-// FP32: each output's terms padded to kTermPad with zero-coefficient terms
-// (coefficient 0, index 0): an output is a fixed kMaxTerms-term sum read in two
-// 16-byte loads (no dependent reads). FP64, where the arithmetic costs more than
+// FP32: each output's terms padded to kMaxTerms with zero-coefficient terms
+// (coefficient 0, index 0): an output is a fixed kMaxTerms-term sum, terms 0..7 in one
+// 16-byte row and term 8 beside it, read with two loads (no dependent reads). FP64, where the arithmetic costs more than
 // the latency: the compact rows, each output looping over its nterm terms, outputs
 // taken in order of decreasing term count.
-constexpr int kTermPad = 16;
-static_assert(qed::kMaxTerms <= 10, "terms 0..7 in one 16-byte load, 8..9 in one 4-byte load");
+static_assert(qed::kMaxTerms == 9, "terms 0..7 in one 16-byte row, term 8 beside it");
 template <class R>
 struct TermLds;
 template <>
 struct TermLds<float> {
-  __attribute__((aligned(16))) unsigned short term[qed::kNL][kTermPad];
+  __attribute__((aligned(16))) unsigned short term[qed::kNL][8];
+  unsigned short term8[qed::kNL];
 };
 template <>
 struct TermLds<double> {
@@ -170,9 +173,11 @@ __device__ inline void wave_sync() {
 
 // copy the term table into LDS; the whole workgroup calls it, then a barrier
 __device__ inline void stage_terms(const qed::TermTable& src, TermLds<float>* dst) {
-  for (int i = threadIdx.x; i < qed::kNL * kTermPad; i += blockDim.x) {
-    const int o = i / kTermPad, j = i % kTermPad;
-    dst->term[o][j] = j < src.nterm[o] ? src.term[o][j] : (unsigned short)0;
+  for (int i = threadIdx.x; i < qed::kNL * qed::kMaxTerms; i += blockDim.x) {
+    const int o = i / qed::kMaxTerms, j = i % qed::kMaxTerms;
+    const unsigned short e = j < src.nterm[o] ? src.term[o][j] : (unsigned short)0;
+    if (j < 8) dst->term[o][j] = e;
+    else dst->term8[o] = e;
   }
 }
 __device__ inline void stage_terms(const qed::TermTable& src, TermLds<double>* dst) {
@@ -189,7 +194,7 @@ __device__ inline void stage_terms(const qed::TermTable& src, TermLds<double>* d
 // reads independent of each other
 __device__ inline float l0_lds(const TermLds<float>& tt, int o, const float* t) {
   const uint4 w = *reinterpret_cast<const uint4*>(&tt.term[o][0]);
-  const unsigned w8 = *reinterpret_cast<const unsigned*>(&tt.term[o][8]);
+  const unsigned w8 = tt.term8[o];
   const unsigned words[5] = {w.x, w.y, w.z, w.w, w8};
   float r = 0.f;
 #pragma unroll
@@ -392,19 +397,19 @@ __device__ __forceinline__ void kfield_wave_axis_rows(const TT* __restrict__ tab
                                                       const SiteGeom<R>& g, const ChannelRows rows, R* w,
                                                       O* __restrict__ x0, O* __restrict__ ox) {
   const int lane = threadIdx.x & 63;
-  R* ch = w;
-  R* T = ch + 6 * 32;
+  R* ch = w;  // [2][32]: (x, 0), (0, x)
+  R* T = ch + 3 * 32;
   R* s0 = T + qed::kNT;
   if (lane < qed::kNChUsed) {
-    ch[1 * 32 + lane] = rows.given ? (R)rows.row(1)[lane] : kp_channel(1, lane, g, table);
-    ch[3 * 32 + lane] = rows.given ? (R)rows.row(3)[lane] : kp_channel(3, lane, g, table);
+    ch[0 * 32 + lane] = rows.given ? (R)rows.row(1)[lane] : kp_channel(1, lane, g, table);
+    ch[1 * 32 + lane] = rows.given ? (R)rows.row(3)[lane] : kp_channel(3, lane, g, table);
   }
   wave_sync();
-  wave_l0_from(1, tt, g, ch + 1 * 32, T, s0);  // (x, 0)
+  wave_l0_from(1, tt, g, ch + 0 * 32, T, s0);  // (x, 0)
 #pragma unroll
   for (int k = 0; k < qed::kNL / 64; ++k) x0[64 * k + lane] = (O)s0[64 * k + lane];
   wave_sync();
-  wave_l0_from(3, tt, g, ch + 3 * 32, T, s0);  // (0, x)
+  wave_l0_from(3, tt, g, ch + 1 * 32, T, s0);  // (0, x)
 #pragma unroll
   for (int k = 0; k < qed::kNL / 64; ++k) ox[64 * k + lane] = (O)s0[64 * k + lane];
   wave_sync();
@@ -412,30 +417,29 @@ __device__ __forceinline__ void kfield_wave_axis_rows(const TT* __restrict__ tab
 
 // kfield_wave with the site-only evaluations read from rows computed by
 // kfield_wave_axis_rows: x0 = L0(x, 0) and ox = L0(0, x) of this site, od =
-// L0(0, d), the (0, x) row of the site d = x - y. Only (x, y), (y, x) and (x, d)
-// are evaluated here.
-template <class R, class TT, class O>
-__device__ __forceinline__ void kfield_wave_axes(const TT* __restrict__ table, const TermLds<R>& tt,
-                                                 const SiteGeom<R>& g, const ChannelRows rows,
-                                                 const R* cst, const float* __restrict__ x0,
+// L0(0, d), the (0, x) row of the site d = x - y (all 16-byte aligned). Only (x, y),
+// (y, x) and (x, d) are evaluated here. w holds kKwLen3 floats.
+template <class TT, class O>
+__device__ __forceinline__ void kfield_wave_axes(const TT* __restrict__ table, const TermLds<float>& tt,
+                                                 const SiteGeom<float>& g, const ChannelRows rows,
+                                                 const float* cst, const float* __restrict__ x0,
                                                  const float* __restrict__ ox, const float* __restrict__ od,
-                                                 R* w, O* __restrict__ out) {
+                                                 float* w, O* __restrict__ out) {
   const int lane = threadIdx.x & 63;
-  R* ch = w;
-  R* T = ch + 6 * 32;
-  R* L = T + qed::kNT;
-  R* C = L + 3 * qed::kNL;
-  R* D = C + kCoefField;
-  R* s0 = L;
-  R* s1 = L + qed::kNL;
-  R* s2 = L + 2 * qed::kNL;
-  // the three rows, loaded first (one latency)
-  float rx0[qed::kNL / 64], rox[qed::kNL / 64], rod[qed::kNL / 64];
+  float* ch = w;  // [3][32]: (x, y), (y, x), (x, d)
+  float* T = ch + 3 * 32;
+  float* L = T + qed::kNT;
+  float* C = L + 3 * qed::kNL;
+  float* D = C + kCoefField;
+  float* s0 = L;
+  float* s1 = L + qed::kNL;
+  float* s2 = L + 2 * qed::kNL;
+  // the site's rows (x, 0) and (0, x), loaded first (one latency with the channels)
+  float rx0[qed::kNL / 64], rox[qed::kNL / 64];
 #pragma unroll
   for (int k = 0; k < qed::kNL / 64; ++k) {
     rx0[k] = x0[64 * k + lane];
     rox[k] = ox[64 * k + lane];
-    rod[k] = od[64 * k + lane];
   }
   if (lane < 16) kp_dmat(lane, g, D);
   // channels of the evaluations 0 (x, y), 2 (y, x), 4 (x, d)
@@ -444,24 +448,25 @@ __device__ __forceinline__ void kfield_wave_axes(const TT* __restrict__ table, c
     const int t = lane + 64 * k;
     if (t < 3 * qed::kNChUsed) {
       const int j = t / qed::kNChUsed, c = t - j * qed::kNChUsed, e = 2 * j;
-      ch[e * 32 + c] = rows.given ? (R)rows.row(e)[c] : kp_channel(e, c, g, table);
+      ch[j * 32 + c] = rows.given ? rows.row(e)[c] : kp_channel(e, c, g, table);
     }
   }
 #pragma unroll
   for (int k = 0; k < qed::kNL / 64; ++k) {
-    s0[64 * k + lane] = (R)rx0[k];
-    s1[64 * k + lane] = (R)rox[k];
+    s0[64 * k + lane] = rx0[k];
+    s1[64 * k + lane] = rox[k];
   }
   wave_sync();
-  const R* Dx = D;
-  const R* Dy = D + 16;
-  const R* Dd = D + 32;
+  const float* Dx = D;
+  const float* Dy = D + 16;
+  const float* Dd = D + 32;
   wave_l0_kernel(0, 0, tt, g, ch + 0 * 32, T, s2, cst, s0, Dx, Dy, g.in12, C);             // K1 = L(x, y)
-  wave_l0_kernel(2, 1, tt, g, ch + 2 * 32, T, s2, s1, cst + qed::kNL, Dy, Dx, g.in12, C);  // K2 = L(y, x)
+  wave_l0_kernel(2, 1, tt, g, ch + 1 * 32, T, s2, s1, cst + qed::kNL, Dy, Dx, g.in12, C);  // K2 = L(y, x)
+  // the (0, d) row into s1, loaded now so that its registers are not live above
 #pragma unroll
-  for (int k = 0; k < qed::kNL / 64; ++k) s1[64 * k + lane] = (R)rod[k];  // (0, d)
+  for (int k = 0; k < qed::kNL / 64; ++k) s1[64 * k + lane] = od[64 * k + lane];
   wave_sync();
-  wave_l0_kernel(4, 2, tt, g, ch + 4 * 32, T, s2, s1, s0, Dx, Dd, g.in3, C);               // K3 = L(x, d)
+  wave_l0_kernel(4, 2, tt, g, ch + 2 * 32, T, s2, s1, s0, Dx, Dd, g.in3, C);               // K3 = L(x, d)
   for (int t = lane; t < 16 * 16; t += 64) {
     const int row = t >> 4, hc = t & 15;
     C[row * kCoefUsed + 24 + hc] = coef_h(row, hc, g, s2);

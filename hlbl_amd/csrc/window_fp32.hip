@@ -73,10 +73,10 @@ __global__ __launch_bounds__(kKmThreads) void kfield_axes(const float* __restric
                                                           double xunit, const float* __restrict__ chbuf,
                                                           float* __restrict__ ax) {
   __shared__ TermLds<float> tt;
-  __shared__ float w[kKmThreads / 64][kKwLen];
+  __shared__ float w[kKmThreads / 64][kKwLen3];
   stage_terms(c_terms32, &tt);
   __syncthreads();
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: site math on the scalar unit
   const size_t V = geom_volume(global);
   const size_t nw = (size_t)gridDim.x * (kKmThreads / 64);
   const hlbl_coord zero = {0, 0, 0, 0};
@@ -102,10 +102,10 @@ __global__ __launch_bounds__(kKmThreads) void kfield_mixed(
     hlbl_geom global, double xunit, const float* __restrict__ cst32,
     const float* __restrict__ chbuf, const float* __restrict__ ax, float* __restrict__ coef) {
   __shared__ TermLds<float> tt;
-  __shared__ float w[kKmThreads / 64][kKwLen];
+  __shared__ float w[kKmThreads / 64][kKwLen3];
   stage_terms(c_terms32, &tt);
   __syncthreads();
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: site math on the scalar unit
   const size_t V = geom_volume(global);
   const size_t total = (size_t)n_unique * V;
   const size_t nw = (size_t)gridDim.x * (kKmThreads / 64);
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(kKmThreads) void kfield_mixed(
       rows.p[2] = chbuf + ((size_t)(2 + 3 * u + 1) * V + site) * qed::kNCh;
       rows.p[4] = chbuf + ((size_t)(2 + 3 * u + 2) * V + site) * qed::kNCh;
     }
-    kfield_wave_axes<float, float, float>(table32, tt, g, rows, cst32 + (size_t)u * 2 * qed::kNL,
+    kfield_wave_axes<float, float>(table32, tt, g, rows, cst32 + (size_t)u * 2 * qed::kNL,
                                           ax + site * qed::kNL, ax + (V + site) * qed::kNL,
                                           ax + (V + sd) * qed::kNL, w[wave], coef + i * kCoefField);
   }
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kKmThreads) void kfield_near(
   __shared__ double w[kKmThreads / 64][kKwLen];
   stage_terms(c_terms32, &tt);
   __syncthreads();
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: site math on the scalar unit
   const size_t V = geom_volume(global);
   const int side[4] = {2 * box.t + 1, 2 * box.x + 1, 2 * box.y + 1, 2 * box.z + 1};
   const size_t per_box = (size_t)side[0] * side[1] * side[2] * side[3];
