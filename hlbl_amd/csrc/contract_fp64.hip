@@ -94,14 +94,13 @@ __global__ void form_gdz(const double* __restrict__ dz, double* __restrict__ gdz
 }
 
 // B operand of the pass-2 traces: bmat[f][r*24 + j][col] = bmat_value (pad columns 0)
-template <class B>
-__global__ void form_bmat(const double* __restrict__ gdz, B* __restrict__ bmat) {
+__global__ void form_bmat(const double* __restrict__ gdz, double* __restrict__ bmat) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (f, kidx, col)
   if (i >= kBmatLen) return;
   gdz += (size_t)blockIdx.y * kGdzLen;
   bmat += (size_t)blockIdx.y * kBmatLen;
   const int col = i % kCoefCols, kidx = (i / kCoefCols) % 288, f = i / (288 * kCoefCols);
-  bmat[i] = (B)(col < 40 ? bmat_value(gdz, f, kidx, col) : 0.0);
+  bmat[i] = col < 40 ? bmat_value(gdz, f, kidx, col) : 0.0;
 }
 
 // ---------------------------------------------------------------- pass 2 --
@@ -288,14 +287,9 @@ void launch_add(double* dst, const double* src, size_t n, hipStream_t st) {
   check_hip(hipGetLastError(), "add_kernel launch");
 }
 
-void launch_forms_batched(const double* xi, double* dz, double* gdz, double* bmat64,
-                          float* bmat32, int n_pairs, hipStream_t st) {
+void launch_forms_batched(const double* xi, double* dz, double* gdz, int n_pairs, hipStream_t st) {
   hipLaunchKernelGGL(form_dz, dim3(div_up(2 * 10 * 144, 256), n_pairs), dim3(256), 0, st, xi, dz);
   hipLaunchKernelGGL(form_gdz, dim3(div_up(2 * 10 * 4 * 144, 256), n_pairs), dim3(256), 0, st, dz, gdz);
-  if (bmat64)
-    hipLaunchKernelGGL(form_bmat<double>, dim3(div_up(kBmatLen, 256), n_pairs), dim3(256), 0, st, gdz, bmat64);
-  if (bmat32)
-    hipLaunchKernelGGL(form_bmat<float>, dim3(div_up(kBmatLen, 256), n_pairs), dim3(256), 0, st, gdz, bmat32);
   check_hip(hipGetLastError(), "form kernels launch");
 }
 
@@ -305,7 +299,7 @@ void launch_gamma(const double* dz, double* gdz, hipStream_t st) {
 }
 
 void launch_bmat(const double* gdz, double* bmat, hipStream_t st) {
-  hipLaunchKernelGGL(form_bmat<double>, dim3(div_up(kBmatLen, 256), 1), dim3(256), 0, st, gdz, bmat);
+  hipLaunchKernelGGL(form_bmat, dim3(div_up(kBmatLen, 256), 1), dim3(256), 0, st, gdz, bmat);
   check_hip(hipGetLastError(), "form_bmat launch");
 }
 

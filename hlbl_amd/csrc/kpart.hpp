@@ -19,7 +19,6 @@
 namespace hlbl {
 
 constexpr int kCoefCols = 48;  // 40 used + 8 pad (three 16-column MFMA blocks)
-constexpr int kCoefLen = 16 * kCoefCols;
 
 // Coordinates of one site relative to the pair.
 template <class R>

@@ -61,47 +61,6 @@ __device__ inline void dxu_site(const R* __restrict__ sa, const R* __restrict__ 
   }
 }
 
-// The same dxu as dxu_site, one sink spin s at a time (a rolled loop): the loads
-// of one step are 6 elements of Y_r and 12 of S_a, which keeps few registers
-// live (for kernels whose other waves need the register file).
-template <class R, class A, class O>
-__device__ inline void dxu_site_lean(const R* __restrict__ sa, const R* __restrict__ ur, O* out,
-                                     int stride) {
-  A dr[4] = {0, 0, 0, 0}, di[4] = {0, 0, 0, 0};
-#pragma unroll 1
-  for (int s = 0; s < 4; ++s) {
-    const int s0 = s ^ 2, s1 = 3 - s;
-    A x0r = 0, x0i = 0, x1r = 0, x1i = 0;
-#pragma unroll
-    for (int cc = 0; cc < 3; ++cc) {
-      const A vr = ur[2 * (3 * s + cc)], vi = ur[2 * (3 * s + cc) + 1];
-      const A t0r = sa[2 * (3 * s0 + cc)], t0i = sa[2 * (3 * s0 + cc) + 1];
-      const A t1r = sa[2 * (3 * s1 + cc)], t1i = sa[2 * (3 * s1 + cc) + 1];
-      x0r += t0r * vr + t0i * vi;
-      x0i += t0r * vi - t0i * vr;
-      x1r += t1r * vr + t1i * vi;
-      x1i += t1r * vi - t1i * vr;
-    }
-    const A eps = s < 2 ? A(1) : A(-1);
-#pragma unroll
-    for (int mu = 0; mu < 4; ++mu) {
-      const bool c0 = (mu == 0 || mu == 3);
-      const A ar = c0 ? x0r : x1r, ai = c0 ? x0i : x1i;
-      // (ar, ai) * i^q, q = 4 - ph_mu(s), by selects
-      const int q = (4 - gamma_phase(mu, s)) & 3;
-      const A rr = q == 0 ? ar : (q == 1 ? -ai : (q == 2 ? -ar : ai));
-      const A ri = q == 0 ? ai : (q == 1 ? ar : (q == 2 ? -ai : -ar));
-      dr[mu] += eps * rr;
-      di[mu] += eps * ri;
-    }
-  }
-#pragma unroll
-  for (int mu = 0; mu < 4; ++mu) {
-    out[mu * stride] = (O)(-dr[mu]);
-    out[mu * stride + 1] = (O)(-di[mu]);
-  }
-}
-
 // dxu_mu[r][a] of dxu_site, mu = 0..3, returned in (re[mu], im[mu]); the sink
 // spins unrolled, so the phases i^q are constants.
 template <class R, class A>
@@ -137,36 +96,6 @@ __device__ inline void dxu_vals(const R* __restrict__ sa, const R* __restrict__ 
       mul_ipow(ar, ai, 4 - gamma_phase(mu, s));
       re[mu] += neg ? -ar : ar;
       im[mu] += neg ? -ai : ai;
-    }
-  }
-}
-
-// The half of dxu_vals of one gamma class: CLS 0 gives mu = (0, 3), CLS 1 gives
-// mu = (1, 2), in (re[m], im[m]) for the m-th of the two; Xi[CLS][s] only.
-template <int CLS, class R, class A>
-__device__ inline void dxu_half(const R* __restrict__ sa, const R* __restrict__ ur, A (&re)[2], A (&im)[2]) {
-  re[0] = re[1] = im[0] = im[1] = A(0);
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int sp = CLS == 0 ? (s ^ 2) : (3 - s);
-    A xr = 0, xi = 0;
-#pragma unroll
-    for (int cc = 0; cc < 3; ++cc) {
-      const A vr = ur[2 * (3 * s + cc)], vi = ur[2 * (3 * s + cc) + 1];
-      const A tr = sa[2 * (3 * sp + cc)], ti = sa[2 * (3 * sp + cc) + 1];
-      xr = fma(tr, vr, xr);
-      xr = fma(ti, vi, xr);
-      xi = fma(tr, vi, xi);
-      xi = fma(-ti, vr, xi);
-    }
-    const bool neg = s < 2;
-#pragma unroll
-    for (int m = 0; m < 2; ++m) {
-      const int mu = CLS == 0 ? (m ? 3 : 0) : (m ? 2 : 1);
-      A ar = xr, ai = xi;
-      mul_ipow(ar, ai, 4 - gamma_phase(mu, s));
-      re[m] += neg ? -ar : ar;
-      im[m] += neg ? -ai : ai;
     }
   }
 }

@@ -63,10 +63,8 @@ size_t window_part2_len(const hlbl_contractor* c, int n_pairs);
 size_t window_bsplit_len(int n_pairs);  // FP16 elements of the pass-2 B operands
 size_t window_bexp_len(int n_pairs);    // their column exponents
 
-// dz, gdz (FP64) and the B operands (FP64 into bmat64 or FP32 into bmat32) of
-// n_pairs reduced pass-1 sums xi[pair][...]
-void launch_forms_batched(const double* xi, double* dz, double* gdz, double* bmat64,
-                          float* bmat32, int n_pairs, hipStream_t st);
+// dz and gdz (FP64) of n_pairs reduced pass-1 sums xi[pair][...]
+void launch_forms_batched(const double* xi, double* dz, double* gdz, int n_pairs, hipStream_t st);
 
 // surrogate channels of the six site evaluations, for every site and yv
 void launch_surrogate_channels(hlbl_contractor* c, WindowPlan& w, hipStream_t st);

@@ -622,6 +622,9 @@ __global__ __launch_bounds__(kS2Threads) void pass2_window(
     // coefficient (mu, lam)(col) of column g = lam*40 + col: index mu*160 + g; the
     // four sites zi = 2q + l/32 of this lane unscaled by 2^-(se + te) each
     const float* cs = sCoef[it & 1] + lh * kS2CoefSite + 32 * cb + lrow;
+    // sites past the end (only in a workgroup's last group): stale coefficients
+    // (in bounds), selected away
+    const int nlive = z0 + kS2Sites <= end ? kS2Sites : (int)(end - z0);
     double v = 0.0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -629,8 +632,8 @@ __global__ __launch_bounds__(kS2Threads) void pass2_window(
       float vq = 0.f;
 #pragma unroll
       for (int m = 0; m < 4; ++m) vq += c[4 * q + m] * cs[2 * q * kS2CoefSite + m * 160];
-      // sites past the end: stale coefficients (in bounds), selected away
-      v += z0 + zi < end ? (double)ldexpf(vq, -(sSe[it & 1][zi] + te)) : 0.0;
+      const double d = (double)ldexpf(vq, -(sSe[it & 1][zi] + te));
+      v += zi < nlive ? d : 0.0;
     }
     acc.add(v);
     if (cur.g == ng - 1) {
@@ -669,7 +672,7 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
   if (stage == 0) {
     launch_pass1_planes<float>(w.d_pairs.p, WinPair{}, np, g, g, c->proc, 3, w.part1.p, w.red1.p,
                                w.xi.p, st);
-    launch_forms_batched(w.xi.p, w.dz.p, w.gdz.p, nullptr, nullptr, np, st);
+    launch_forms_batched(w.xi.p, w.dz.p, w.gdz.p, np, st);
     hipLaunchKernelGGL(bsplit_exp, dim3(1, np), dim3(128), 0, st, w.gdz.p, w.bexp.p);
     hipLaunchKernelGGL(form_bsplit, dim3((unsigned)div_up(kBsplitLen, 256), np), dim3(256), 0, st, w.gdz.p,
                        w.bexp.p, w.bsplit.p);
