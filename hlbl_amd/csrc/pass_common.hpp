@@ -104,7 +104,7 @@ __device__ inline void dxu_vals(const R* __restrict__ sa, const R* __restrict__ 
 // gdz[f][10][4][12][12] (complex): j = 2i: X_re[i][r], j = 2i+1: -X_im[i][r],
 // X = G[k][nu] (col = k*4+nu < 24, gdz rows 0..5: dzu) or H[s][nu]
 // (col = 24 + s*4+nu, gdz rows 6..9: dzsu).
-HLBL_HD inline double bmat_value(const double* gdz, int f, int kidx, int col) {
+HLBL_HD double bmat_value(const double* gdz, int f, int kidx, int col) {
   const int r = kidx / 24, jj = kidx % 24, ii = jj >> 1, part = jj & 1;
   const int jrow = col < 24 ? (col >> 2) : 6 + ((col - 24) >> 2);
   const int nu = col & 3;

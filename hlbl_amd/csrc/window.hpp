@@ -33,6 +33,9 @@ struct WindowPlan {
   DevBuf<hlbl_coord> d_yvs;
   PinBuf<WinPair> h_pairs;
   PinBuf<hlbl_coord> h_yvs;
+  int unit_slots = 0;      // pass-1 unit lists: kPass1Groups x unit_slots (pair * 2 + f, or -1)
+  DevBuf<int> d_units;
+  PinBuf<int> h_units;
   DevBuf<float> part1;     // pass-1 plane partials
   DevBuf<double> red1, xi, dz, gdz;
   DevBuf<unsigned short> bsplit;  // pass-2 B operands, FP16 hi/lo in MFMA fragment order

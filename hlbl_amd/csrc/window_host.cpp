@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <cmath>
 #include <memory>
 #include <vector>
@@ -34,6 +36,76 @@ bool same(const hlbl_coord& a, const hlbl_coord& b) {
 constexpr int kJudgeSamples = 64;
 
 }  // namespace
+
+void pass1_group_units(const WinPair* pairs, int n_pairs, int* units, int unit_slots) {
+  const int nu = 2 * n_pairs, G = kPass1Groups;
+  // the two (block, flavour) halves a unit reads: S[f] of x and Y[1-f] of y
+  std::map<std::pair<const double*, int>, int> ids;
+  auto hid = [&](const double* b, int f) {
+    return ids.emplace(std::make_pair(b, f), (int)ids.size()).first->second;
+  };
+  std::vector<std::array<int, 2>> uh(nu);
+  for (int i = 0; i < n_pairs; ++i)
+    for (int f = 0; f < 2; ++f) uh[2 * i + f] = {hid(pairs[i].S, f), hid(pairs[i].Y, 1 - f)};
+  const int nh = (int)ids.size();
+  std::vector<int> cap(G), cnt((size_t)G * nh, 0);
+  for (int k = 0; k < G; ++k) cap[k] = (nu % G == 0 || k < nu % G) ? unit_slots : unit_slots - 1;
+  std::vector<std::vector<int>> grp(G);
+  auto put = [&](int k, int u, int s) {
+    cnt[(size_t)k * nh + uh[u][0]] += s;
+    cnt[(size_t)k * nh + uh[u][1]] += s;
+  };
+  // halves of u that group k lacks
+  auto lacks = [&](int k, int u) {
+    const int h0 = uh[u][0], h1 = uh[u][1];
+    return (cnt[(size_t)k * nh + h0] == 0) + (h1 != h0 && cnt[(size_t)k * nh + h1] == 0);
+  };
+  for (int u = 0; u < nu; ++u) {
+    int best = -1, bc = 0;
+    for (int k = 0; k < G; ++k) {
+      if ((int)grp[k].size() >= cap[k]) continue;
+      const int c = lacks(k, u);
+      if (best < 0 || c < bc || (c == bc && grp[k].size() < grp[best].size())) best = k, bc = c;
+    }
+    grp[best].push_back(u);
+    put(best, u, 1);
+  }
+  // swaps between lists while one lowers the number of distinct halves
+  for (int sweep = 0; sweep < 20; ++sweep) {
+    bool improved = false;
+    for (int a = 0; a < G; ++a)
+      for (int b = a + 1; b < G; ++b)
+        for (size_t ia = 0; ia < grp[a].size(); ++ia)
+          for (size_t ib = 0; ib < grp[b].size(); ++ib) {
+            const int u = grp[a][ia], v = grp[b][ib];
+            put(a, u, -1);
+            put(b, v, -1);
+            const int before = lacks(a, u) + lacks(b, v), after = lacks(a, v) + lacks(b, u);
+            if (after < before) {
+              put(a, v, 1);
+              put(b, u, 1);
+              grp[a][ia] = v;
+              grp[b][ib] = u;
+              improved = true;
+            } else {
+              put(a, u, 1);
+              put(b, v, 1);
+            }
+          }
+    if (!improved) break;
+  }
+  std::vector<int> seen(nu, 0);
+  for (int k = 0; k < G; ++k) {
+    if ((int)grp[k].size() > unit_slots) throw Error("pass1_group_units: a list overflows its slots");
+    for (int j = 0; j < unit_slots; ++j) {
+      const int u = j < (int)grp[k].size() ? grp[k][j] : -1;
+      units[k * unit_slots + j] = u;
+      if (u >= 0) ++seen[u];
+    }
+  }
+  for (int u = 0; u < nu; ++u)
+    if (seen[u] != 1) throw Error("pass1_group_units: a unit is not listed exactly once");
+}
 
 struct WindowCache {
   std::vector<std::unique_ptr<WindowPlan>> plans;
@@ -100,6 +172,9 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
     const size_t V = c->volume;
     p->d_pairs.ensure(n);
     p->h_pairs.ensure(n);
+    p->unit_slots = (2 * n + kPass1Groups - 1) / kPass1Groups;
+    p->d_units.ensure((size_t)kPass1Groups * p->unit_slots);
+    p->h_units.ensure((size_t)kPass1Groups * p->unit_slots);
     p->d_yvs.ensure(nu);
     p->h_yvs.ensure(nu);
     p->part1.ensure(pass1_part_len(c->global, n));
@@ -135,6 +210,7 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
     w->h_pairs.p[i] = WinPair{pairs[i].d_x, pairs[i].d_y, gs, pairs[i].yv, slot[i], 0};
   }
   for (int u = 0; u < nu; ++u) w->h_yvs.p[u] = yvs[u];
+  pass1_group_units(w->h_pairs.p, n, w->h_units.p, w->unit_slots);
   if (!cache->stream) check_hip(hipStreamCreateWithFlags(&cache->stream, hipStreamNonBlocking), "hipStreamCreate");
   return w;
 }
@@ -158,6 +234,8 @@ void window_tail_copies(WindowPlan* w, hipStream_t st) {
 void window_head_copies(WindowPlan* w, hipStream_t st) {
   check_hip(hipMemcpyAsync(w->d_pairs.p, w->h_pairs.p, w->n_pairs * sizeof(WinPair), hipMemcpyHostToDevice, st), "copy pairs");
   check_hip(hipMemcpyAsync(w->d_yvs.p, w->h_yvs.p, w->n_unique * sizeof(hlbl_coord), hipMemcpyHostToDevice, st), "copy yv");
+  check_hip(hipMemcpyAsync(w->d_units.p, w->h_units.p, (size_t)kPass1Groups * w->unit_slots * sizeof(int),
+                           hipMemcpyHostToDevice, st), "copy units");
 }
 }  // namespace
 

@@ -170,6 +170,25 @@ def test_window_forms(qed):
     con.close()
 
 
+@pytest.mark.parametrize("dims", [(6, 6, 6, 6), (6, 6, 6, 12)])
+def test_window_form1_short_lines(qed, dims):
+    """Form 1 where a (t, x, y) line is not a whole number of the plane sums' 8-site
+    chunks (z extent 6: one short chunk; 12: a full and a short one) against FP64."""
+    lat = H.Lattice(dims)
+    con = H.Contractor(lat, qed)
+    blocks = [dev_block(lat, s) for s in (41, 42, 43)]
+    pairs = [(blocks[0].data_ptr(), blocks[1].data_ptr(), (0, 0, 0, 0), (1, 1, 1, 1)),
+             (blocks[2].data_ptr(), blocks[1].data_ptr(), (1, 2, 3, 5), (-1, 2, -1, 1)),
+             (blocks[2].data_ptr(), blocks[0].data_ptr(), (1, 2, 3, 5), (2, 0, 1, -2))]
+    xunit = 0.0535446292
+    ref = np.array([con.kernel_sum(a, b, g, y, xunit) for a, b, g, y in pairs])
+    f1 = np.array(con.window_kernel_sum(pairs, xunit, form=1))
+    err = np.abs(f1 - ref) / np.abs(ref)
+    log(dims, "form 1 rel err", err)
+    assert np.all(err < 1e-4)
+    con.close()
+
+
 def test_run_chain_tiers_and_direct(qed):
     """The schedule loop (src/driver.cpp:23-165): rows in order, device tier = host tier,
     pairs recomputed directly, cache peak = simulate's, window batches = pair by pair."""
