@@ -164,7 +164,7 @@ __global__ __launch_bounds__(kPlThreads, 2) void pass1_planes(
 }
 
 // ---------------------------------------------------------------------------
-// The FP32 form on the matrix cores. Every spin pair p = (s', s) has s' and s in
+// The window form on the matrix cores. Every spin pair p = (s', s) has s' and s in
 // opposite halves ({0, 1} and {2, 3}), so for spin half g (g = 0: s in {0, 1},
 // s' in {2, 3}; g = 1: s in {2, 3}, s' in {0, 1}) the four pairs of the half are
 // one real 48 x 48 product, summed over K = (site, colour, re/im):
@@ -175,99 +175,81 @@ __global__ __launch_bounds__(kPlThreads, 2) void pass1_planes(
 //   sum_y xz_2(y) L_y = sum_y (xz_2(y) - xz_2(y+1)) R_y,   xz_2(LY) = 0,
 // one multiply-add per accumulator at the end of each line.
 //
-// One workgroup of 12 waves per (pair, flavour, plane): wave = (g, K slice kq).
-// The plane is walked in chunks of 8 sites of a line; a chunk's FP64 rows (12 of
-// S, 12 of Y, 1,536 bytes each) are staged with direct global-to-LDS loads into
-// one of three buffers, two chunks ahead. A chunk is 48 K values = 12 MFMA steps
-// of 4 (lane group kb = l / 16 takes k = 12 kb + step); slice kq takes steps
-// 2 kq and 2 kq + 1, which are the (re, im) of one complex element: colour
-// kq % 3 of site 2 kb + kq / 3. v_mfma_f32_16x16x4f32 operands: A[i][k] and
-// B[k][j] in lane i + 16 k (resp. j + 16 k); result element r of lane l is
+// Precision: S and Y are rounded to FP32, scaled by a power of two per chunk and
+// operand so that the chunk's largest magnitude stays below 2^14, split into FP16
+// hi + lo and multiplied as hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_f16 (FP32
+// accumulation; about 21 significant bits per product); each chunk's sums are
+// unscaled exactly into FP32 running sums.
+//
+// One workgroup of 8 waves per (pair, flavour, plane), walking the plane in chunks
+// of 8 sites of a line. K of a chunk: site 4 ks + kb (MFMA step ks = 0, 1; lane group
+// kb = l / 16) supplies the lane's 8 FP16 values, (Re, Im) of colours 0..2 and two
+// zeros. MFMA waves 0..3 = (g, ks) hold all nine 16 x 16 tiles of their half and
+// step. A chunk passes three stages, one per iteration: loader waves 8..11 load it
+// into registers (two chunks ahead) and stage it in LDS as FP32 with its largest |S|
+// and |Y| (LDS atomics); splitter waves 4..7 write the scaled hi/lo operands in MFMA
+// fragment order; the MFMA waves multiply. v_mfma_f32_16x16x32_f16: A[i][k] is element k % 8 of lane
+// i + 16 (k / 8), B[k][j] of lane j + 16 (k / 8); result r of lane l is
 // (4 (l / 16) + r, l % 16).
 namespace {
-constexpr int kM1Waves = 12;
-constexpr int kM1Threads = 64 * kM1Waves;
-constexpr int kM1Sites = 8;                     // sites per chunk
-constexpr int kM1Row = kM1Sites * 24 * 8 + 96;  // bytes per row a of a chunk (padded: the
-                                                // rows of a 16-lane read fall in distinct banks)
-constexpr int kM1Half = 12 * kM1Row;            // the 12 rows of S (or of Y)
-constexpr int kM1Buf = 2 * kM1Half;
-constexpr int kM1NBuf = 3;                      // chunks staged: one read, two in flight
-constexpr int kM1Vals = 72;                     // accumulator values per lane (9 tiles x 4, x 2)
-static_assert(6 * kM1Vals * 64 * 4 <= kM1NBuf * kM1Buf, "the end reduction fits the buffers");
+constexpr int kH1Mfma = 4;
+constexpr int kH1Threads = 64 * (kH1Mfma + 8);  // + 4 splitters + 4 loaders
+constexpr int kH1Sites = 8;                       // sites per chunk
+constexpr int kH1Units = 2 * 12 * kH1Sites * 12;  // 16-byte units of a chunk (S and Y rows)
+constexpr int kH1Fetch = kH1Units / 256;          // per producer lane
+static_assert(kH1Fetch * 256 == kH1Units, "the producers cover a chunk");
+constexpr int kH1Row = kH1Sites * 24 + 8;         // FP32 staging floats per row a (padded)
+constexpr int kH1F = 2 * 12 * kH1Row;             // [isy][a][site][24]
+constexpr int kH1Frag = 64 * 8;                   // FP16 of one fragment (a ds_read_b128 per lane)
+constexpr int kH1H = 2 * 2 * 2 * 3 * 2 * kH1Frag;  // [A, B][g][ks][tile][hi, lo][lane][8]
+constexpr int kH1Items = 2 * 2 * 2 * 4 * 12 * 2;  // (A, B) x g x ks x kb x row a (b) x si (sj)
+static_assert(kH1Items == 3 * 256, "three split items per producer lane");
+constexpr int kH1Vals = 72;                       // running sums per MFMA lane (9 tiles x 4, x 2)
+constexpr int kH1Max = 14;                        // scaled magnitudes stay below 2^kH1Max
+static_assert(2 * kH1Vals * 64 * 4 <= 2 * kH1H * 2, "the end reduction fits the fragments");
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef double d2v __attribute__((ext_vector_type(2)));
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+typedef __attribute__((address_space(1))) const d2v gd2v;
 
-// waits for this wave's loads of the chunk about to be read, then the workgroup
-// barrier; keep: the next chunk is a full one (four loads), which may stay in flight
-__device__ inline void m1_barrier(bool keep) {
-  static_assert(kM1NBuf == 3, "vmcnt(4): one later chunk in flight");
-  if (keep) asm volatile("s_waitcnt vmcnt(4) lgkmcnt(0)\n s_barrier" ::: "memory");
-  else asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n s_barrier" ::: "memory");
+// LDS barrier that leaves global loads (the producers' register prefetch) in flight
+__device__ inline void h1_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n s_barrier" ::: "memory"); }
+
+__device__ inline int h1_frag(int op, int g, int ks, int tile, int hl) {
+  return ((((op * 2 + g) * 2 + ks) * 3 + tile) * 2 + hl) * kH1Frag;
 }
 
-// Rows 2 wave and 2 wave + 1 (row R: R < 12 S[f][R], else Y[1-f][R-12]) of the
-// chunk of nz sites at site0 into buf: 12 nz 16-byte units per row, lane l unit l
-// and l + 64 (four loads per wave for a full chunk). The units of a short chunk's
-// missing sites are zeroed (LDS stores), so every chunk is read as a full one.
-__device__ inline void m1_issue(const double* S, const double* Y, size_t V, size_t site0, int nz,
-                                int wave, int lane, char* buf) {
-  const int units = 12 * nz;
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int R = 2 * wave + q, isy = R >= 12, a = R - 12 * isy;
-    const char* src = reinterpret_cast<const char*>((isy ? Y : S) + ((size_t)a * V + site0) * 24);
-    char* dst = buf + isy * kM1Half + a * kM1Row;
-    if (lane < units)
-      __builtin_amdgcn_global_load_lds((const void*)(src + 16 * lane),
-                                       (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
-    if (lane + 64 < units)
-      __builtin_amdgcn_global_load_lds((const void*)(src + 16 * (lane + 64)),
-                                       (__attribute__((address_space(3))) void*)(dst + 1024), 16,
-                                       0, 0);
-    if (units < 12 * kM1Sites) {
-      for (int u = units + lane; u < 12 * kM1Sites; u += 64)
-        *reinterpret_cast<d2v*>(dst + 16 * u) = d2v{0.0, 0.0};
-    }
-  }
+// exponent e with max < 2^e, from the bits of a non-negative float maximum
+__device__ inline int h1_exp(unsigned m) { return (int)((m >> 23) & 0xffu) - 126; }
+__device__ inline int h1_clamp(int t) { return t < -120 ? -120 : (t > 120 ? 120 : t); }
+
+// maximum over each group of 8 lanes (all lanes active), by DPP
+__device__ inline unsigned h1_max8(unsigned v) {
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));
+  return v;
 }
 
-// one chunk of one wave: two MFMA steps per tile
-__device__ inline void m1_chunk(const char* buf, const int (&offA)[3], const int (&offB)[3],
-                                float w3, int r16, f4v (&acc)[3][3]) {
-  float a0[3], a1[3], b0[3], b1[3];
-#pragma unroll
-  for (int rt = 0; rt < 3; ++rt) {
-    const d2v x = *reinterpret_cast<const d2v*>(buf + offA[rt]);
-    const float re = (float)x.x, im = (float)x.y;
-    const bool imrow = rt * 16 + r16 >= 24;
-    a0[rt] = imrow ? im : re;
-    a1[rt] = imrow ? -re : im;
-  }
-#pragma unroll
-  for (int ct = 0; ct < 3; ++ct) {
-    const d2v v = *reinterpret_cast<const d2v*>(buf + offB[ct]);
-    const float re = (float)v.x, im = (float)v.y;
-    const float m = ct * 16 + r16 >= 24 ? w3 : 1.f;
-    b0[ct] = re * m;
-    b1[ct] = im * m;
-  }
-#pragma unroll
-  for (int rt = 0; rt < 3; ++rt)
-#pragma unroll
-    for (int ct = 0; ct < 3; ++ct) {
-      acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt], b0[ct], acc[rt][ct], 0, 0, 0);
-      acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[rt], b1[ct], acc[rt][ct], 0, 0, 0);
-    }
+// (x0..x5) * sc as FP16 hi and lo, elements 6 and 7 zero
+__device__ inline void h1_split6(const float (&x)[6], float sc, uint4& hi, uint4& lo) {
+  f16_split2(x[0] * sc, x[1] * sc, hi.x, lo.x);
+  f16_split2(x[2] * sc, x[3] * sc, hi.y, lo.y);
+  f16_split2(x[4] * sc, x[5] * sc, hi.z, lo.z);
+  hi.w = lo.w = 0u;
 }
+// the im row of packed (Re, Im) pairs: (Im, -Re)
+__device__ inline unsigned h1_imrow(unsigned d) { return __builtin_amdgcn_alignbit(d, d, 16) ^ 0x80000000u; }
 
 }  // namespace
 
-__global__ __launch_bounds__(kM1Threads, 1) void pass1_mfma(
+__global__ __launch_bounds__(kH1Threads, 1) void pass1_mfma(
     const WinPair* __restrict__ pairs, WinPair single, hlbl_geom global, hlbl_geom local,
     hlbl_coord proc, int flav_mask, const int* __restrict__ units, int unit_slots,
     float* __restrict__ part) {
-  __shared__ __attribute__((aligned(16))) char lds[kM1NBuf * kM1Buf];
+  __shared__ __attribute__((aligned(16))) float sF[2][kH1F];
+  __shared__ __attribute__((aligned(16))) _Float16 sH[2][kH1H];
+  __shared__ unsigned sMax[4][2];  // per chunk c % 4: largest |S|, |Y| (float bits)
   int unit = blockIdx.x;
   if (units) {
     unit = units[(blockIdx.x % kPass1Groups) * unit_slots + blockIdx.x / kPass1Groups];
@@ -281,118 +263,248 @@ __global__ __launch_bounds__(kM1Threads, 1) void pass1_mfma(
   const size_t V = geom_volume(local);
   const size_t pbase = (size_t)plane * LY * LZ;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int g = wave >= 6, kq = wave - 6 * g;
-  const int kb = lane >> 4, r16 = lane & 15;
-  const double* Sf = wp.S + (size_t)f * 12 * V * 24;
-  const double* Yg = wp.Y + (size_t)(1 - f) * 12 * V * 24;
   const int oy = proc.y * local.LY - wp.gsx.y, oz = proc.z * local.LZ - wp.gsx.z;
+  const int cpl = (LZ + kH1Sites - 1) / kH1Sites, nch = LY * cpl;
+  const int nit = (nch + 3) & ~1;  // iterations -2 .. nit - 3 (an even count, one barrier each)
+  auto chunk_z0 = [&](int c) { return (c % cpl) * kH1Sites; };
+  auto chunk_site0 = [&](int c) { return pbase + (size_t)(c / cpl) * LZ + chunk_z0(c); };
+  auto chunk_nz = [&](int c) { return LZ - chunk_z0(c) < kH1Sites ? LZ - chunk_z0(c) : kH1Sites; };
+  if (threadIdx.x < 8) (&sMax[0][0])[threadIdx.x] = 0u;
 
-  // this lane's element of the chunk: site 2 kb + kq / 3, colour kq % 3
-  const int site = 2 * kb + (kq >= 3), cc = kq % 3;
-  int offA[3], offB[3];
+  // xz_3 headroom: |xz_3| <= LZ / 2 < 2^hz
+  int hz = 0;
+  while ((1 << hz) <= global.LZ / 2) ++hz;
+
+  if (wave >= kH1Mfma + 4) {
+    // ---- loaders (waves 8..11): iteration it stages chunk it + 2 (FP32 and its
+    // maxima), clears the maxima slot of chunk it + 3 and loads chunk it + 4 into the
+    // registers chunk it + 2 came from. Always kH1Fetch loads per lane (a chunk past
+    // the end loads the last one again, a site past a short chunk's end its first
+    // site; staging zeroes those), so the later chunk's loads stay in flight.
+    const int ql = threadIdx.x - 64 * (kH1Mfma + 4);
+    const size_t pS = (size_t)(wp.S + (size_t)f * 12 * V * 24);
+    const size_t pY = (size_t)(wp.Y + (size_t)(1 - f) * 12 * V * 24);
+    // unit k of this lane: u = ql + 256 k = ((isy * 12 + a) * 8 + site) * 12 + j
+    size_t uoff[kH1Fetch];
+    int usite[kH1Fetch], uF[kH1Fetch];
+    unsigned uisy = 0u;
 #pragma unroll
-  for (int t = 0; t < 3; ++t) {
-    const int r = t * 16 + r16, r24 = r >= 24 ? r - 24 : r;
-    offA[t] = (r24 >> 1) * kM1Row + site * 192 + ((g ? 0 : 2) + (r24 & 1)) * 48 + cc * 16;
-    offB[t] = kM1Half + (r24 >> 1) * kM1Row + site * 192 + ((g ? 2 : 0) + (r24 & 1)) * 48 + cc * 16;
+    for (int k = 0; k < kH1Fetch; ++k) {
+      const int u = ql + 256 * k, isy = u >= kH1Units / 2, r = u - isy * (kH1Units / 2);
+      const int a = r / (kH1Sites * 12), site = (r / 12) % kH1Sites, j = r % 12;
+      uoff[k] = (isy ? pY : pS) + ((size_t)a * V * 24 + 2 * j) * sizeof(double);
+      usite[k] = site;
+      uF[k] = (isy * 12 + a) * kH1Row + site * 24 + 2 * j;
+      uisy |= (unsigned)isy << k;
+    }
+    auto issue = [&](int c, d2v (&r)[kH1Fetch]) {
+      c = c < nch ? c : nch - 1;
+      const size_t s0 = chunk_site0(c);
+      const int nz = chunk_nz(c);
+#pragma unroll
+      for (int k = 0; k < kH1Fetch; ++k) {
+        const size_t z = s0 + (usite[k] < nz ? usite[k] : 0);
+        r[k] = *reinterpret_cast<gd2v*>(uoff[k] + z * 24 * sizeof(double));
+      }
+    };
+    auto stage = [&](int c, const d2v (&r)[kH1Fetch]) {
+      float* F = sF[c & 1];
+      const int nz = chunk_nz(c);
+      unsigned mS = 0u, mY = 0u;
+#pragma unroll
+      for (int k = 0; k < kH1Fetch; ++k) {
+        float2 v;
+        v.x = usite[k] < nz ? (float)r[k].x : 0.f;
+        v.y = usite[k] < nz ? (float)r[k].y : 0.f;
+        const unsigned m = max(__float_as_uint(fabsf(v.x)), __float_as_uint(fabsf(v.y)));
+        if ((uisy >> k) & 1u) mY = max(mY, m);
+        else mS = max(mS, m);
+        *reinterpret_cast<float2*>(F + uF[k]) = v;
+      }
+      mS = h1_max8(mS);
+      mY = h1_max8(mY);
+      if ((lane & 7) == 0) {
+        atomicMax(&sMax[c & 3][0], mS);
+        atomicMax(&sMax[c & 3][1], mY);
+      }
+    };
+    auto step = [&](int it, d2v (&r)[kH1Fetch]) {
+      if (it + 2 < nch) stage(it + 2, r);
+      if (ql < 2) sMax[(it + 7) & 3][ql] = 0u;
+      issue(it + 4, r);
+      h1_barrier();
+    };
+    d2v r0[kH1Fetch], r1[kH1Fetch];
+    issue(0, r0);
+    issue(1, r1);
+    h1_barrier();
+    for (int it = -2; it < nit - 2; it += 2) {
+      step(it, r0);
+      step(it + 1, r1);
+    }
+    return;
   }
 
-  f4v acc[3][3], py[3][3];
+  if (wave >= kH1Mfma) {
+    // ---- splitters (waves 4..7): iteration it writes the scaled hi/lo fragments of
+    // chunk it + 1. Item i = pl + 256 k: op = A (S rows) or B (Y columns), then
+    // ((g * 2 + ks) * 4 + kb) * 12 + a, * 2 + si; an A item writes its re and im rows,
+    // a B item its w = 0 and w = 1 columns.
+    const int pl = threadIdx.x - 64 * kH1Mfma;
+    int iF[3], iH0[3], iH1[3], isite[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int i = pl + 256 * k, op = i >= kH1Items / 2, jj = i - op * (kH1Items / 2);
+      const int si = jj & 1, a = (jj >> 1) % 12, kb = (jj / 24) & 3, gks = jj / 96;
+      const int g = gks >> 1, ks = gks & 1, site = 4 * ks + kb;
+      const int spin = op ? (g ? 2 : 0) + si : (g ? 0 : 2) + si;
+      iF[k] = (op * 12 + a) * kH1Row + site * 24 + spin * 6;
+      const int r0 = 2 * a + si, r1 = r0 + 24;
+      iH0[k] = h1_frag(op, g, ks, r0 >> 4, 0) + 8 * ((r0 & 15) + 16 * kb);
+      iH1[k] = h1_frag(op, g, ks, r1 >> 4, 0) + 8 * ((r1 & 15) + 16 * kb);
+      isite[k] = site;
+    }
+    auto split = [&](int c) {
+      const float* F = sF[c & 1];
+      _Float16* H = sH[c & 1];
+      const int tS = h1_clamp(kH1Max - h1_exp(sMax[c & 3][0]));
+      const int tY = h1_clamp(kH1Max - h1_exp(sMax[c & 3][1]) - hz);
+      const float scS = ldexpf(1.f, tS), scY = ldexpf(1.f, tY);
+      const int zb = mod_pos(chunk_z0(c) + oz, global.LZ);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const bool opB = pl + 256 * k >= kH1Items / 2;  // uniform per wave and k
+        float x[6];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          const float2 v = *reinterpret_cast<const float2*>(F + iF[k] + 2 * e);
+          x[2 * e] = v.x;
+          x[2 * e + 1] = v.y;
+        }
+        uint4 hi, lo;
+        h1_split6(x, opB ? scY : scS, hi, lo);
+        *reinterpret_cast<uint4*>(H + iH0[k]) = hi;
+        *reinterpret_cast<uint4*>(H + iH0[k] + kH1Frag) = lo;
+        if (!opB) {
+          hi = uint4{h1_imrow(hi.x), h1_imrow(hi.y), h1_imrow(hi.z), 0u};
+          lo = uint4{h1_imrow(lo.x), h1_imrow(lo.y), h1_imrow(lo.z), 0u};
+        } else {
+          int zs = zb + isite[k];
+          zs = zs >= global.LZ ? zs - global.LZ : zs;
+          zs = zs >= global.LZ ? zs % global.LZ : zs;
+          h1_split6(x, scY * (float)coord_map_zerohalf(zs, global.LZ), hi, lo);
+        }
+        *reinterpret_cast<uint4*>(H + iH1[k]) = hi;
+        *reinterpret_cast<uint4*>(H + iH1[k] + kH1Frag) = lo;
+      }
+    };
+    h1_barrier();
+    for (int it = -2; it < nit - 2; ++it) {
+      if (it + 1 >= 0 && it + 1 < nch) split(it + 1);
+      h1_barrier();
+    }
+    return;
+  }
+
+  // ---- MFMA waves: (g, ks)
+  const int g = wave >> 1, ks = wave & 1;
+  f4v R[3][3], py[3][3];
 #pragma unroll
   for (int rt = 0; rt < 3; ++rt)
 #pragma unroll
-    for (int ct = 0; ct < 3; ++ct) acc[rt][ct] = py[rt][ct] = f4v{0.f, 0.f, 0.f, 0.f};
-
-  const int cpl = (LZ + kM1Sites - 1) / kM1Sites, nch = LY * cpl;
-  auto chunk_nz = [&](int c) {
-    const int z0 = (c % cpl) * kM1Sites;
-    return LZ - z0 < kM1Sites ? LZ - z0 : kM1Sites;
-  };
-  auto issue = [&](int c) {
-    const int y = c / cpl, z0 = (c - y * cpl) * kM1Sites;
-    m1_issue(Sf, Yg, V, pbase + (size_t)y * LZ + z0, chunk_nz(c), wave, lane,
-             lds + (c % kM1NBuf) * kM1Buf);
-  };
+    for (int ct = 0; ct < 3; ++ct) R[rt][ct] = py[rt][ct] = f4v{0.f, 0.f, 0.f, 0.f};
   auto wy = [&](int y) {
     return y < LY ? (float)coord_map_zerohalf(mod_pos(y + oy, global.LY), global.LY) : 0.f;
   };
-  for (int c = 0; c < kM1NBuf - 1 && c < nch; ++c) issue(c);
-  int y = 0, zp = 0;  // line and part of chunk c
-  for (int c = 0; c < nch; ++c) {
-    m1_barrier(c + 1 < nch && chunk_nz(c + 1) == kM1Sites);
-    if (c + kM1NBuf - 1 < nch) issue(c + kM1NBuf - 1);
-    // xz_3 of this lane's site z0 + site: the chunk's base on the scalar unit
-    const int zb = mod_pos(zp * kM1Sites + oz, global.LZ);
-    int zs = zb + site;
-    zs = global.LZ >= kM1Sites ? (zs >= global.LZ ? zs - global.LZ : zs) : zs % global.LZ;
-    const float w3 = (float)coord_map_zerohalf(zs, global.LZ);
-    m1_chunk(lds + (c % kM1NBuf) * kM1Buf, offA, offB, w3, r16, acc);
-    if (++zp == cpl) {  // end of line y: the xz_2 sums by partial summation
-      const float d = wy(y) - wy(y + 1);
+  h1_barrier();
+  for (int it = -2; it < nit - 2; ++it) {
+    if (it >= 0 && it < nch) {
+      const _Float16* H = sH[it & 1];
+      h8v A[3][2], B[3][2];
+#pragma unroll
+      for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int hl = 0; hl < 2; ++hl) {
+          A[t][hl] = *reinterpret_cast<const h8v*>(H + h1_frag(0, g, ks, t, hl) + 8 * lane);
+          B[t][hl] = *reinterpret_cast<const h8v*>(H + h1_frag(1, g, ks, t, hl) + 8 * lane);
+        }
+      const int tS = h1_clamp(kH1Max - h1_exp(sMax[it & 3][0]));
+      const int tY = h1_clamp(kH1Max - h1_exp(sMax[it & 3][1]) - hz);
+      const int ue = -(tS + tY);
+      f4v c[3][3];
 #pragma unroll
       for (int rt = 0; rt < 3; ++rt)
 #pragma unroll
-        for (int ct = 0; ct < 3; ++ct)
+        for (int ct = 0; ct < 3; ++ct) {
+          c[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[rt][0], B[ct][0], f4v{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+          c[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[rt][0], B[ct][1], c[rt][ct], 0, 0, 0);
+          c[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[rt][1], B[ct][0], c[rt][ct], 0, 0, 0);
+        }
+      if (ue >= -126) {  // 2^ue a normal float: one multiply-add per value
+        const float u = ldexpf(1.f, ue);
 #pragma unroll
-          for (int i = 0; i < 4; ++i) py[rt][ct][i] = fmaf(d, acc[rt][ct][i], py[rt][ct][i]);
-      zp = 0;
-      ++y;
+        for (int rt = 0; rt < 3; ++rt)
+#pragma unroll
+          for (int ct = 0; ct < 3; ++ct)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) R[rt][ct][i] = fmaf(c[rt][ct][i], u, R[rt][ct][i]);
+      } else {
+#pragma unroll
+        for (int rt = 0; rt < 3; ++rt)
+#pragma unroll
+          for (int ct = 0; ct < 3; ++ct)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) R[rt][ct][i] += ldexpf(c[rt][ct][i], ue);
+      }
+      if ((it + 1) % cpl == 0) {  // end of line y: the xz_2 sums by partial summation
+        const int y = it / cpl;
+        const float d = wy(y) - wy(y + 1);
+#pragma unroll
+        for (int rt = 0; rt < 3; ++rt)
+#pragma unroll
+          for (int ct = 0; ct < 3; ++ct)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) py[rt][ct][i] = fmaf(d, R[rt][ct][i], py[rt][ct][i]);
+      }
     }
+    h1_barrier();
   }
 
-  // sum of the six K slices in a fixed order: (3 + 0, 4 + 1, 5 + 2), then 0 + 1 + 2
-  float* red = reinterpret_cast<float*>(lds);
-  auto put = [&](int slot) {
-    float* d = red + (size_t)slot * kM1Vals * 64 + lane;
+  // the two K steps' sums (the producers have left): step 1 through LDS into step 0
+  float* red = reinterpret_cast<float*>(&sH[0][0]) + (size_t)g * kH1Vals * 64 + lane;
+  if (ks) {
 #pragma unroll
     for (int rt = 0; rt < 3; ++rt)
 #pragma unroll
       for (int ct = 0; ct < 3; ++ct)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          d[((rt * 3 + ct) * 4 + i) * 64] = acc[rt][ct][i];
-          d[(36 + (rt * 3 + ct) * 4 + i) * 64] = py[rt][ct][i];
+          red[((rt * 3 + ct) * 4 + i) * 64] = R[rt][ct][i];
+          red[(36 + (rt * 3 + ct) * 4 + i) * 64] = py[rt][ct][i];
         }
-  };
-  auto take = [&](int slot) {
-    const float* d = red + (size_t)slot * kM1Vals * 64 + lane;
-#pragma unroll
-    for (int rt = 0; rt < 3; ++rt)
-#pragma unroll
-      for (int ct = 0; ct < 3; ++ct)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          acc[rt][ct][i] += d[((rt * 3 + ct) * 4 + i) * 64];
-          py[rt][ct][i] += d[(36 + (rt * 3 + ct) * 4 + i) * 64];
-        }
-  };
+  }
   __syncthreads();
-  if (kq >= 3) put(g * 3 + kq - 3);
-  __syncthreads();
-  if (kq < 3) take(g * 3 + kq);
-  __syncthreads();
-  if (kq == 1 || kq == 2) put(g * 2 + kq - 1);
-  __syncthreads();
-  if (kq != 0) return;
-  take(g * 2);
-  take(g * 2 + 1);
+  if (ks) return;
   float* out = part + (((size_t)ip * 2 + f) * gridDim.y + plane) * kPlPart;
+  const int kb = lane >> 4, r16 = lane & 15;
 #pragma unroll
   for (int rt = 0; rt < 3; ++rt)
 #pragma unroll
     for (int ct = 0; ct < 3; ++ct)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
+        const float vR = R[rt][ct][i] + red[((rt * 3 + ct) * 4 + i) * 64];
+        const float vP = py[rt][ct][i] + red[(36 + (rt * 3 + ct) * 4 + i) * 64];
         const int r = rt * 16 + 4 * kb + i, im = r >= 24, r24 = r - 24 * im;
         const int cl = ct * 16 + r16, w = cl >= 24, c24 = cl - 24 * w;
         const int si = r24 & 1, sj = c24 & 1;
         const int p = (si != sj) * 4 + 2 * g + sj;
         const int e = (p * 144 + (r24 >> 1) * 12 + (c24 >> 1)) * 2 + im;
         if (w) {
-          out[2 * kXiReal + e] = acc[rt][ct][i];
+          out[2 * kXiReal + e] = vR;
         } else {
-          out[e] = acc[rt][ct][i];
-          out[kXiReal + e] = py[rt][ct][i];
+          out[e] = vR;
+          out[kXiReal + e] = vP;
         }
       }
 }
@@ -466,7 +578,7 @@ void launch_pass1_planes(const WinPair* d_pairs, const WinPair& single, int n_pa
   const int np = n_planes(local), nc = n_chunks(local);
   if constexpr (std::is_same<T, float>::value)
     hipLaunchKernelGGL(pass1_mfma, dim3(d_units ? kPass1Groups * unit_slots : 2 * n_pairs, np),
-                       dim3(kM1Threads), 0, st, d_pairs, single, global, local, proc, flav_mask,
+                       dim3(kH1Threads), 0, st, d_pairs, single, global, local, proc, flav_mask,
                        d_units, unit_slots, part);
   else
     hipLaunchKernelGGL(pass1_planes<T>, dim3(2 * n_pairs, np), dim3(kPlThreads), 0, st, d_pairs,

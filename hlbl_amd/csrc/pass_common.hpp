@@ -112,6 +112,18 @@ HLBL_HD double bmat_value(const double* gdz, int f, int kidx, int col) {
   return part ? -x[1] : x[0];
 }
 
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+// (x, y) = hi + lo in FP16, each packed as (x bits, y bits) (v_cvt_pk_f16_f32: two
+// values per conversion, nearest, ties to even)
+__device__ inline void f16_split2(float x, float y, unsigned& hi, unsigned& lo) {
+  const f2v v = {x, y};
+  const h2v h = __builtin_convertvector(v, h2v);
+  const h2v l = __builtin_convertvector(v - __builtin_convertvector(h, f2v), h2v);
+  __builtin_memcpy(&hi, &h, 4);
+  __builtin_memcpy(&lo, &l, 4);
+}
+
 // Kahan-compensated accumulation
 struct Kahan {
   double s = 0.0, c = 0.0;

@@ -284,17 +284,6 @@ __device__ inline float f16_float(unsigned short u) {
   __builtin_memcpy(&h, &u, 2);
   return (float)h;
 }
-typedef _Float16 h2v __attribute__((ext_vector_type(2)));
-typedef float f2v __attribute__((ext_vector_type(2)));
-// (x, y) = hi + lo in FP16, each packed as (x bits, y bits) (v_cvt_pk_f16_f32: two
-// values per conversion, nearest, ties to even)
-__device__ inline void f16_split2(float x, float y, unsigned& hi, unsigned& lo) {
-  const f2v v = {x, y};
-  const h2v h = __builtin_convertvector(v, h2v);
-  const h2v l = __builtin_convertvector(v - __builtin_convertvector(h, f2v), h2v);
-  __builtin_memcpy(&hi, &h, 4);
-  __builtin_memcpy(&lo, &l, 4);
-}
 
 // Column exponents of Bg: per (pair, flavour, col < 40), te = kS2AMax - E with
 // max_k |B_f[k][col]| = m 2^E, m in [0.5, 1) (0 for an all-zero column); the
