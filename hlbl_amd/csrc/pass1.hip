@@ -212,6 +212,13 @@ typedef double d2v __attribute__((ext_vector_type(2)));
 typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 typedef __attribute__((address_space(1))) const d2v gd2v;
 
+// a 64-bit value the wave holds in common, as a scalar (both halves zero-extended)
+__device__ inline size_t uniform_u64(size_t v) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v & 0xffffffffu));
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+  return ((size_t)hi << 32) | (size_t)lo;
+}
+
 // LDS barrier that leaves global loads (the producers' register prefetch) in flight
 __device__ inline void h1_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n s_barrier" ::: "memory"); }
 
@@ -221,7 +228,9 @@ __device__ inline int h1_frag(int op, int g, int ks, int tile, int hl) {
 
 // exponent e with max < 2^e, from the bits of a non-negative float maximum
 __device__ inline int h1_exp(unsigned m) { return (int)((m >> 23) & 0xffu) - 126; }
-__device__ inline int h1_clamp(int t) { return t < -120 ? -120 : (t > 120 ? 120 : t); }
+// scale exponents within [-63, 63]: the unscale 2^-(tS + tY) is a normal float (the
+// upscale of a chunk whose largest magnitude is below 2^-49 stops at 2^63)
+__device__ inline int h1_clamp(int t) { return t < -63 ? -63 : (t > 63 ? 63 : t); }
 
 // maximum over each group of 8 lanes (all lanes active), by DPP
 __device__ inline unsigned h1_max8(unsigned v) {
@@ -231,12 +240,20 @@ __device__ inline unsigned h1_max8(unsigned v) {
   return v;
 }
 
-// (x0..x5) * sc as FP16 hi and lo, elements 6 and 7 zero
-__device__ inline void h1_split6(const float (&x)[6], float sc, uint4& hi, uint4& lo) {
+// maximum over the wave (all lanes active), as a wave-uniform value
+__device__ inline unsigned h1_max64(unsigned v) {
+  v = h1_max8(v);
+  unsigned m = (unsigned)__builtin_amdgcn_readlane((int)v, 0);
+#pragma unroll
+  for (int gidx = 1; gidx < 8; ++gidx) m = max(m, (unsigned)__builtin_amdgcn_readlane((int)v, 8 * gidx));
+  return m;
+}
+
+// (x0..x5) * sc as FP16 hi and lo (elements 0..5 of a lane's eight; 6 and 7 stay zero)
+__device__ inline void h1_split6(const float (&x)[6], float sc, uint3& hi, uint3& lo) {
   f16_split2(x[0] * sc, x[1] * sc, hi.x, lo.x);
   f16_split2(x[2] * sc, x[3] * sc, hi.y, lo.y);
   f16_split2(x[4] * sc, x[5] * sc, hi.z, lo.z);
-  hi.w = lo.w = 0u;
 }
 // the im row of packed (Re, Im) pairs: (Im, -Re)
 __device__ inline unsigned h1_imrow(unsigned d) { return __builtin_amdgcn_alignbit(d, d, 16) ^ 0x80000000u; }
@@ -270,6 +287,9 @@ __global__ __launch_bounds__(kH1Threads, 1) void pass1_mfma(
   auto chunk_site0 = [&](int c) { return pbase + (size_t)(c / cpl) * LZ + chunk_z0(c); };
   auto chunk_nz = [&](int c) { return LZ - chunk_z0(c) < kH1Sites ? LZ - chunk_z0(c) : kH1Sites; };
   if (threadIdx.x < 8) (&sMax[0][0])[threadIdx.x] = 0u;
+  // elements 6 and 7 of every lane's operand are the zero padding of K: written once
+  for (int i = threadIdx.x; i < 2 * kH1H / 8; i += kH1Threads)
+    *reinterpret_cast<unsigned*>(&sH[0][0] + 8 * i + 6) = 0u;  // halves 6 and 7
 
   // xz_3 headroom: |xz_3| <= LZ / 2 < 2^hz
   int hz = 0;
@@ -284,10 +304,11 @@ __global__ __launch_bounds__(kH1Threads, 1) void pass1_mfma(
     const int ql = threadIdx.x - 64 * (kH1Mfma + 4);
     const size_t pS = (size_t)(wp.S + (size_t)f * 12 * V * 24);
     const size_t pY = (size_t)(wp.Y + (size_t)(1 - f) * 12 * V * 24);
-    // unit k of this lane: u = ql + 256 k = ((isy * 12 + a) * 8 + site) * 12 + j
+    // unit k of this lane: u = ql + 256 k = ((isy * 12 + a) * 8 + site) * 12 + j;
+    // uoff[k] the address of its row and 16 bytes at site 0 (a site adds 192 bytes)
     size_t uoff[kH1Fetch];
     int usite[kH1Fetch], uF[kH1Fetch];
-    unsigned uisy = 0u;
+    bool uisy = false;  // unit 4 is a row of Y
 #pragma unroll
     for (int k = 0; k < kH1Fetch; ++k) {
       const int u = ql + 256 * k, isy = u >= kH1Units / 2, r = u - isy * (kH1Units / 2);
@@ -295,48 +316,58 @@ __global__ __launch_bounds__(kH1Threads, 1) void pass1_mfma(
       uoff[k] = (isy ? pY : pS) + ((size_t)a * V * 24 + 2 * j) * sizeof(double);
       usite[k] = site;
       uF[k] = (isy * 12 + a) * kH1Row + site * 24 + 2 * j;
-      uisy |= (unsigned)isy << k;
+      if (k == 4) uisy = isy;
     }
-    auto issue = [&](int c, d2v (&r)[kH1Fetch]) {
+    // one code path for every chunk (a site past a short chunk's end reads the chunk's
+    // first site), so that the compiler's load counting lets the later chunk fly
+    auto issue_any = [&](int c, d2v (&r)[kH1Fetch]) {
       c = c < nch ? c : nch - 1;
-      const size_t s0 = chunk_site0(c);
+      const unsigned s0 = (unsigned)chunk_site0(c);  // the volume is below 2^32
       const int nz = chunk_nz(c);
 #pragma unroll
       for (int k = 0; k < kH1Fetch; ++k) {
-        const size_t z = s0 + (usite[k] < nz ? usite[k] : 0);
-        r[k] = *reinterpret_cast<gd2v*>(uoff[k] + z * 24 * sizeof(double));
+        const unsigned z = s0 + (unsigned)(usite[k] < nz ? usite[k] : 0);
+        r[k] = *reinterpret_cast<gd2v*>(uoff[k] + (size_t)z * (24 * sizeof(double)));
       }
     };
-    auto stage = [&](int c, const d2v (&r)[kH1Fetch]) {
+    // units k < 4 are rows of S, k > 4 rows of Y, k = 4 either (u = ql + 1024)
+    static_assert(4 * 256 <= kH1Units / 2 && 5 * 256 >= kH1Units / 2, "one mixed unit per lane");
+    auto stage = [&](auto kFull, int c, const d2v (&r)[kH1Fetch]) {
       float* F = sF[c & 1];
       const int nz = chunk_nz(c);
       unsigned mS = 0u, mY = 0u;
 #pragma unroll
       for (int k = 0; k < kH1Fetch; ++k) {
         float2 v;
-        v.x = usite[k] < nz ? (float)r[k].x : 0.f;
-        v.y = usite[k] < nz ? (float)r[k].y : 0.f;
+        v.x = (float)r[k].x;
+        v.y = (float)r[k].y;
+        if (!decltype(kFull)::value && usite[k] >= nz) v = make_float2(0.f, 0.f);
         const unsigned m = max(__float_as_uint(fabsf(v.x)), __float_as_uint(fabsf(v.y)));
-        if ((uisy >> k) & 1u) mY = max(mY, m);
+        if (k < 4) mS = max(mS, m);
+        else if (k > 4) mY = max(mY, m);
+        else if (uisy) mY = max(mY, m);
         else mS = max(mS, m);
         *reinterpret_cast<float2*>(F + uF[k]) = v;
       }
-      mS = h1_max8(mS);
-      mY = h1_max8(mY);
-      if ((lane & 7) == 0) {
+      mS = h1_max64(mS);
+      mY = h1_max64(mY);
+      if (lane == 0) {
         atomicMax(&sMax[c & 3][0], mS);
         atomicMax(&sMax[c & 3][1], mY);
       }
     };
     auto step = [&](int it, d2v (&r)[kH1Fetch]) {
-      if (it + 2 < nch) stage(it + 2, r);
+      if (it + 2 < nch) {
+        if (chunk_nz(it + 2) == kH1Sites) stage(std::true_type{}, it + 2, r);
+        else stage(std::false_type{}, it + 2, r);
+      }
       if (ql < 2) sMax[(it + 7) & 3][ql] = 0u;
-      issue(it + 4, r);
+      issue_any(it + 4, r);
       h1_barrier();
     };
     d2v r0[kH1Fetch], r1[kH1Fetch];
-    issue(0, r0);
-    issue(1, r1);
+    issue_any(0, r0);
+    issue_any(1, r1);
     h1_barrier();
     for (int it = -2; it < nit - 2; it += 2) {
       step(it, r0);
@@ -381,21 +412,21 @@ __global__ __launch_bounds__(kH1Threads, 1) void pass1_mfma(
           x[2 * e] = v.x;
           x[2 * e + 1] = v.y;
         }
-        uint4 hi, lo;
+        uint3 hi, lo;
         h1_split6(x, opB ? scY : scS, hi, lo);
-        *reinterpret_cast<uint4*>(H + iH0[k]) = hi;
-        *reinterpret_cast<uint4*>(H + iH0[k] + kH1Frag) = lo;
+        *reinterpret_cast<uint3*>(H + iH0[k]) = hi;
+        *reinterpret_cast<uint3*>(H + iH0[k] + kH1Frag) = lo;
         if (!opB) {
-          hi = uint4{h1_imrow(hi.x), h1_imrow(hi.y), h1_imrow(hi.z), 0u};
-          lo = uint4{h1_imrow(lo.x), h1_imrow(lo.y), h1_imrow(lo.z), 0u};
+          hi = uint3{h1_imrow(hi.x), h1_imrow(hi.y), h1_imrow(hi.z)};
+          lo = uint3{h1_imrow(lo.x), h1_imrow(lo.y), h1_imrow(lo.z)};
         } else {
           int zs = zb + isite[k];
           zs = zs >= global.LZ ? zs - global.LZ : zs;
           zs = zs >= global.LZ ? zs % global.LZ : zs;
           h1_split6(x, scY * (float)coord_map_zerohalf(zs, global.LZ), hi, lo);
         }
-        *reinterpret_cast<uint4*>(H + iH1[k]) = hi;
-        *reinterpret_cast<uint4*>(H + iH1[k] + kH1Frag) = lo;
+        *reinterpret_cast<uint3*>(H + iH1[k]) = hi;
+        *reinterpret_cast<uint3*>(H + iH1[k] + kH1Frag) = lo;
       }
     };
     h1_barrier();
@@ -430,7 +461,7 @@ __global__ __launch_bounds__(kH1Threads, 1) void pass1_mfma(
         }
       const int tS = h1_clamp(kH1Max - h1_exp(sMax[it & 3][0]));
       const int tY = h1_clamp(kH1Max - h1_exp(sMax[it & 3][1]) - hz);
-      const int ue = -(tS + tY);
+      const float u = ldexpf(1.f, -(tS + tY));
       f4v c[3][3];
 #pragma unroll
       for (int rt = 0; rt < 3; ++rt)
@@ -440,22 +471,12 @@ __global__ __launch_bounds__(kH1Threads, 1) void pass1_mfma(
           c[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[rt][0], B[ct][1], c[rt][ct], 0, 0, 0);
           c[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[rt][1], B[ct][0], c[rt][ct], 0, 0, 0);
         }
-      if (ue >= -126) {  // 2^ue a normal float: one multiply-add per value
-        const float u = ldexpf(1.f, ue);
 #pragma unroll
-        for (int rt = 0; rt < 3; ++rt)
+      for (int rt = 0; rt < 3; ++rt)
 #pragma unroll
-          for (int ct = 0; ct < 3; ++ct)
+        for (int ct = 0; ct < 3; ++ct)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) R[rt][ct][i] = fmaf(c[rt][ct][i], u, R[rt][ct][i]);
-      } else {
-#pragma unroll
-        for (int rt = 0; rt < 3; ++rt)
-#pragma unroll
-          for (int ct = 0; ct < 3; ++ct)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) R[rt][ct][i] += ldexpf(c[rt][ct][i], ue);
-      }
+          for (int i = 0; i < 4; ++i) R[rt][ct][i] = fmaf(c[rt][ct][i], u, R[rt][ct][i]);
       if ((it + 1) % cpl == 0) {  // end of line y: the xz_2 sums by partial summation
         const int y = it / cpl;
         const float d = wy(y) - wy(y + 1);
