@@ -81,12 +81,16 @@ __device__ inline R kp_channel(int e, int ch, const SiteGeom<R>& g, const TT* ta
   return qed::interp(table, cell, ch);
 }
 
-// Phase 2: component i of the T tensors of evaluation e.
+// Phase 2: the T tensors of evaluation e, components j, 64 + j and 128 + j of lane j
+// (they share their coordinates and invariants) into T.
 template <class R>
-__device__ inline R kp_tensor(int e, int i, const SiteGeom<R>& g, const R* ch_e) {
+__device__ inline void kp_tensors(int e, int j, const SiteGeom<R>& g, const R* ch_e, R* T) {
+  static_assert(qed::kNT == 3 * 64, "three components per lane");
   R u[4], v[4];
   eval_args(e, g, u, v);
-  return qed::t_component(i, u, v, ch_e);
+  const qed::TArgs<R> t = qed::t_args(j, u, v);
+#pragma unroll
+  for (int A = 0; A < 3; ++A) T[64 * A + j] = qed::t_value(A, t, ch_e);
 }
 
 // v[i] as a blend of the four values: the compiler cannot turn it into a load
@@ -144,18 +148,20 @@ constexpr int kKwLen = 6 * 32 + qed::kNT + 3 * qed::kNL + kCoefField + 48;
 constexpr int kKwLen3 = 3 * 32 + qed::kNT + 3 * qed::kNL + kCoefField + 48;
 
 // L0 sparse terms, staged in LDS by the workgroup, one layout per precision.
-// FP32: each output's terms padded to kMaxTerms with zero-coefficient terms
-// (coefficient 0, index 0): an output is a fixed kMaxTerms-term sum, terms 0..7 in one
-// 16-byte row and term 8 beside it, read with two loads (no dependent reads). FP64, where the arithmetic costs more than
-// the latency: the compact rows, each output looping over its nterm terms, outputs
-// taken in order of decreasing term count.
-static_assert(qed::kMaxTerms == 9, "terms 0..7 in one 16-byte row, term 8 beside it");
+// FP32: each output's terms padded to kMaxTerms with zero terms: an output is a fixed
+// kMaxTerms-term sum of 32-bit words (low half: the coefficient as FP16, exact for these
+// small integers; high half: the byte offset of the T entry), terms 0..3 and 4..7 in
+// two 16-byte rows and term 8 beside them, read with three independent loads; a term
+// costs one address add, one load and one mixed-precision multiply-add. FP64, where the
+// arithmetic costs more than the latency: the compact rows, each output looping over
+// its nterm terms, outputs taken in order of decreasing term count.
+static_assert(qed::kMaxTerms == 9, "terms 0..3, 4..7 in 16-byte rows, term 8 beside them");
 template <class R>
 struct TermLds;
 template <>
 struct TermLds<float> {
-  __attribute__((aligned(16))) unsigned short term[qed::kNL][8];
-  unsigned short term8[qed::kNL];
+  __attribute__((aligned(16))) unsigned term[2][qed::kNL][4];
+  unsigned term8[qed::kNL];
 };
 template <>
 struct TermLds<double> {
@@ -174,9 +180,16 @@ __device__ inline void wave_sync() {
 __device__ inline void stage_terms(const qed::TermTable& src, TermLds<float>* dst) {
   for (int i = threadIdx.x; i < qed::kNL * qed::kMaxTerms; i += blockDim.x) {
     const int o = i / qed::kMaxTerms, j = i % qed::kMaxTerms;
-    const unsigned short e = j < src.nterm[o] ? src.term[o][j] : (unsigned short)0;
-    if (j < 8) dst->term[o][j] = e;
-    else dst->term8[o] = e;
+    unsigned w = 0u;  // padding: coefficient 0 (T entry 0)
+    if (j < src.nterm[o]) {
+      const unsigned short e = src.term[o][j];
+      const _Float16 c = (_Float16)(float)qed::term_coef(e);
+      unsigned short cb;
+      __builtin_memcpy(&cb, &c, 2);
+      w = (unsigned)cb | ((unsigned)(qed::term_index(e) * sizeof(float)) << 16);
+    }
+    if (j < 8) dst->term[j >> 2][o][j & 3] = w;
+    else dst->term8[o] = w;
   }
 }
 __device__ inline void stage_terms(const qed::TermTable& src, TermLds<double>* dst) {
@@ -192,14 +205,17 @@ __device__ inline void stage_terms(const qed::TermTable& src, TermLds<double>* d
 // L0 output o: FP32, the padded terms in term order, all term reads and all T
 // reads independent of each other
 __device__ inline float l0_lds(const TermLds<float>& tt, int o, const float* t) {
-  const uint4 w = *reinterpret_cast<const uint4*>(&tt.term[o][0]);
-  const unsigned w8 = tt.term8[o];
-  const unsigned words[5] = {w.x, w.y, w.z, w.w, w8};
+  const uint4 w0 = *reinterpret_cast<const uint4*>(&tt.term[0][o][0]);
+  const uint4 w1 = *reinterpret_cast<const uint4*>(&tt.term[1][o][0]);
+  const unsigned words[qed::kMaxTerms] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w, tt.term8[o]};
+  const char* tb = reinterpret_cast<const char*>(t);
   float r = 0.f;
 #pragma unroll
   for (int j = 0; j < qed::kMaxTerms; ++j) {
-    const unsigned short e = (unsigned short)(words[j >> 1] >> (16 * (j & 1)));
-    r += float(qed::term_coef(e)) * t[qed::term_index(e)];
+    const unsigned short cb = (unsigned short)(words[j] & 0xffffu);
+    _Float16 c;
+    __builtin_memcpy(&c, &cb, 2);
+    r = fmaf((float)c, *reinterpret_cast<const float*>(tb + (words[j] >> 16)), r);
   }
   return r;
 }
@@ -239,8 +255,7 @@ template <class R>
 __device__ __forceinline__ void wave_l0_from(int e, const TermLds<R>& tt, const SiteGeom<R>& g,
                                              const R* ch, R* T, R* dst) {
   const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int k = 0; k < qed::kNT / 64; ++k) T[64 * k + lane] = kp_tensor(e, 64 * k + lane, g, ch);
+  kp_tensors(e, lane, g, ch, T);
   wave_sync();
 #pragma unroll 2
   for (int t = lane; t < qed::kNL; t += 64) {
@@ -304,8 +319,7 @@ __device__ __forceinline__ void wave_l0_kernel(int e, int m, const TermLds<float
                                                const float* l0x0, const float* du, const float* dv, bool in,
                                                float* C) {
   const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int k = 0; k < qed::kNT / 64; ++k) T[64 * k + lane] = kp_tensor(e, 64 * k + lane, g, ch);
+  kp_tensors(e, lane, g, ch, T);
   wave_sync();
   const int c = lane & 3, b = (lane >> 2) & 3, a = (lane >> 4) & 3;
   float dm[4], dn[4];

@@ -157,26 +157,42 @@ HLBL_HD R dxdsum(const R* f, R xa, R ya, R xb, R yb, bool ab) {
   return r;
 }
 
-// Component i = A*64 + (al*4+be)*4+de of the T tensors, from the channels ch
-// at the argument pair (x, y).
+// What the three T components of one index j = (al*4+be)*4+de share at the argument
+// pair (x, y): the picked coordinates, the Kronecker deltas and the invariants.
 template <class R>
-HLBL_HD R t_component(int i, const R x[4], const R y[4], const R* ch) {
-  const int A = i >> 6, al = (i >> 4) & 3, be = (i >> 2) & 3, de = i & 3;
-  const R dab = al == be ? R(1) : R(0);
-  const R dad = al == de ? R(1) : R(0);
-  const R dbd = be == de ? R(1) : R(0);
-  const R xa = pick4(x, al), xb = pick4(x, be), xd = pick4(x, de);
-  const R ya = pick4(y, al), yb = pick4(y, be), yd = pick4(y, de);
+struct TArgs {
+  R xa, xb, xd, ya, yb, yd, dab, dad, dbd, u, v, w;
+  bool ab;
+};
+template <class R>
+HLBL_HD TArgs<R> t_args(int j, const R x[4], const R y[4]) {
+  const int al = (j >> 4) & 3, be = (j >> 2) & 3, de = j & 3;
+  TArgs<R> t;
+  t.ab = al == be;
+  t.dab = al == be ? R(1) : R(0);
+  t.dad = al == de ? R(1) : R(0);
+  t.dbd = be == de ? R(1) : R(0);
+  t.xa = pick4(x, al), t.xb = pick4(x, be), t.xd = pick4(x, de);
+  t.ya = pick4(y, al), t.yb = pick4(y, be), t.yd = pick4(y, de);
+  t.u = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
+  t.v = y[0] * y[0] + y[1] * y[1] + y[2] * y[2] + y[3] * y[3];
+  t.w = x[0] * y[0] + x[1] * y[1] + x[2] * y[2] + x[3] * y[3];
+  return t;
+}
+
+// Component A*64 + j of the T tensors from the channels ch, with t = t_args(j, x, y).
+template <class R>
+HLBL_HD R t_value(int A, const TArgs<R>& t, const R* ch) {
+  const R xa = t.xa, xb = t.xb, xd = t.xd, ya = t.ya, yb = t.yb, yd = t.yd;
+  const R dab = t.dab, dad = t.dad, dbd = t.dbd;
   if (A == 0) {
     R r = dbd * (dx(ch[G1U], ch[G1W], xa, ya) + dx(ch[G2U], ch[G2W], xa, ya));
     r += dad * dsum(ch[G1U], ch[G1V], ch[G1W], xb, yb);
-    r += xd * dxdsum(ch + G1U, xa, ya, xb, yb, al == be);
-    r += yd * dxdsum(ch + G2U, xa, ya, xb, yb, al == be);
+    r += xd * dxdsum(ch + G1U, xa, ya, xb, yb, t.ab);
+    r += yd * dxdsum(ch + G2U, xa, ya, xb, yb, t.ab);
     return r;
   }
-  const R u = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
-  const R v = y[0] * y[0] + y[1] * y[1] + y[2] * y[2] + y[3] * y[3];
-  const R w = x[0] * y[0] + x[1] * y[1] + x[2] * y[2] + x[3] * y[3];
+  const R u = t.u, v = t.v, w = t.w;
   const R q = R(0.25), h = R(0.5);
   if (A == 1) {
     R r = (dab * xd + xb * dad - h * dbd * xa) * ch[L1];
@@ -195,6 +211,13 @@ HLBL_HD R t_component(int i, const R x[4], const R y[4], const R* ch) {
   r += (xa * yd + ya * xd - h * dad * w) * dsum(ch[L3U], ch[L3V], ch[L3W], xb, yb);
   r += q * dad * dsum(ch[G0U], ch[G0V], ch[G0W], xb, yb);
   return r;
+}
+
+// Component i = A*64 + (al*4+be)*4+de of the T tensors, from the channels ch
+// at the argument pair (x, y).
+template <class R>
+HLBL_HD R t_component(int i, const R x[4], const R y[4], const R* ch) {
+  return t_value(i >> 6, t_args(i & 63, x, y), ch);
 }
 
 // L0 output o from the T tensors

@@ -43,6 +43,10 @@ size_t div_up(size_t a, size_t b) { return (a + b - 1) / b; }
 // kNearRadius, a list built on the host) are then recomputed in FP64 with the
 // FP64 table, where the Lambda subtraction cancels.
 constexpr int kKmThreads = 256;
+// kfield_mixed: 8 waves share the term table; two workgroups per CU (four waves per
+// SIMD, which its launch bound asks for) fill the LDS exactly (term table 13,824 B +
+// 8 wave areas of 8,512 B = 81,920 B)
+constexpr int kKxThreads = 512;
 
 // pair constants of each distinct yv: cst64[u][768] and an FP32 copy; one wave per u
 __global__ __launch_bounds__(64) void kfield_consts_win(const double* __restrict__ table64,
@@ -97,20 +101,22 @@ __global__ __launch_bounds__(kKmThreads) void kfield_axes(const float* __restric
 // The field of each distinct yv in FP32 from the axis rows: per (yv, site) only
 // (x, y), (y, x) and (x, d) are evaluated; (x, 0) and (0, x) are this site's rows
 // and (0, d) is the (0, x) row of the site d = x - y.
-__global__ __launch_bounds__(kKmThreads) void kfield_mixed(
+__global__ __launch_bounds__(kKxThreads, 4) void kfield_mixed(
     const float* __restrict__ table32, const hlbl_coord* __restrict__ yvs, int n_unique,
     hlbl_geom global, double xunit, const float* __restrict__ cst32,
     const float* __restrict__ chbuf, const float* __restrict__ ax, float* __restrict__ coef) {
+  static_assert(sizeof(TermLds<float>) + kKxThreads / 64 * kKwLen3 * sizeof(float) <= 163840 / 2,
+                "two workgroups per CU");
   __shared__ TermLds<float> tt;
-  __shared__ float w[kKmThreads / 64][kKwLen3];
+  __shared__ float w[kKxThreads / 64][kKwLen3];
   stage_terms(c_terms32, &tt);
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: site math on the scalar unit
   const size_t V = geom_volume(global);
   const size_t total = (size_t)n_unique * V;
-  const size_t nw = (size_t)gridDim.x * (kKmThreads / 64);
+  const size_t nw = (size_t)gridDim.x * (kKxThreads / 64);
   const hlbl_coord zero = {0, 0, 0, 0};
-  for (size_t i = (size_t)blockIdx.x * (kKmThreads / 64) + wave; i < total; i += nw) {
+  for (size_t i = (size_t)blockIdx.x * (kKxThreads / 64) + wave; i < total; i += nw) {
     const int u = (int)(i / V);
     const size_t site = i - (size_t)u * V;
     const hlbl_coord yv = yvs[u];
@@ -676,8 +682,8 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
     hipLaunchKernelGGL(kfield_axes, dim3(nwa), dim3(kKmThreads), 0, st, c->qed->table32.p, g, w.xunit,
                        w.form == 2 ? w.chbuf.p : nullptr, w.ax.p);
     check_hip(hipGetLastError(), "kfield_axes launch");
-    const int nwg = (int)std::min<size_t>(div_up((size_t)nu * V, kKmThreads / 64), (size_t)4 * c->n_cu);
-    hipLaunchKernelGGL(kfield_mixed, dim3(nwg), dim3(kKmThreads), 0, st, c->qed->table32.p,
+    const int nwg = (int)std::min<size_t>(div_up((size_t)nu * V, kKxThreads / 64), (size_t)2 * c->n_cu);
+    hipLaunchKernelGGL(kfield_mixed, dim3(nwg), dim3(kKxThreads), 0, st, c->qed->table32.p,
                        w.d_yvs.p, nu, g, w.xunit, w.cst32.p, w.form == 2 ? w.chbuf.p : nullptr,
                        w.ax.p, w.coef.p);
     check_hip(hipGetLastError(), "kfield_mixed launch");
