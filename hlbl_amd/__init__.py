@@ -108,6 +108,9 @@ _SIGS = {
     "hlbl_max_steps_without_wrap": (C.c_int, [Coord, C.c_int, Coord, Coord, C.c_int]),
     "hlbl_run_chain": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_double, C.c_int,
                                  C.c_int, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "hlbl_run_chain_each_window": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_double, C.c_int,
+                                             C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
 }
 
 
@@ -403,15 +406,24 @@ def max_steps_without_wrap(h, stride, lattice, origin=(0, 0, 0, 0), cap=10000):
                                              cap)
 
 
-def run_chain(contractor, schedule, seed, xunit_, residency=0, form=0, batch=0):
+def run_chain(contractor, schedule, seed, xunit_, residency=0, form=0, batch=0, writer=None,
+              chain=0):
     """The schedule loop on random fields (src/driver.cpp:23-165).
 
+    With a ResultFile as writer, each window's rows are appended to it (and flushed)
+    when the window is done, as the reference's driver does (src/driver.cpp:134-137);
+    chain is the chain index stored in the rows.
     Returns (rows, stats); rows are dicts {x, y, yv, window, chain, kernel_sum}."""
     n_pairs = len(schedule.pairs())
     buf = (C.c_double * (16 * max(1, n_pairs)))()
     st = (C.c_double * 5)()
-    check(lib().hlbl_run_chain(contractor.handle, schedule.handle, C.c_uint64(seed), xunit_,
-                               residency, form, batch, buf, n_pairs, st))
+    fn = user = None
+    if writer is not None:
+        fn = C.cast(lib_h5().hlbl_result_append_rows, C.c_void_p)
+        user = writer.handle
+    check(lib().hlbl_run_chain_each_window(contractor.handle, schedule.handle, C.c_uint64(seed),
+                                           xunit_, residency, form, batch, chain, fn, user, buf,
+                                           n_pairs, st))
     rows = []
     for i in range(n_pairs):
         r = buf[16 * i:16 * i + 16]
@@ -420,3 +432,93 @@ def run_chain(contractor, schedule, seed, xunit_, residency=0, form=0, batch=0):
                      "chain": int(r[13]), "kernel_sum": [r[14], r[15]]})
     keys = ("pairs", "positions_solved", "cache_peak", "seconds_contract", "seconds_solve")
     return rows, dict(zip(keys, list(st)))
+
+
+# ---- result file (include/hlbl_h5.h; the reference's ResultWriter, src/output.cpp) ----
+
+H5_LIB_PATH = os.path.join(_HERE, "libhlbl_h5.so")
+_h5 = None
+
+
+class ChainInfo(C.Structure):
+    _fields_ = [("start", Coord), ("direction", Coord), ("stride", C.c_int), ("steps", C.c_int)]
+
+
+class RunInfo(C.Structure):
+    _fields_ = [("lattice", Coord), ("procs", Coord), ("nconf", C.c_int),
+                ("chains", C.POINTER(ChainInfo)), ("n_chains", C.c_int), ("kernel", C.c_char_p),
+                ("muon_mass_mev", C.c_double), ("lattice_spacing_fm", C.c_double),
+                ("xunit", C.c_double)]
+
+
+def lib_h5():
+    """The loaded result-file library; raises if it has not been built (it needs HDF5)."""
+    global _h5
+    if _h5 is None:
+        if not os.path.exists(H5_LIB_PATH):
+            raise HlblError(f"{H5_LIB_PATH} is not built: HDF5 headers were not found at build time "
+                            "(HLBL_HDF5_ROOT)")
+        L = C.CDLL(H5_LIB_PATH)
+        L.hlbl_result_open.restype = C.c_void_p
+        L.hlbl_result_open.argtypes = [C.c_char_p, C.POINTER(RunInfo)]
+        L.hlbl_result_append.restype = C.c_int
+        L.hlbl_result_append.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int]
+        L.hlbl_result_rows.restype = C.c_longlong
+        L.hlbl_result_rows.argtypes = [C.c_void_p]
+        L.hlbl_result_close.restype = C.c_int
+        L.hlbl_result_close.argtypes = [C.c_void_p]
+        L.hlbl_h5_last_error.restype = C.c_char_p
+        _h5 = L
+    return _h5
+
+
+class ResultFile:
+    """The reference's ResultWriter (src/output.cpp:131-247): one HDF5 file per run with
+    the kernel_sum group (x, y, yv, chain, window, kernel_sum), the chains table and the
+    run attributes; every append is flushed.
+
+    chains: [(start, direction, stride, steps)], steps resolved. The kernel attribute
+    names the QED kernel used; this build's plain kernel has stand-in weight functions,
+    so the default says so."""
+
+    def __init__(self, path, lattice_dims, chains, xunit_, muon_mass_mev=0.0, lattice_spacing_fm=0.0,
+                 procs=(1, 1, 1, 1), nconf=-1, kernel="LLambda0.4 (stand-in weight functions)"):
+        L = lib_h5()
+        arr = (ChainInfo * len(chains))(*[ChainInfo(_coord(s), _coord(d), int(st), int(n))
+                                          for s, d, st, n in chains])
+        info = RunInfo(_coord(lattice_dims), _coord(procs), int(nconf), arr, len(chains),
+                       kernel.encode(), float(muon_mass_mev), float(lattice_spacing_fm), float(xunit_))
+        h = L.hlbl_result_open(os.fsencode(path), C.byref(info))
+        if not h:
+            raise HlblError(L.hlbl_h5_last_error().decode())
+        self.handle = C.c_void_p(h)
+        self.path = path
+
+    def append(self, rows):
+        """rows: dicts as run_chain returns them"""
+        buf = (C.c_double * (16 * max(1, len(rows))))()
+        for i, r in enumerate(rows):
+            buf[16 * i:16 * i + 16] = [*r["x"], *r["y"], *r["yv"], r["window"], r.get("chain", 0),
+                                       *r["kernel_sum"]]
+        if lib_h5().hlbl_result_append(self.handle, buf, len(rows)) != 0:
+            raise HlblError(lib_h5().hlbl_h5_last_error().decode())
+
+    def rows(self):
+        return int(lib_h5().hlbl_result_rows(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            rc = lib_h5().hlbl_result_close(self.handle)
+            self.handle = None
+            if rc != 0:
+                raise HlblError(lib_h5().hlbl_h5_last_error().decode())
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _h5 is not None:
+            _h5.hlbl_result_close(self.handle)

@@ -59,7 +59,38 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    build_h5(verbose)
     return LIB
+
+
+# The result-file library (libhlbl_h5.so): host C++ linked against HDF5, kept apart so
+# that libhlbl_amd.so does not depend on HDF5. Built where HDF5's headers are found
+# (HLBL_HDF5_ROOT, else /opt/conda); otherwise skipped with a message.
+H5_SRC = os.path.join(HERE, "h5", "result_file.cpp")
+H5_LIB = os.path.join(HERE, "libhlbl_h5.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def hdf5_root():
+    root = os.environ.get("HLBL_HDF5_ROOT", "/opt/conda")
+    return root if os.path.exists(os.path.join(root, "include", "hdf5.h")) else None
+
+
+def build_h5(verbose=True):
+    root = hdf5_root()
+    if root is None:
+        if verbose:
+            print("HDF5 headers not found (HLBL_HDF5_ROOT): libhlbl_h5.so not built", file=sys.stderr)
+        return None
+    tmp = H5_LIB + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+           f"-I{INCLUDE}", f"-I{os.path.join(root, 'include')}", H5_SRC, "-o", tmp,
+           f"-L{os.path.join(root, 'lib')}", "-lhdf5", f"-Wl,-rpath,{os.path.join(root, 'lib')}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("libhlbl_h5.so build failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, H5_LIB)
+    return H5_LIB
 
 
 if __name__ == "__main__":

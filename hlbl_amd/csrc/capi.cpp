@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <thread>
 #include <vector>
@@ -431,7 +432,21 @@ struct PairRow {
   double ks[2];
 };
 void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xunit, bool host_tier,
-               int form, bool batch, std::vector<PairRow>& rows, double stats[5]);
+               int form, bool batch, std::vector<PairRow>& rows, double stats[5],
+               const std::function<void(size_t, size_t)>& on_window);
+
+// a row as the C ABI hands it out: x[4], y[4], yv[4], window, chain, ks0, ks1
+void pack_row(const PairRow& p, int chain, double* r) {
+  for (int k = 0; k < 4; ++k) {
+    r[k] = p.x[k];
+    r[4 + k] = p.y[k];
+    r[8 + k] = p.yv[k];
+  }
+  r[12] = p.window;
+  r[13] = chain;
+  r[14] = p.ks[0];
+  r[15] = p.ks[1];
+}
 
 // the window entry for every form; form 0 runs the FP64 pair path per pair
 void window_entry(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit, int form,
@@ -562,31 +577,38 @@ int hlbl_mlp_loss_grad(hlbl_qed* q, const float* d_in, const float* d_target, in
   });
 }
 
-int hlbl_run_chain(hlbl_contractor* c, const hlbl_schedule* s, uint64_t seed, double xunit,
-                   int residency, int form, int batch, double* rows, int max_rows,
-                   double stats[5]) {
+int hlbl_run_chain_each_window(hlbl_contractor* c, const hlbl_schedule* s, uint64_t seed,
+                               double xunit, int residency, int form, int batch, int chain,
+                               hlbl_window_rows_fn fn, void* user, double* rows, int max_rows,
+                               double stats[5]) {
   return guarded([&] {
     check_ptr(c, "contractor");
     check_ptr(s, "schedule");
     check_ptr(stats, "stats");
     if (residency != 0 && residency != 1) throw Error("residency must be 0 (device) or 1 (host)");
+    if (chain < 0) throw Error("chain must be non-negative");
     std::vector<PairRow> out;
-    run_chain(c, s->steps, seed, xunit, residency == 1, form, batch != 0, out, stats);
+    std::vector<double> packed;
+    std::function<void(size_t, size_t)> on_window;
+    if (fn)
+      on_window = [&](size_t first, size_t n) {
+        packed.assign(16 * n, 0.0);
+        for (size_t i = 0; i < n; ++i) pack_row(out[first + i], chain, packed.data() + 16 * i);
+        if (fn(user, packed.data(), (int)n) != 0)
+          throw Error("hlbl_run_chain: the window callback failed");
+      };
+    run_chain(c, s->steps, seed, xunit, residency == 1, form, batch != 0, out, stats, on_window);
     if ((int)out.size() > max_rows) throw Error("hlbl_run_chain: rows buffer too small");
     if (!out.empty()) check_ptr(rows, "rows");
-    for (size_t i = 0; i < out.size(); ++i) {
-      double* r = rows + 16 * i;
-      for (int k = 0; k < 4; ++k) {
-        r[k] = out[i].x[k];
-        r[4 + k] = out[i].y[k];
-        r[8 + k] = out[i].yv[k];
-      }
-      r[12] = out[i].window;
-      r[13] = 0;
-      r[14] = out[i].ks[0];
-      r[15] = out[i].ks[1];
-    }
+    for (size_t i = 0; i < out.size(); ++i) pack_row(out[i], chain, rows + 16 * i);
   });
+}
+
+int hlbl_run_chain(hlbl_contractor* c, const hlbl_schedule* s, uint64_t seed, double xunit,
+                   int residency, int form, int batch, double* rows, int max_rows,
+                   double stats[5]) {
+  return hlbl_run_chain_each_window(c, s, seed, xunit, residency, form, batch, 0, nullptr, nullptr,
+                                    rows, max_rows, stats);
 }
 
 }  // extern "C"

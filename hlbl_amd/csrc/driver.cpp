@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <functional>
 #include <map>
 #include <memory>
 #include <optional>
@@ -120,9 +121,12 @@ struct PairRow {
   double ks[2];
 };
 
-// rows in the reference's order: windows, new sources in solve order, partners
+// rows in the reference's order: windows, new sources in solve order, partners;
+// on_window(first, n) after each window with its rows [first, first + n), as the
+// reference writes each window's rows when it is done (src/driver.cpp:134-137)
 void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xunit, bool host_tier,
-               int form, bool batch, std::vector<PairRow>& rows, double stats[5]) {
+               int form, bool batch, std::vector<PairRow>& rows, double stats[5],
+               const std::function<void(size_t, size_t)>& on_window) {
   if (c->local.T != c->global.T || c->local.LX != c->global.LX || c->local.LY != c->global.LY ||
       c->local.LZ != c->global.LZ)
     throw Error("hlbl_run_chain: one process holds the whole lattice here");
@@ -153,6 +157,7 @@ void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xuni
   };
 
   for (const Step& st : s) {
+    const size_t window_first = rows.size();
     if (!batch) {
       for (const Source& src : st.sources) {
         solve(src.position);
@@ -203,6 +208,7 @@ void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xuni
       }
       for (const auto& d : done) cache.pair_done(d.first, d.second);
     }
+    if (on_window) on_window(window_first, rows.size() - window_first);
   }
   if (cache.size() != 0) throw Error("run_chain: propagators left in the cache");
   stats[0] = (double)rows.size();

@@ -490,13 +490,15 @@ __device__ inline void s2_dxu(int t0, int lane, const float* buf, const unsigned
   }
 }
 
-// Producer roles. Waves sit on SIMD (wave % 4), and on a SIMD the matrix and vector
-// work of its waves add up (they hardly co-issue), so the dxu tasks are spread to
-// even out MFMA + vector time per SIMD: SIMDs 0 and 1 carry three MFMA waves and
-// producer 12 or 13 with 4 tasks per lane; SIMDs 2 and 3 carry two MFMA waves and
-// producers 10 + 14 or 11 + 15 with 3 + 2 tasks, and these four stage the coefficients
-// of two sites each. Task ranges: pw 2: [0, 256), pw 3: [256, 512), pw 0: [512, 704),
-// pw 1: [704, 896), pw 4: [896, 1024), pw 5: [1024, 1152).
+// Producer roles. Waves whose indices agree mod 4 share a SIMD (the hardware id
+// register shows the SIMD numbers as a permutation of wave % 4), and on a SIMD the
+// matrix and vector work of its waves add up (they hardly co-issue), so the dxu tasks
+// are spread to even out MFMA + vector time per SIMD: the SIMDs of waves = 0 and 1
+// (mod 4) carry three MFMA waves and producer 12 or 13 with 4 tasks per lane; those of
+// waves = 2 and 3 carry two MFMA waves and producers 10 + 14 or 11 + 15 with 3 + 2
+// tasks, and these four stage the coefficients of two sites each. Task ranges:
+// pw 2: [0, 256), pw 3: [256, 512), pw 0: [512, 704), pw 1: [704, 896),
+// pw 4: [896, 1024), pw 5: [1024, 1152).
 __device__ inline int s2_task0(int pw) { return pw == 2 ? 0 : pw == 3 ? 256 : pw == 0 ? 512 : pw == 1 ? 704 : pw == 4 ? 896 : 1024; }
 __device__ inline int s2_n_tasks(int pw) { return (pw == 2 || pw == 3) ? 4 : (pw == 0 || pw == 1) ? 3 : 2; }
 __device__ inline int s2_coef_site0(int pw) { return pw == 0 ? 0 : pw == 1 ? 2 : pw == 4 ? 4 : pw == 5 ? 6 : -1; }

@@ -189,6 +189,16 @@ int hlbl_max_steps_without_wrap(hlbl_coord h, int stride, hlbl_coord lattice,
 int hlbl_run_chain(hlbl_contractor* c, const hlbl_schedule* s, uint64_t seed, double xunit,
                    int residency, int form, int batch, double* rows, int max_rows,
                    double stats[5]);
+/* The same for chain index `chain` (stored in each row), calling fn(user, rows, n)
+ * after each window with that window's rows (16 doubles each, as above), where the
+ * reference appends them to its result file (src/driver.cpp:134-137); a nonzero return
+ * from fn stops the run with an error. fn may be NULL. hlbl_result_append_rows of
+ * libhlbl_h5.so (include/hlbl_h5.h) is such an fn. */
+typedef int (*hlbl_window_rows_fn)(void* user, const double* rows, int n);
+int hlbl_run_chain_each_window(hlbl_contractor* c, const hlbl_schedule* s, uint64_t seed,
+                               double xunit, int residency, int form, int batch, int chain,
+                               hlbl_window_rows_fn fn, void* user, double* rows, int max_rows,
+                               double stats[5]);
 
 #ifdef __cplusplus
 }

@@ -4,6 +4,7 @@ FP64 forms: 1e-10 relative to the oracle (the reference's own tolerance,
 sanity_check/test_contract_random.cpp:55-57). Integer/byte work (fields,
 schedule rows): bit-exact. Mixed-precision forms: stated per test."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -213,6 +214,29 @@ def test_run_chain_tiers_and_direct(qed):
         assert r["yv"] == lat.site_map_diff(r["y"], r["x"])
     win_rows, _ = H.run_chain(con, sch, seed, xunit, residency=0, form=0, batch=1)
     assert win_rows == dev_rows
+    con.close()
+
+
+@pytest.mark.skipif(not os.path.exists(H.H5_LIB_PATH), reason="libhlbl_h5.so absent (no HDF5 at build time)")
+@pytest.mark.parametrize("batch", [0, 1])
+def test_run_chain_writes_result_file(qed, tmp_path, batch):
+    """The driver appends each window's rows to the result file as the window finishes
+    (src/driver.cpp:134-137): read back with h5dump, the file holds exactly the rows
+    run_chain returns (doubles bitwise), with the chain index given."""
+    from test_result_file import dataset
+    dims, xunit, seed = (6, 6, 6, 6), 1.0, 7
+    lat = H.Lattice(dims)
+    con = H.Contractor(lat, qed)
+    sch = H.Schedule((1, 1, 1, 1), 1, 2)
+    path = str(tmp_path / "chain.h5")
+    with H.ResultFile(path, dims, [((0, 0, 0, 0), (1, 1, 1, 1), 1, 2)], xunit) as f:
+        rows, _ = H.run_chain(con, sch, seed, xunit, residency=0, form=0, batch=batch, writer=f, chain=3)
+        assert f.rows() == len(rows) == 15 + 2 * 65
+    for name, key in (("x", "x"), ("y", "y"), ("yv", "yv"), ("window", "window"),
+                      ("kernel_sum", "kernel_sum"), ("chain", "chain")):
+        _, _, vals, _ = dataset(path, "/kernel_sum/" + name, tmp_path)
+        assert np.array_equal(vals, np.array([r[key] for r in rows])), name
+    assert {r["chain"] for r in rows} == {3}
     con.close()
 
 
