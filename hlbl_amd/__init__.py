@@ -410,6 +410,8 @@ def run_chain(contractor, schedule, seed, xunit_, residency=0, form=0, batch=0, 
               chain=0):
     """The schedule loop on random fields (src/driver.cpp:23-165).
 
+    batch 0: one pair per call; 1: one call per window (its sources solved first);
+    2: one call per new source with all its partners (the cache peak of batch 0).
     With a ResultFile as writer, each window's rows are appended to it (and flushed)
     when the window is done, as the reference's driver does (src/driver.cpp:134-137);
     chain is the chain index stored in the rows.

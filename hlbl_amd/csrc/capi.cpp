@@ -432,7 +432,7 @@ struct PairRow {
   double ks[2];
 };
 void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xunit, bool host_tier,
-               int form, bool batch, std::vector<PairRow>& rows, double stats[5],
+               int form, int batch, std::vector<PairRow>& rows, double stats[5],
                const std::function<void(size_t, size_t)>& on_window);
 
 // a row as the C ABI hands it out: x[4], y[4], yv[4], window, chain, ks0, ks1
@@ -587,6 +587,7 @@ int hlbl_run_chain_each_window(hlbl_contractor* c, const hlbl_schedule* s, uint6
     check_ptr(stats, "stats");
     if (residency != 0 && residency != 1) throw Error("residency must be 0 (device) or 1 (host)");
     if (chain < 0) throw Error("chain must be non-negative");
+    if (batch < 0 || batch > 2) throw Error("batch must be 0 (pairs), 1 (windows) or 2 (sources)");
     std::vector<PairRow> out;
     std::vector<double> packed;
     std::function<void(size_t, size_t)> on_window;
@@ -597,7 +598,7 @@ int hlbl_run_chain_each_window(hlbl_contractor* c, const hlbl_schedule* s, uint6
         if (fn(user, packed.data(), (int)n) != 0)
           throw Error("hlbl_run_chain: the window callback failed");
       };
-    run_chain(c, s->steps, seed, xunit, residency == 1, form, batch != 0, out, stats, on_window);
+    run_chain(c, s->steps, seed, xunit, residency == 1, form, batch, out, stats, on_window);
     if ((int)out.size() > max_rows) throw Error("hlbl_run_chain: rows buffer too small");
     if (!out.empty()) check_ptr(rows, "rows");
     for (size_t i = 0; i < out.size(); ++i) pack_row(out[i], chain, rows + 16 * i);

@@ -124,8 +124,11 @@ struct PairRow {
 // rows in the reference's order: windows, new sources in solve order, partners;
 // on_window(first, n) after each window with its rows [first, first + n), as the
 // reference writes each window's rows when it is done (src/driver.cpp:134-137)
+// batch 0: one pair per call; 1: one call per window (its sources solved first);
+// 2: one call per new source with all its partners (the solve order and cache peak of
+// batch 0, for lattices where a whole window's blocks do not fit)
 void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xunit, bool host_tier,
-               int form, bool batch, std::vector<PairRow>& rows, double stats[5],
+               int form, int batch, std::vector<PairRow>& rows, double stats[5],
                const std::function<void(size_t, size_t)>& on_window) {
   if (c->local.T != c->global.T || c->local.LX != c->global.LX || c->local.LY != c->global.LY ||
       c->local.LZ != c->global.LZ)
@@ -158,7 +161,31 @@ void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xuni
 
   for (const Step& st : s) {
     const size_t window_first = rows.size();
-    if (!batch) {
+    if (batch == 2) {
+      if (host_tier) throw Error("hlbl_run_chain: source batches read every block in place (device tier)");
+      for (const Source& src : st.sources) {
+        solve(src.position);
+        if (src.partners.empty()) continue;
+        const double* d_y = cache.device_ptr(src.position, 0, stm);
+        const size_t first = rows.size();
+        std::vector<hlbl_pair_desc> descs;
+        for (const Vec4& q : src.partners) {
+          PairRow r{wrap4(q, c->global), wrap4(src.position, c->global),
+                    min_image(src.position, q, c->global), st.index, {0, 0}};
+          descs.push_back({cache.device_ptr(q, 1, stm), d_y, to_c(r.x), to_c(r.yv)});
+          rows.push_back(r);
+        }
+        std::vector<double> out(2 * descs.size());
+        const auto t0 = clk::now();
+        window_entry(c, descs.data(), (int)descs.size(), xunit, form, out.data(), stm);
+        t_contract += std::chrono::duration<double>(clk::now() - t0).count();
+        for (size_t i = 0; i < descs.size(); ++i) {
+          rows[first + i].ks[0] = out[2 * i];
+          rows[first + i].ks[1] = out[2 * i + 1];
+        }
+        for (const Vec4& q : src.partners) cache.pair_done(q, src.position);
+      }
+    } else if (batch == 0) {
       for (const Source& src : st.sources) {
         solve(src.position);
         if (src.partners.empty()) continue;

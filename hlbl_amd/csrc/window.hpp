@@ -52,6 +52,12 @@ struct WindowPlan {
   PinBuf<double> h_judge;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
   ~WindowPlan() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);

@@ -157,51 +157,51 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
   }
   const int nu = (int)yvs.size();
 
-  WindowPlan* w = nullptr;
-  for (auto& p : cache->plans)
-    if (p->n_pairs == n && p->n_unique == nu && p->form == form && p->xunit == xunit) w = p.get();
-  if (!w) {
-    // one plan at a time keeps the footprint to one window
-    cache->plans.clear();
-    auto p = std::make_unique<WindowPlan>();
-    p->n_pairs = n;
-    p->n_unique = nu;
-    p->form = form;
-    p->xunit = xunit;
-    p->n_samples = kJudgeSamples;
+  // One plan whose buffers only grow: a window of another size or form reuses them and
+  // its graph is captured again, so the footprint is that of the largest window so far
+  // and a run of differently sized batches does not reallocate.
+  if (cache->plans.empty()) cache->plans.push_back(std::make_unique<WindowPlan>());
+  WindowPlan* w = cache->plans.front().get();
+  if (!(w->n_pairs == n && w->n_unique == nu && w->form == form && w->xunit == xunit)) {
+    w->drop_graph();
+    w->n_pairs = n;
+    w->n_unique = nu;
+    w->form = form;
+    w->xunit = xunit;
+    w->n_samples = kJudgeSamples;
     const size_t V = c->volume;
-    p->d_pairs.ensure(n);
-    p->h_pairs.ensure(n);
-    p->unit_slots = (2 * n + kPass1Groups - 1) / kPass1Groups;
-    p->d_units.ensure((size_t)kPass1Groups * p->unit_slots);
-    p->h_units.ensure((size_t)kPass1Groups * p->unit_slots);
-    p->d_yvs.ensure(nu);
-    p->h_yvs.ensure(nu);
-    p->part1.ensure(pass1_part_len(c->global, n));
-    p->red1.ensure(pass1_red_len(c->global, n));
-    p->xi.ensure((size_t)n * 2 * 8 * 5 * 144 * 2);
-    p->dz.ensure((size_t)n * 2 * 10 * 144 * 2);
-    p->gdz.ensure((size_t)n * 2 * 40 * 144 * 2);
-    p->bsplit.ensure(window_bsplit_len(n));
-    p->bexp.ensure(window_bexp_len(n));
-    p->coef.ensure((size_t)nu * V * 640);
-    p->ax.ensure((size_t)2 * V * qed::kNL);
-    p->cst64.ensure((size_t)nu * 2 * qed::kNL);
-    p->cst32.ensure((size_t)nu * 2 * qed::kNL);
-    if (form == 2) p->chbuf.ensure((size_t)(2 + 3 * nu) * V * qed::kNCh);
-    p->part2.ensure(window_part2_len(c, n));
-    p->result.ensure((size_t)2 * n);
-    p->h_result.ensure((size_t)2 * n);
-    // judge samples: a fixed spread of sites
-    std::vector<size_t> smp(kJudgeSamples);
-    for (int k = 0; k < kJudgeSamples; ++k) smp[k] = (size_t)(((unsigned long long)k * 2654435761ULL) % V);
-    p->d_samples.ensure(kJudgeSamples);
-    check_hip(hipMemcpy(p->d_samples.p, smp.data(), smp.size() * sizeof(size_t), hipMemcpyHostToDevice),
-              "hipMemcpy samples");
-    p->judge.ensure((size_t)nu * kJudgeSamples);
-    p->h_judge.ensure((size_t)nu * kJudgeSamples);
-    cache->plans.push_back(std::move(p));
-    w = cache->plans.back().get();
+    w->d_pairs.ensure(n);
+    w->h_pairs.ensure(n);
+    w->unit_slots = (2 * n + kPass1Groups - 1) / kPass1Groups;
+    w->d_units.ensure((size_t)kPass1Groups * w->unit_slots);
+    w->h_units.ensure((size_t)kPass1Groups * w->unit_slots);
+    w->d_yvs.ensure(nu);
+    w->h_yvs.ensure(nu);
+    w->part1.ensure(pass1_part_len(c->global, n));
+    w->red1.ensure(pass1_red_len(c->global, n));
+    w->xi.ensure((size_t)n * 2 * 8 * 5 * 144 * 2);
+    w->dz.ensure((size_t)n * 2 * 10 * 144 * 2);
+    w->gdz.ensure((size_t)n * 2 * 40 * 144 * 2);
+    w->bsplit.ensure(window_bsplit_len(n));
+    w->bexp.ensure(window_bexp_len(n));
+    w->coef.ensure((size_t)nu * V * 640);
+    w->ax.ensure((size_t)2 * V * qed::kNL);
+    w->cst64.ensure((size_t)nu * 2 * qed::kNL);
+    w->cst32.ensure((size_t)nu * 2 * qed::kNL);
+    if (form == 2) w->chbuf.ensure((size_t)(2 + 3 * nu) * V * qed::kNCh);
+    w->part2.ensure(window_part2_len(c, n));
+    w->result.ensure((size_t)2 * n);
+    w->h_result.ensure((size_t)2 * n);
+    if (w->d_samples.n == 0) {
+      // judge samples: a fixed spread of sites
+      std::vector<size_t> smp(kJudgeSamples);
+      for (int k = 0; k < kJudgeSamples; ++k) smp[k] = (size_t)(((unsigned long long)k * 2654435761ULL) % V);
+      w->d_samples.ensure(kJudgeSamples);
+      check_hip(hipMemcpy(w->d_samples.p, smp.data(), smp.size() * sizeof(size_t), hipMemcpyHostToDevice),
+                "hipMemcpy samples");
+    }
+    w->judge.ensure((size_t)nu * kJudgeSamples);
+    w->h_judge.ensure((size_t)nu * kJudgeSamples);
   }
   for (int i = 0; i < n; ++i) {
     // gsx inside the lattice: the kernels step site coordinates relative to it

@@ -182,7 +182,9 @@ int hlbl_max_steps_without_wrap(hlbl_coord h, int stride, hlbl_coord lattice,
 /* ---- driver: the schedule loop on random fields (src/driver.cpp:23-208) ------
  * Runs one chain: solve (device RandomProvider), cache (residency 0 = device,
  * 1 = pinned host with two staging slots), contract per pair (form as above;
- * batch 0 = one pair per call in solve order, 1 = window batches), free blocks
+ * batch 0 = one pair per call in solve order, 1 = one call per window, its sources
+ * solved first, 2 = one call per new source with all its partners, in the solve order
+ * and with the cache peak of batch 0; 1 and 2 need the device tier), free blocks
  * after their last pair. rows gets n_pairs * 16 doubles:
  * x[4], y[4], yv[4], window, chain(=0), ks0, ks1 (ints stored as doubles).
  * stats = {pairs, positions_solved, cache_peak, seconds_contract, seconds_solve}. */

@@ -214,6 +214,9 @@ def test_run_chain_tiers_and_direct(qed):
         assert r["yv"] == lat.site_map_diff(r["y"], r["x"])
     win_rows, _ = H.run_chain(con, sch, seed, xunit, residency=0, form=0, batch=1)
     assert win_rows == dev_rows
+    # one call per new source: same rows, and the pair loop's cache peak
+    src_rows, sst = H.run_chain(con, sch, seed, xunit, residency=0, form=0, batch=2)
+    assert src_rows == dev_rows and sst["cache_peak"] == expect["peak_cache"]
     con.close()
 
 
@@ -281,4 +284,42 @@ def test_config_16x32_fp64_and_window(qed, oracle):
     err = np.abs(f1 - ref) / np.abs(ref)
     log("16^3x32 window 1, form 1: max rel err", err.max(), "median", np.median(err))
     assert err.max() < 1e-3
+    con.close()
+
+
+def test_config_32x64_two_chains_source_batches(qed):
+    """The at-scale workload: 32^3x64 (9.66 GB blocks), two chains of K=4 with
+    h=(+1,+1,+1,+1) and (-1,+1,-1,+1), n=4, RandomProvider seed 3, at most 16 blocks
+    alive. A window touches 31 blocks (300 GB, more than HBM), so each chain runs the
+    FP32 window form with one call per new source and all its partners (batch 2): the
+    solve order and cache peak of the pair loop (the schedule's interleaved peak, 16).
+    The first, middle and last pair of every window against the FP64 path: below 1e-3
+    relative per pair and flavour."""
+    torch.cuda.empty_cache()  # the chain's blocks are allocated outside torch's cache
+    dims = (64, 32, 32, 32)
+    xunit, seed = 105.658 / 197.3269804 * 0.1, 3
+    lat = H.Lattice(dims)
+    con = H.Contractor(lat, qed)
+    errs = []
+    for h in ((1, 1, 1, 1), (-1, 1, -1, 1)):
+        sch = H.Schedule(h, 4, 4)
+        assert sch.simulate(True)["peak_cache"] == 16
+        rows, st = H.run_chain(con, sch, seed, xunit, residency=0, form=1, batch=2)
+        assert len(rows) == 275 and st["cache_peak"] == 16
+        assert [(r["y"], r["x"]) for r in rows] == \
+            [(tuple(np.mod(p, dims)), tuple(np.mod(q, dims))) for q, p in sch.pairs()]
+        by_window = {}
+        for i, r in enumerate(rows):
+            by_window.setdefault(r["window"], []).append(i)
+        for idx in by_window.values():
+            for i in sorted({idx[0], idx[len(idx) // 2], idx[-1]}):
+                r = rows[i]
+                X = dev_block(lat, H.source_seed(seed, r["x"]))
+                Y = dev_block(lat, H.source_seed(seed, r["y"]))
+                ref = con.kernel_sum(X.data_ptr(), Y.data_ptr(), r["x"], r["yv"], xunit)
+                errs.append(max(abs(a - b) / abs(b) for a, b in zip(r["kernel_sum"], ref)))
+                del X, Y
+        log("32^3x64 chain", h, "contract s", st["seconds_contract"], "solve s", st["seconds_solve"])
+    log("32^3x64 form 1, sampled pairs: max rel err", max(errs), "median", np.median(errs))
+    assert len(errs) == 30 and max(errs) < 1e-3
     con.close()
