@@ -138,8 +138,31 @@ int as_int(double v, const char* what) {
   return (int)v;
 }
 
+// HDF5's automatic error printing off for the duration of one entry point (errors come
+// back as status codes and a message here), the caller's handler restored afterwards
+class QuietErrors {
+ public:
+  QuietErrors() {
+    if (H5Eget_auto2(H5E_DEFAULT, &func_, &data_) >= 0) {
+      saved_ = true;
+      H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+    }
+  }
+  ~QuietErrors() {
+    if (saved_) H5Eset_auto2(H5E_DEFAULT, func_, data_);
+  }
+  QuietErrors(const QuietErrors&) = delete;
+  QuietErrors& operator=(const QuietErrors&) = delete;
+
+ private:
+  H5E_auto2_t func_ = nullptr;
+  void* data_ = nullptr;
+  bool saved_ = false;
+};
+
 template <class F>
 int guarded(F&& f) {
+  QuietErrors quiet;
   try {
     f();
     return 0;
@@ -166,7 +189,6 @@ hlbl_result_file* hlbl_result_open(const char* path, const hlbl_run_info* info) 
   guarded([&] {
     if (!path || !info) throw H5Error("path and run info are required");
     if (info->n_chains <= 0 || !info->chains) throw H5Error("no chains");
-    H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);  // errors come back as status codes
     auto f = std::make_unique<hlbl_result_file>();
     f->file = Id(H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose, "create file");
     f->group = Id(H5Gcreate2(*f->file, "kernel_sum", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Gclose,
