@@ -436,6 +436,25 @@ def run_chain(contractor, schedule, seed, xunit_, residency=0, form=0, batch=0, 
     return rows, dict(zip(keys, list(st)))
 
 
+def run_chains(contractor, schedules, seed, xunit_, residency=0, form=0, batch=0, writer=None):
+    """run_chains (src/driver.cpp:169-208): the chains one after the other, chain index
+    c stored in their rows; rows concatenated in chain order; stats summed, the cache
+    peak the largest of the chains'. With a writer, every window of every chain is
+    appended as it finishes."""
+    if not schedules:
+        raise HlblError("run_chains: no chains")
+    rows, total = [], None
+    for c, sch in enumerate(schedules):
+        r, st = run_chain(contractor, sch, seed, xunit_, residency, form, batch, writer, chain=c)
+        rows += r
+        if total is None:
+            total = dict(st)
+        else:
+            for k, v in st.items():
+                total[k] = max(total[k], v) if k == "cache_peak" else total[k] + v
+    return rows, total
+
+
 # ---- result file (include/hlbl_h5.h; the reference's ResultWriter, src/output.cpp) ----
 
 H5_LIB_PATH = os.path.join(_HERE, "libhlbl_h5.so")

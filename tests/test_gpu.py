@@ -243,6 +243,34 @@ def test_run_chain_writes_result_file(qed, tmp_path, batch):
     con.close()
 
 
+@pytest.mark.skipif(not os.path.exists(H.H5_LIB_PATH), reason="libhlbl_h5.so absent (no HDF5 at build time)")
+def test_run_chains_two_chains(qed, tmp_path):
+    """run_chains (src/driver.cpp:169-208) against independent runs, as the reference's
+    test_driver_random.cpp:237-310: rows = each chain's own rows with its index, cache
+    peak = the larger one, one file with both chains' rows and the chain-list layout."""
+    from test_result_file import dataset, has_attribute
+    dims, xunit, seed = (6, 6, 6, 6), 1.0, 7
+    lat = H.Lattice(dims)
+    con = H.Contractor(lat, qed)
+    chains = [((1, 1, 1, 1), 1, 2), ((-1, 1, -1, 1), 1, 1)]
+    schedules = [H.Schedule(h, n, k) for h, n, k in chains]
+    path = str(tmp_path / "two.h5")
+    info = [((0, 0, 0, 0), h, n, k) for h, n, k in chains]
+    with H.ResultFile(path, dims, info, xunit) as f:
+        rows, st = H.run_chains(con, schedules, seed, xunit, residency=0, form=0, batch=1, writer=f)
+    alone = [H.run_chain(con, s, seed, xunit, residency=0, form=0, batch=1) for s in schedules]
+    assert rows == [dict(r, chain=c) for c, (rs, _) in enumerate(alone) for r in rs]
+    assert st["pairs"] == len(rows) == sum(len(s.pairs()) for s in schedules)
+    assert st["cache_peak"] == max(a[1]["cache_peak"] for a in alone)
+    assert st["positions_solved"] == sum(a[1]["positions_solved"] for a in alone)
+    _, _, ks, _ = dataset(path, "/kernel_sum/kernel_sum", tmp_path)
+    _, _, ch, _ = dataset(path, "/kernel_sum/chain", tmp_path)
+    assert np.array_equal(ks, np.array([r["kernel_sum"] for r in rows]))
+    assert ch.tolist() == [r["chain"] for r in rows]
+    assert not has_attribute(path, "direction")
+    con.close()
+
+
 def test_config_16x32_fp64_and_window(qed, oracle):
     """The configured plain-path workload: 16^3x32, chain K=4, n=2, h=(+1,-1,+1,-1),
     RandomProvider seed 1, a = 0.1 fm. The FP64 path against the oracle to 1e-10 on
