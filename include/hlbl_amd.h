@@ -83,7 +83,8 @@ int hlbl_qed_channels(hlbl_qed* q, int mode, const double* d_acb, int n, double*
  * 3 inputs (|x|, |y|, cos beta in muon units), 4 hidden layers of 128 GELU,
  * n_channels outputs. BF16 weights, FP32 accumulation, FP32 master weights.
  * Trained on the GPU by this library's own forward/backward kernels on samples of
- * the plain table evaluation. No counterpart in the reference (north-star part). */
+ * the plain table evaluation. No counterpart in the reference (the surrogate is this
+ * library's addition). */
 int hlbl_mlp_param_count(const hlbl_qed* q);
 int hlbl_mlp_init(hlbl_qed* q, uint64_t seed);
 /* one full training run; loss history (one value per epoch) goes to host `loss` */
@@ -141,9 +142,10 @@ int hlbl_4pt_contraction(hlbl_contractor* c, double* d_kernel_sum, const double*
  * form 0 = plain FP64; 1 = FP32 products + FP64 Kahan sums + plain kernel;
  * 2 = FP32 + surrogate kernel (plain FP64 near the origin and outside the trained
  * domain). Pass 1 and pass 2 of all pairs each run as one persistent launch; the
- * sequence is captured once as a hipGraph and replayed for later windows with the
- * same n. out is host [n][2]. No counterpart in the reference (north-star
- * window batching); it replaces the pair loop of src/driver.cpp:110-132. */
+ * sequence is captured as a hipGraph and replayed for later windows of the same size
+ * and form (another size reuses the buffers, which only grow, and is captured again).
+ * out is host [n][2]. No counterpart in the reference (window batching is this
+ * library's addition); it replaces the pair loop of src/driver.cpp:110-132. */
 typedef struct {
   const double* d_x;
   const double* d_y;
