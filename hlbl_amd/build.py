@@ -83,9 +83,14 @@ def build_h5(verbose=True):
             print("HDF5 headers not found (HLBL_HDF5_ROOT): libhlbl_h5.so not built", file=sys.stderr)
         return None
     tmp = H5_LIB + ".tmp"
+    # The C++ runtime is linked in statically: the HDF5 directory's rpath would otherwise
+    # also resolve libstdc++ there, and an older one loaded first breaks libhlbl_amd.so.
+    # libhdf5 is linked by its full path, so no -L puts that directory's libstdc++ first.
+    lib_dir = os.path.join(root, "lib")
     cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
            f"-I{INCLUDE}", f"-I{os.path.join(root, 'include')}", H5_SRC, "-o", tmp,
-           f"-L{os.path.join(root, 'lib')}", "-lhdf5", f"-Wl,-rpath,{os.path.join(root, 'lib')}"]
+           os.path.join(lib_dir, "libhdf5.so"), f"-Wl,-rpath,{lib_dir}",
+           "-static-libstdc++", "-static-libgcc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("libhlbl_h5.so build failed:\n" + r.stdout + r.stderr)

@@ -184,3 +184,13 @@ def test_errors(tmp_path):
     f.append(rows_of(2, 0))
     assert f.rows() == 2
     f.close()
+
+
+def test_loads_before_the_main_library():
+    """The result-file library carries its own C++ runtime: loaded first in a process,
+    it must not bring in an older libstdc++ from HDF5's directory that the main
+    library cannot use."""
+    code = (f"import sys; sys.path.insert(0, {ROOT!r}); import hlbl_amd as H\n"
+            "H.lib_h5(); H.lib(); print('ok')\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr[-2000:]
