@@ -88,6 +88,8 @@ _SIGS = {
                                 C.c_int, Coord, Coord, Geom, Geom, C.c_void_p]),
     "hlbl_insert_g5_gamma": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p]),
+    "hlbl_pieces_2p2_p1": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, Coord, C.c_void_p,
+                                     C.c_void_p]),
     "hlbl_4pt_contraction": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_int, Coord, Coord, Pair,
                                        Coord, Geom, Geom, C.c_void_p]),
@@ -344,6 +346,14 @@ class Contractor:
                                          C.c_void_p(fwd_src), C.c_void_p(fwd_y), iflavor,
                                          _coord(L.proc_coords), _coord(gsx), Pair(xunit_, 0.0),
                                          _coord(yv), _geom(L.global_), _geom(L.local), stream))
+
+
+    def pieces_2p2_p1(self, d_fwd, gsw, d_p1, iflavor=0, stream=None):
+        """The P1 piece of the 2+2 contribution of the block at d_fwd for source gsw
+        (Contractor::pieces_2p2, src/contract.cpp:116-150) into the device array d_p1 of
+        [4][4][4][Lmax] doubles, Lmax the largest global extent (synchronous)."""
+        check(lib().hlbl_pieces_2p2_p1(self.handle, C.c_void_p(d_fwd), iflavor, _coord(gsw),
+                                       C.c_void_p(d_p1), stream))
 
 
 def insert_g5_gamma(d_dzu, d_dzsu, d_g_dzu, d_g_dzsu, stream=None):

@@ -281,6 +281,23 @@ int hlbl_dzu_dzsu(hlbl_contractor* c, double* d_dzu, double* d_dzsu, const doubl
   });
 }
 
+int hlbl_pieces_2p2_p1(hlbl_contractor* c, const double* d_fwd, int iflavor, hlbl_coord gsw,
+                       double* d_p1, hlbl_stream stream) {
+  return guarded([&] {
+    check_ptr(c, "contractor");
+    check_ptr(d_fwd, "d_fwd");
+    check_ptr(d_p1, "d_p1");
+    if (iflavor != 0 && iflavor != 1) throw Error("iflavor must be 0 or 1");
+    const hlbl_geom& g = c->global;
+    if (gsw.t < 0 || gsw.t >= g.T || gsw.x < 0 || gsw.x >= g.LX || gsw.y < 0 || gsw.y >= g.LY ||
+        gsw.z < 0 || gsw.z >= g.LZ)
+      throw Error("gsw must be a global site in [0, L)");
+    hipStream_t st = as_stream(stream);
+    launch_p1_2p2(c, d_fwd, iflavor, gsw, d_p1, st);
+    check_hip(hipStreamSynchronize(st), "pieces_2p2_p1");
+  });
+}
+
 int hlbl_insert_g5_gamma(const double* d_dzu, const double* d_dzsu, double* d_g_dzu,
                          double* d_g_dzsu, hlbl_stream stream) {
   return guarded([&] {

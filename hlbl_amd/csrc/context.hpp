@@ -127,6 +127,7 @@ struct hlbl_contractor {
   hlbl::DevBuf<double> coef;    // pass-2 coefficient field [V][16][40]
   hlbl::DevBuf<double> kcst;    // the pair constants L0(0,y), L0(y,0) [2][384]
   hlbl::DevBuf<double> part2;   // pass-2 partials [nwg2][12]
+  hlbl::DevBuf<double> pi2p2;   // the 2+2 pieces' Pi_{mu nu}(z) [V][16]
   hlbl::DevBuf<double> result;  // [2]
   hlbl::PinBuf<double> host_result;
   hlbl::WindowCache* window = nullptr;

@@ -27,6 +27,9 @@ void launch_pass2(hlbl_contractor* c, const double* S, const double* Y, const do
 
 // dst[i] += src[i]
 void launch_add(double* dst, const double* src, size_t n, hipStream_t st);
+// the 2+2 piece P1 of one block (flavour f) for source w into d_p1 [4][4][4][lmax]
+void launch_p1_2p2(hlbl_contractor* c, const double* blk, int f, hlbl_coord gsw, double* d_p1,
+                   hipStream_t st);
 
 // device random fill of one block
 void launch_fill_random(double* d_out, hlbl_geom local, hlbl_coord proc, uint64_t seed,

@@ -164,6 +164,20 @@ int hlbl_window_judge(const hlbl_contractor* c, double* max_rel_err, int* n_samp
 int hlbl_window_profile(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n_pairs,
                         double xunit, int form, double* out, double ms[4]);
 
+/* ---- 2+2 pieces: P1 ------------------------------------------------------------
+ * The P1 piece of the 2+2 contribution of one source block d_fwd at the global site
+ * gsw (Contractor::pieces_2p2, src/contract.cpp:116-150; kernel_pi and kernel_p1 of
+ * compute_2p2_gpu, kernels/hlbl_kernels/kernels.cu:241-340), FP64 in a fixed order:
+ *   Pi_{mu nu}(z) = sum_a Re[(gamma_nu gamma_5 w_a)[a]],
+ *   w_a[b] = sum_c conj(D_b[c]) (gamma_5 gamma_mu U_a)[c], U = block[iflavor], D = the other;
+ *   d_p1[rho][mu][nu][zeta] (device, 64 * Lmax doubles, Lmax the largest global extent,
+ *   overwritten) = sum of Pi_{mu nu}(z) over this process's sites with
+ *   (z - gsw)_rho mod L_rho = zeta; 0 for bins it does not hold (sum over processes:
+ *   the identity on one). Synchronous on stream. The pieces P2..P4, which need the QED
+ *   kernel at the target geometries, are not provided. */
+int hlbl_pieces_2p2_p1(hlbl_contractor* c, const double* d_fwd, int iflavor, hlbl_coord gsw,
+                       double* d_p1, hlbl_stream stream);
+
 /* ---- sliding-hypercube schedule (src/schedule.cpp:34-214) -------------------- */
 hlbl_schedule* hlbl_schedule_build(hlbl_coord h, int stride, int n_steps, hlbl_coord origin);
 void hlbl_schedule_free(hlbl_schedule* s);

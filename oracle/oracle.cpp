@@ -761,11 +761,42 @@ int oracle_g_terms(int* o, int* i, double* c, int max) {
   return n;
 }
 
-// sum over all entries of the 2+2 piece P1 of one block (kernels.cu:241-340):
-// P1[rho][sigma][nu][zeta] bins Pi_{sigma nu}(z) along rho, so its total is
-// 4 * sum_z sum_{sigma nu} Pi_{sigma nu}(z), with
+}  // extern "C"
+
+// Pi_{mu nu}(z) of the 2+2 pieces (kernels.cu:241-294) at local site z:
 //   Pi_{mu nu}(z) = sum_a Re[(gamma_nu gamma_5 w_a)[a]],
 //   w_a[b] = sum_c conj(D_b[c]) (g5 gamma_mu U_a)[c], U = block[f], D = block[1-f]
+static void pi_site(const double* blk, size_t V, int f, size_t z, double pi[16]) {
+  cd u[12][12], d[12][12];
+  for (int a = 0; a < 12; ++a) {
+    load12(spinor(blk, V, f, a, z), u[a]);
+    load12(spinor(blk, V, 1 - f, a, z), d[a]);
+  }
+  for (int mu = 0; mu < 4; ++mu)
+    for (int nu = 0; nu < 4; ++nu) {
+      double acc = 0;
+      for (int a = 0; a < 12; ++a) {
+        cd t1[12], gu[12], wv[12], t5[12], q[12];
+        gmul(mu, u[a], t1);
+        gmul(5, t1, gu);
+        for (int b = 0; b < 12; ++b) {
+          cd w = 0;
+          for (int c = 0; c < 12; ++c) w += std::conj(d[b][c]) * gu[c];
+          wv[b] = w;
+        }
+        gmul(5, wv, t5);
+        gmul(nu, t5, q);
+        acc += q[a].real();
+      }
+      pi[mu * 4 + nu] = acc;
+    }
+}
+
+extern "C" {
+
+// sum over all entries of the 2+2 piece P1 of one block (kernels.cu:241-340):
+// P1[rho][sigma][nu][zeta] bins Pi_{sigma nu}(z) along rho, so its total is
+// 4 * sum_z sum_{sigma nu} Pi_{sigma nu}(z)
 double oracle_sum_p1(const double* blk, const int* local, int f) {
   const int pc0[4] = {0, 0, 0, 0};
   const Geo g = make_geo(local, local, pc0);
@@ -776,33 +807,39 @@ double oracle_sum_p1(const double* blk, const int* local, int f) {
     double mine = 0;
 #pragma omp for schedule(static)
     for (long long z = 0; z < (long long)g.V; ++z) {
-      for (int mu = 0; mu < 4; ++mu)
-        for (int nu = 0; nu < 4; ++nu) {
-          double pi = 0;
-          for (int a = 0; a < 12; ++a) {
-            cd u[12], t1[12], gu[12], wv[12], t5[12], q[12];
-            load12(spinor(blk, g.V, f, a, z), u);
-            gmul(mu, u, t1);
-            gmul(5, t1, gu);
-            for (int b = 0; b < 12; ++b) {
-              cd d[12];
-              load12(spinor(blk, g.V, 1 - f, b, z), d);
-              cd acc = 0;
-              for (int c = 0; c < 12; ++c) acc += std::conj(d[c]) * gu[c];
-              wv[b] = acc;
-            }
-            gmul(5, wv, t5);
-            gmul(nu, t5, q);
-            pi += q[a].real();
-          }
-          mine += pi;
-        }
+      double pi[16];
+      pi_site(blk, g.V, f, (size_t)z, pi);
+      for (int k = 0; k < 16; ++k) mine += pi[k];
     }
     part[omp_get_thread_num()] = mine;
   }
   double s = 0;
   for (double v : part) s += v;
   return 4.0 * s;
+}
+
+// The 2+2 piece P1 of one block on one process (kernel_p1, kernels.cu:296-340):
+// out[rho][sigma][nu][zeta] = sum of Pi_{sigma nu}(z) over the sites with
+// (z - w)_rho mod L_rho = zeta, zeta < lmax (0 for zeta >= L_rho); w = gsw.
+void oracle_p1(const double* blk, const int* dims, int f, const int* gsw, int lmax, double* out) {
+  const int pc0[4] = {0, 0, 0, 0};
+  const Geo g = make_geo(dims, dims, pc0);
+  std::vector<double> pi(g.V * 16);
+#pragma omp parallel for schedule(static)
+  for (long long z = 0; z < (long long)g.V; ++z) pi_site(blk, g.V, f, (size_t)z, pi.data() + 16 * z);
+  std::memset(out, 0, sizeof(double) * 64 * lmax);
+  for (size_t z = 0; z < g.V; ++z) {
+    int c[4];
+    size_t q = z;
+    for (int r = 3; r >= 0; --r) {
+      c[r] = (int)(q % dims[r]);
+      q /= dims[r];
+    }
+    for (int rho = 0; rho < 4; ++rho) {
+      const int zeta = ((c[rho] - gsw[rho]) % dims[rho] + dims[rho]) % dims[rho];
+      for (int k = 0; k < 16; ++k) out[(rho * 16 + k) * lmax + zeta] += pi[16 * z + k];
+    }
+  }
 }
 
 void oracle_set_threads(int n) {

@@ -147,6 +147,40 @@ def test_seam2_launchers_vs_oracle(qed, oracle):
     con.close()
 
 
+@pytest.mark.parametrize("dims,seed,gsw,f", [((6, 6, 6, 6), 3, (0, 0, 0, 0), 0),
+                                             ((8, 4, 6, 6), 17, (3, 1, 5, 2), 1)])
+def test_pieces_2p2_p1_vs_oracle(qed, oracle, dims, seed, gsw, f):
+    """The 2+2 piece P1 (kernels.cu:241-340) against the oracle to 1e-10 of its largest
+    entry: on the reference's golden geometry (6^4, seed 3, flavour 0), whose total is
+    the reference's golden sum, and on a non-cubic lattice with the source off the
+    origin (bins beyond L_rho zero). Two calls agree bitwise (fixed summation order)."""
+    import json
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_values.json")))
+    lat = H.Lattice(dims)
+    con = H.Contractor(lat, qed)
+    blk = dev_block(lat, seed)
+    lmax = max(dims)
+    d = torch.empty(64 * lmax, dtype=torch.float64, device="cuda")
+    con.pieces_2p2_p1(blk.data_ptr(), gsw, d.data_ptr(), iflavor=f)
+    got = d.cpu().numpy().reshape(4, 4, 4, lmax)
+    con.pieces_2p2_p1(blk.data_ptr(), gsw, d.data_ptr(), iflavor=f)
+    assert np.array_equal(d.cpu().numpy().reshape(4, 4, 4, lmax), got)
+    want = np.zeros((4, 4, 4, lmax))
+    oracle.oracle_p1.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                 ctypes.c_int, ctypes.c_void_p]
+    oracle.oracle_p1(P(blk.cpu().numpy()), P(np.array(dims, np.int32)), f, P(np.array(gsw, np.int32)),
+                     lmax, P(want))
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    log("P1", dims, "max rel err", err)
+    assert err <= 1e-10
+    for rho, L in enumerate(dims):
+        assert np.all(got[rho, :, :, L:] == 0)
+    g = gold["sum_p1_2p2"]
+    if (dims, seed, f) == (tuple(g["lattice"]), g["seed"], g["flavour"]):
+        assert abs(got.sum() - g["sum_p1"]) <= 1e-10 * abs(g["sum_p1"])
+    con.close()
+
+
 def test_window_forms(qed):
     """Form 0 (FP64 per pair) equals kernel_sum; form 1 (FP32 products, FP64 Kahan
     sums) agrees with FP64 to 1e-4 relative per pair sum."""

@@ -36,6 +36,45 @@ def test_sum_p1_detects_a_wrong_seed(oracle):
     assert abs(s - g["sum_p1"]) > 1e-3
 
 
+def oracle_p1(oracle, blk, dims, gsw, f=0):
+    lmax = max(dims)
+    out = np.zeros((4, 4, 4, lmax))
+    oracle.oracle_p1.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                 ctypes.c_int, ctypes.c_void_p]
+    oracle.oracle_p1(P(blk), P(np.array(dims, np.int32)), f, P(np.array(gsw, np.int32)), lmax, P(out))
+    return out
+
+
+def test_p1_total_golden(oracle):
+    """The binned P1 (kernels.cu:296-340) holds the reference's golden total in every
+    rho slice: each slice bins every site once."""
+    g = GOLD["sum_p1_2p2"]
+    blk = fill_block(oracle, g["lattice"], g["seed"])
+    p1 = oracle_p1(oracle, blk, g["lattice"], (1, 4, 0, 5), g["flavour"])
+    assert abs(p1.sum() - g["sum_p1"]) <= 1e-10 * abs(g["sum_p1"])
+    per_rho = p1.sum(axis=(1, 2, 3))
+    assert np.allclose(per_rho, g["sum_p1"] / 4, rtol=1e-12)
+
+
+def test_p1_bins_and_translation(oracle):
+    """Bins beyond L_rho are zero; moving the source w by one site along rho rotates
+    that rho's bins by one and leaves the other directions' bins unchanged."""
+    dims = (8, 4, 6, 6)
+    blk = fill_block(oracle, dims, 17)
+    w = (3, 1, 5, 2)
+    p1 = oracle_p1(oracle, blk, dims, w)
+    for rho, L in enumerate(dims):
+        assert np.all(p1[rho, :, :, L:] == 0)
+    for rho in range(4):
+        w2 = list(w)
+        w2[rho] += 1
+        p2 = oracle_p1(oracle, blk, dims, w2)
+        L = dims[rho]
+        assert np.array_equal(p2[rho, :, :, :L], np.roll(p1[rho, :, :, :L], -1, axis=-1))
+        others = [r for r in range(4) if r != rho]
+        assert np.array_equal(p2[others], p1[others])
+
+
 def test_gamma_tables_match_reference(oracle, ref):
     for g in (0, 1, 2, 3, 5):
         p1, s1 = np.zeros(24, np.int32), np.zeros(24, np.int32)
