@@ -90,6 +90,8 @@ _SIGS = {
                                        C.c_void_p]),
     "hlbl_pieces_2p2_p1": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, Coord, C.c_void_p,
                                      C.c_void_p]),
+    "hlbl_pieces_2p2": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, Coord, C.POINTER(Coord), C.c_int,
+                                  C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
     "hlbl_4pt_contraction": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_int, Coord, Coord, Pair,
                                        Coord, Geom, Geom, C.c_void_p]),
@@ -354,6 +356,15 @@ class Contractor:
         [4][4][4][Lmax] doubles, Lmax the largest global extent (synchronous)."""
         check(lib().hlbl_pieces_2p2_p1(self.handle, C.c_void_p(d_fwd), iflavor, _coord(gsw),
                                        C.c_void_p(d_p1), stream))
+
+
+    def pieces_2p2(self, d_fwd, gsw, targets, xunit_, d_p1, d_p234, iflavor=0, stream=None):
+        """All 2+2 pieces of the block at d_fwd for source gsw (Contractor::pieces_2p2,
+        src/contract.cpp:116-150): P1 into d_p1 as pieces_2p2_p1, and for each target
+        site P2_0, P2_1, P3, P4_0, P4_1 into the device array d_p234 [n][5][4][4][4]."""
+        arr = (Coord * max(1, len(targets)))(*[_coord(t) for t in targets])
+        check(lib().hlbl_pieces_2p2(self.handle, C.c_void_p(d_fwd), iflavor, _coord(gsw), arr,
+                                    len(targets), xunit_, C.c_void_p(d_p1), C.c_void_p(d_p234), stream))
 
 
 def insert_g5_gamma(d_dzu, d_dzsu, d_g_dzu, d_g_dzsu, stream=None):

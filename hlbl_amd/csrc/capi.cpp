@@ -298,6 +298,35 @@ int hlbl_pieces_2p2_p1(hlbl_contractor* c, const double* d_fwd, int iflavor, hlb
   });
 }
 
+int hlbl_pieces_2p2(hlbl_contractor* c, const double* d_fwd, int iflavor, hlbl_coord gsw,
+                    const hlbl_coord* targets, int n_targets, double xunit, double* d_p1,
+                    double* d_p234, hlbl_stream stream) {
+  return guarded([&] {
+    check_ptr(c, "contractor");
+    check_ptr(d_fwd, "d_fwd");
+    check_ptr(d_p1, "d_p1");
+    if (iflavor != 0 && iflavor != 1) throw Error("iflavor must be 0 or 1");
+    if (n_targets < 0) throw Error("n_targets must be non-negative");
+    if (n_targets > 0) {
+      check_ptr(targets, "targets");
+      check_ptr(d_p234, "d_p234");
+    }
+    if (!(xunit > 0)) throw Error("xunit must be positive");
+    const hlbl_geom& g = c->global;
+    auto inside = [&](const hlbl_coord& p) {
+      return p.t >= 0 && p.t < g.T && p.x >= 0 && p.x < g.LX && p.y >= 0 && p.y < g.LY && p.z >= 0 &&
+             p.z < g.LZ;
+    };
+    if (!inside(gsw)) throw Error("gsw must be a global site in [0, L)");
+    for (int t = 0; t < n_targets; ++t)
+      if (!inside(targets[t])) throw Error("targets must be global sites in [0, L)");
+    hipStream_t st = as_stream(stream);
+    launch_p1_2p2(c, d_fwd, iflavor, gsw, d_p1, st);
+    if (n_targets > 0) launch_p234_2p2(c, c->pi2p2.p, gsw, targets, n_targets, xunit, d_p234, st);
+    check_hip(hipStreamSynchronize(st), "pieces_2p2");
+  });
+}
+
 int hlbl_insert_g5_gamma(const double* d_dzu, const double* d_dzsu, double* d_g_dzu,
                          double* d_g_dzsu, hlbl_stream stream) {
   return guarded([&] {

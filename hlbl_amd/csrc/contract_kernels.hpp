@@ -30,6 +30,10 @@ void launch_add(double* dst, const double* src, size_t n, hipStream_t st);
 // the 2+2 piece P1 of one block (flavour f) for source w into d_p1 [4][4][4][lmax]
 void launch_p1_2p2(hlbl_contractor* c, const double* blk, int f, hlbl_coord gsw, double* d_p1,
                    hipStream_t st);
+// the 2+2 pieces P2_0, P2_1, P3, P4_0, P4_1 from Pi [V][16] for each target (global
+// sites, host array) into d_p234 [n_targets][5][4][4][4]
+void launch_p234_2p2(hlbl_contractor* c, const double* pi, hlbl_coord gsw, const hlbl_coord* targets,
+                     int n_targets, double xunit, double* d_p234, hipStream_t st);
 
 // device random fill of one block
 void launch_fill_random(double* d_out, hlbl_geom local, hlbl_coord proc, uint64_t seed,

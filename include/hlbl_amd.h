@@ -173,10 +173,22 @@ int hlbl_window_profile(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n_p
  *   d_p1[rho][mu][nu][zeta] (device, 64 * Lmax doubles, Lmax the largest global extent,
  *   overwritten) = sum of Pi_{mu nu}(z) over this process's sites with
  *   (z - gsw)_rho mod L_rho = zeta; 0 for bins it does not hold (sum over processes:
- *   the identity on one). Synchronous on stream. The pieces P2..P4, which need the QED
- *   kernel at the target geometries, are not provided. */
+ *   the identity on one). Synchronous on stream. hlbl_pieces_2p2 adds P2..P4. */
 int hlbl_pieces_2p2_p1(hlbl_contractor* c, const double* d_fwd, int iflavor, hlbl_coord gsw,
                        double* d_p1, hlbl_stream stream);
+/* All 2+2 pieces of one source block (Contractor::pieces_2p2, src/contract.cpp:116-150;
+ * kernel_p20, kernel_p21, kernel_p3, kernel_p4, kernels.cu:374-760): d_p1 as above, and
+ * for each of the n_targets global sites (host array) d_p234[t][5][rho][sigma][nu]
+ * (device) = P2_0, P2_1, P3, P4_0, P4_1 with the plain QED kernel at xunit, summed over
+ * this process's sites in a fixed order; only rho < sigma is filled (P4_1: rho != sigma).
+ * With x = z - gsw and y = target - gsw, each argument difference min-imaged on its own:
+ *   P2_0 = L(x, y)[mu][nu][lam] Pi_{mu lam},  P2_1 = L(y, x)[nu][mu][lam] Pi_{mu lam},
+ *   P3 = L(x - y, -y)[mu][lam][nu] Pi_{mu lam},  P4_0 = L(y - x, -x)[nu][lam][mu] Pi_{mu lam},
+ *   P4_1 = x_rho (P4_0 term) at [rho][sigma] and -x_sigma (P4_0 term) at [sigma][rho].
+ * Synchronous on stream. */
+int hlbl_pieces_2p2(hlbl_contractor* c, const double* d_fwd, int iflavor, hlbl_coord gsw,
+                    const hlbl_coord* targets, int n_targets, double xunit, double* d_p1,
+                    double* d_p234, hlbl_stream stream);
 
 /* ---- sliding-hypercube schedule (src/schedule.cpp:34-214) -------------------- */
 hlbl_schedule* hlbl_schedule_build(hlbl_coord h, int stride, int n_steps, hlbl_coord origin);

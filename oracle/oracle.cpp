@@ -11,8 +11,13 @@
 //   gamma insertion     src/contract.cpp:19-24, 58-77
 //   pass 2 (4-pt)       kernels/cuda_lattice.cu:277-484
 //   kernel_sum          src/contract.cpp:104-114
-//   2+2 Pi and P1 sum   kernels/hlbl_kernels/kernels.cu:241-340 (only the sum of P1,
-//                       which needs no QED kernel: a pin of the fields and gammas)
+//   2+2 Pi and P1       kernels/hlbl_kernels/kernels.cu:241-340 (no QED kernel: its
+//                       sum is the reference's golden value, a pin of the fields and
+//                       gammas; the binning pinned by its invariants in the tests)
+//   2+2 P2, P3, P4      kernels/hlbl_kernels/kernels.cu:374-760 with this build's plain
+//                       kernel: parity unpinned against the reference (its golden sums
+//                       involve KQED); only the vanishing of P2, P3 at the source is
+//                       fixed independently
 // The QED kernel L^(Lambda) is KQED's in the reference (third party, absent here).
 // This oracle restates the build's own plain kernel: the Mainz tensor structure
 // (trace tensors G^A from gamma matrices, T^A from the scalar weight functions,
@@ -839,6 +844,102 @@ void oracle_p1(const double* blk, const int* dims, int f, const int* gsw, int lm
       const int zeta = ((c[rho] - gsw[rho]) % dims[rho] + dims[rho]) % dims[rho];
       for (int k = 0; k < 16; ++k) out[(rho * 16 + k) * lmax + zeta] += pi[16 * z + k];
     }
+  }
+}
+
+// The 2+2 pieces P2_0, P2_1, P3, P4_0, P4_1 of one block for one target (kernel_p20,
+// kernel_p21, kernel_p3, kernel_p4, kernels.cu:374-760), on one process:
+// out[5][rho][sigma][nu]. With x = z - w and y = target - w (mod L), each argument
+// difference min-imaged on its own (site_map, [-L/2, L/2)) and scaled by xunit, and k
+// over (r, s) = (0,1),(0,2),(0,3),(1,2),(1,3),(2,3):
+//   P2_0[r][s][nu] += L[k](x, y)[mu][nu][lam]     Pi_{mu lam}(z)
+//   P2_1[r][s][nu] += L[k](y, x)[nu][mu][lam]     Pi_{mu lam}(z)
+//   P3  [r][s][nu] += L[k](x - y, -y)[mu][lam][nu] Pi_{mu lam}(z)
+//   P4_0[r][s][nu] += L[k](y - x, -x)[nu][lam][mu] Pi_{mu lam}(z)
+//   P4_1[r][s][nu] += xv_r (that term), P4_1[s][r][nu] -= xv_s (that term), xv the
+//   integer min-image of x.
+void oracle_p234(const double* blk, const int* dims, int f, const int* gsw, const int* target,
+                 double xunit, double* out) {
+  const int pc0[4] = {0, 0, 0, 0};
+  const Geo g = make_geo(dims, dims, pc0);
+  auto smap = [&](int v, int r) {
+    v = ((v % dims[r]) + dims[r]) % dims[r];
+    return v >= dims[r] / 2 ? v - dims[r] : v;
+  };
+  static const int kr[6] = {0, 0, 0, 1, 1, 2}, ks[6] = {1, 2, 3, 2, 3, 3};
+  int y[4], yv[4], ymv[4];
+  for (int r = 0; r < 4; ++r) {
+    y[r] = ((target[r] - gsw[r]) % dims[r] + dims[r]) % dims[r];
+    yv[r] = smap(y[r], r);
+    ymv[r] = smap(-y[r], r);
+  }
+  const int nt = omp_get_max_threads();
+  std::vector<double> part((size_t)nt * 320, 0.0);
+#pragma omp parallel num_threads(nt)
+  {
+    double* acc = part.data() + (size_t)omp_get_thread_num() * 320;
+    std::vector<double> K(384);
+#pragma omp for schedule(static)
+    for (long long z = 0; z < (long long)g.V; ++z) {
+      double pi[16];
+      pi_site(blk, g.V, f, (size_t)z, pi);
+      int c[4];
+      long long q = z;
+      for (int r = 3; r >= 0; --r) {
+        c[r] = (int)(q % dims[r]);
+        q /= dims[r];
+      }
+      int x[4], xv[4];
+      double xm[4], ym[4], a[4], b[4];
+      for (int r = 0; r < 4; ++r) {
+        x[r] = ((c[r] - gsw[r]) % dims[r] + dims[r]) % dims[r];
+        xv[r] = smap(x[r], r);
+        xm[r] = xv[r] * xunit;
+        ym[r] = yv[r] * xunit;
+      }
+      auto L = [&](int k, int i, int j, int l) { return K[((size_t)k * 4 + i) * 16 + j * 4 + l]; };
+      for (int geo = 0; geo < 4; ++geo) {
+        if (geo == 0) kernel_plain(xm, ym, K.data());
+        if (geo == 1) kernel_plain(ym, xm, K.data());
+        if (geo == 2) {
+          for (int r = 0; r < 4; ++r) {
+            a[r] = smap(x[r] - y[r], r) * xunit;
+            b[r] = ymv[r] * xunit;
+          }
+          kernel_plain(a, b, K.data());
+        }
+        if (geo == 3) {
+          for (int r = 0; r < 4; ++r) {
+            a[r] = smap(y[r] - x[r], r) * xunit;
+            b[r] = smap(-x[r], r) * xunit;
+          }
+          kernel_plain(a, b, K.data());
+        }
+        for (int k = 0; k < 6; ++k)
+          for (int nu = 0; nu < 4; ++nu) {
+            double t = 0;
+            for (int mu = 0; mu < 4; ++mu)
+              for (int lam = 0; lam < 4; ++lam) {
+                const double v = geo == 0 ? L(k, mu, nu, lam)
+                               : geo == 1 ? L(k, nu, mu, lam)
+                               : geo == 2 ? L(k, mu, lam, nu)
+                                          : L(k, nu, lam, mu);
+                t += v * pi[mu * 4 + lam];
+              }
+            const int rs = kr[k] * 16 + ks[k] * 4 + nu, sr = ks[k] * 16 + kr[k] * 4 + nu;
+            acc[geo * 64 + rs] += t;
+            if (geo == 3) {
+              acc[4 * 64 + rs] += xv[kr[k]] * t;
+              acc[4 * 64 + sr] -= xv[ks[k]] * t;
+            }
+          }
+      }
+    }
+  }
+  for (int j = 0; j < 320; ++j) {
+    double s = 0;
+    for (int t = 0; t < nt; ++t) s += part[(size_t)t * 320 + j];
+    out[j] = s;
   }
 }
 

@@ -181,6 +181,39 @@ def test_pieces_2p2_p1_vs_oracle(qed, oracle, dims, seed, gsw, f):
     con.close()
 
 
+def test_pieces_2p2_vs_oracle(qed, oracle):
+    """All 2+2 pieces (kernels.cu:241-760) against the oracle to 1e-10 of the largest entry
+    per target, on 6^4 (seed 3) with the source off the origin: P1 as the P1-only entry;
+    P2_0, P2_1, P3, P4_0, P4_1 for three targets, the source itself among them (there the
+    Lambda-subtracted kernel vanishes in P2 and P3)."""
+    dims, seed, w, xunit = (6, 6, 6, 6), 3, (1, 2, 0, 3), 0.7
+    targets = [(3, 3, 3, 3), (0, 5, 1, 2), w]
+    lat = H.Lattice(dims)
+    con = H.Contractor(lat, qed)
+    blk = dev_block(lat, seed)
+    lmax = max(dims)
+    d1 = torch.empty(64 * lmax, dtype=torch.float64, device="cuda")
+    d1b = torch.empty(64 * lmax, dtype=torch.float64, device="cuda")
+    d234 = torch.empty(len(targets) * 320, dtype=torch.float64, device="cuda")
+    con.pieces_2p2(blk.data_ptr(), w, targets, xunit, d1.data_ptr(), d234.data_ptr())
+    con.pieces_2p2_p1(blk.data_ptr(), w, d1b.data_ptr())
+    assert torch.equal(d1, d1b)
+    got = d234.cpu().numpy().reshape(len(targets), 320)
+    oracle.oracle_p234.argtypes = [ctypes.c_void_p] * 2 + [ctypes.c_int] + [ctypes.c_void_p] * 2 + \
+        [ctypes.c_double, ctypes.c_void_p]
+    hb = blk.cpu().numpy()
+    for t, tgt in enumerate(targets):
+        want = np.zeros(320)
+        oracle.oracle_p234(P(hb), P(np.array(dims, np.int32)), 0, P(np.array(w, np.int32)),
+                           P(np.array(tgt, np.int32)), xunit, P(want))
+        err = np.max(np.abs(got[t] - want)) / np.max(np.abs(want))
+        log("P234 target", tgt, "max rel err", err)
+        assert err <= 1e-10
+    scale = np.max(np.abs(got[2]))
+    assert np.max(np.abs(got[2, :192])) <= 1e-12 * scale  # P2_0, P2_1, P3 at the source
+    con.close()
+
+
 def test_window_forms(qed):
     """Form 0 (FP64 per pair) equals kernel_sum; form 1 (FP32 products, FP64 Kahan
     sums) agrees with FP64 to 1e-4 relative per pair sum."""
