@@ -490,7 +490,13 @@ class RunInfo(C.Structure):
     _fields_ = [("lattice", Coord), ("procs", Coord), ("nconf", C.c_int),
                 ("chains", C.POINTER(ChainInfo)), ("n_chains", C.c_int), ("kernel", C.c_char_p),
                 ("muon_mass_mev", C.c_double), ("lattice_spacing_fm", C.c_double),
-                ("xunit", C.c_double)]
+                ("xunit", C.c_double), ("with_2p2", C.c_int)]
+
+
+class Source2p2(C.Structure):
+    _fields_ = [("w", Coord), ("chain", C.c_int), ("window", C.c_int),
+                ("p1", C.POINTER(C.c_double)), ("targets", C.POINTER(Coord)), ("n_targets", C.c_int),
+                ("p234", C.POINTER(C.c_double))]
 
 
 def lib_h5():
@@ -505,6 +511,8 @@ def lib_h5():
         L.hlbl_result_open.argtypes = [C.c_char_p, C.POINTER(RunInfo)]
         L.hlbl_result_append.restype = C.c_int
         L.hlbl_result_append.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int]
+        L.hlbl_result_append_2p2.restype = C.c_int
+        L.hlbl_result_append_2p2.argtypes = [C.c_void_p, C.POINTER(Source2p2), C.c_int]
         L.hlbl_result_rows.restype = C.c_longlong
         L.hlbl_result_rows.argtypes = [C.c_void_p]
         L.hlbl_result_close.restype = C.c_int
@@ -524,12 +532,14 @@ class ResultFile:
     so the default says so."""
 
     def __init__(self, path, lattice_dims, chains, xunit_, muon_mass_mev=0.0, lattice_spacing_fm=0.0,
-                 procs=(1, 1, 1, 1), nconf=-1, kernel="LLambda0.4 (stand-in weight functions)"):
+                 procs=(1, 1, 1, 1), nconf=-1, kernel="LLambda0.4 (stand-in weight functions)",
+                 with_2p2=False):
         L = lib_h5()
         arr = (ChainInfo * len(chains))(*[ChainInfo(_coord(s), _coord(d), int(st), int(n))
                                           for s, d, st, n in chains])
         info = RunInfo(_coord(lattice_dims), _coord(procs), int(nconf), arr, len(chains),
-                       kernel.encode(), float(muon_mass_mev), float(lattice_spacing_fm), float(xunit_))
+                       kernel.encode(), float(muon_mass_mev), float(lattice_spacing_fm), float(xunit_),
+                       1 if with_2p2 else 0)
         h = L.hlbl_result_open(os.fsencode(path), C.byref(info))
         if not h:
             raise HlblError(L.hlbl_h5_last_error().decode())
@@ -543,6 +553,21 @@ class ResultFile:
             buf[16 * i:16 * i + 16] = [*r["x"], *r["y"], *r["yv"], r["window"], r.get("chain", 0),
                                        *r["kernel_sum"]]
         if lib_h5().hlbl_result_append(self.handle, buf, len(rows)) != 0:
+            raise HlblError(lib_h5().hlbl_h5_last_error().decode())
+
+    def append_2p2(self, sources):
+        """ResultWriter::append_2p2 (src/output.cpp:249-297). sources: dicts {w, chain,
+        window, p1 (64 * Lmax doubles), targets [(t, x, y, z), ...], p234 (n_targets * 320
+        doubles)} as Contractor.pieces_2p2 gives them (copied to the host)."""
+        keep, rows = [], (Source2p2 * max(1, len(sources)))()
+        for i, s in enumerate(sources):
+            p1 = (C.c_double * len(s["p1"]))(*[float(v) for v in s["p1"]])
+            tg = (Coord * max(1, len(s["targets"])))(*[_coord(t) for t in s["targets"]])
+            p234 = (C.c_double * max(1, len(s["p234"])))(*[float(v) for v in s["p234"]])
+            keep += [p1, tg, p234]
+            rows[i] = Source2p2(_coord(s["w"]), int(s.get("chain", 0)), int(s.get("window", 0)), p1, tg,
+                                len(s["targets"]), p234)
+        if lib_h5().hlbl_result_append_2p2(self.handle, rows, len(sources)) != 0:
             raise HlblError(lib_h5().hlbl_h5_last_error().decode())
 
     def rows(self):

@@ -177,9 +177,13 @@ int guarded(F&& f) {
 }  // namespace
 
 struct hlbl_result_file {
-  Id file, group;
+  Id file, group, group_2p2;
   RowSet x, y, yv, chain, window, kernel_sum;
-  long long rows = 0;
+  RowSet w, chain_2p2, window_2p2, p1, pair_w, pair_y, pair_yv, p234;
+  long long rows = 0, rows_2p2 = 0, pairs_2p2 = 0;
+  int dims[4] = {0, 0, 0, 0};
+  int lmax = 0;
+  bool with_2p2 = false;
 };
 
 extern "C" {
@@ -200,6 +204,35 @@ hlbl_result_file* hlbl_result_open(const char* path, const hlbl_run_info* info) 
     f->chain = RowSet(g, "chain", H5T_NATIVE_INT, {});
     f->window = RowSet(g, "window", H5T_NATIVE_INT, {});
     f->kernel_sum = RowSet(g, "kernel_sum", H5T_NATIVE_DOUBLE, {2});
+    coord4(info->lattice, f->dims);
+    f->lmax = std::max(std::max(f->dims[0], f->dims[1]), std::max(f->dims[2], f->dims[3]));
+    if (info->with_2p2) {
+      f->with_2p2 = true;
+      f->group_2p2 = Id(H5Gcreate2(*f->file, "2p2", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Gclose,
+                        "create group 2p2");
+      const hid_t g2 = *f->group_2p2;
+      const hsize_t lm = (hsize_t)f->lmax;
+      f->w = RowSet(g2, "w", H5T_NATIVE_INT, {4});
+      f->chain_2p2 = RowSet(g2, "chain", H5T_NATIVE_INT, {});
+      f->window_2p2 = RowSet(g2, "window", H5T_NATIVE_INT, {});
+      f->p1 = RowSet(g2, "P1", H5T_NATIVE_DOUBLE, {4, 4, 4, lm});
+      f->pair_w = RowSet(g2, "pair_w", H5T_NATIVE_INT, {4});
+      f->pair_y = RowSet(g2, "pair_y", H5T_NATIVE_INT, {4});
+      f->pair_yv = RowSet(g2, "pair_yv", H5T_NATIVE_INT, {4});
+      f->p234 = RowSet(g2, "P234", H5T_NATIVE_DOUBLE, {5, 4, 4, 4});
+      attr_text(g2, "rows",
+                "w, chain, window, P1: one row per 2+2 source, at its first solve; "
+                "pair_w, pair_y, pair_yv, P234: one row per (source, target)");
+      attr_text(g2, "P1_index",
+                "P1[row][rho][sigma][nu][zeta]: Pi_{sigma nu}(z) summed over the sites z with "
+                "(z - w)_rho mod L_rho = zeta; 0 for zeta >= L_rho");
+      attr_text(g2, "P234_index",
+                "P234[row][geometry][rho][sigma][nu], geometry P2_0, P2_1, P3, P4_0, P4_1; "
+                "only rho < sigma is filled (P4_1: rho != sigma)");
+      attr_text(g2, "flavour",
+                "operator 0 only (iflavor 0 of compute_2p2_gpu, as cvc): the two-point function "
+                "is the same for both flavours");
+    }
 
     // chains[C][10]: start, direction, stride, steps
     const hsize_t nc = (hsize_t)info->n_chains;
@@ -277,6 +310,59 @@ int hlbl_result_append(hlbl_result_file* f, const double* rows, int n) {
     f->window.write(window.data(), first, m);
     f->kernel_sum.write(ks.data(), first, m);
     f->rows += n;
+    ok(H5Fflush(*f->file, H5F_SCOPE_GLOBAL), "flush");
+  });
+}
+
+int hlbl_result_append_2p2(hlbl_result_file* f, const hlbl_source_2p2* rows, int n) {
+  return guarded([&] {
+    if (!f) throw H5Error("no result file");
+    if (!f->with_2p2) throw H5Error("the file was opened without the 2p2 group (with_2p2)");
+    if (n < 0) throw H5Error("negative row count");
+    if (n == 0) return;
+    if (!rows) throw H5Error("rows is NULL");
+    const size_t p1_len = (size_t)64 * f->lmax;
+    std::vector<int> w, chain, window, pw, py, pyv;
+    std::vector<double> p1, p234;
+    for (int i = 0; i < n; ++i) {
+      const hlbl_source_2p2& r = rows[i];
+      if (!r.p1 || r.n_targets < 0 || (r.n_targets > 0 && (!r.targets || !r.p234)))
+        throw H5Error("a 2+2 row lacks its arrays");
+      int wv[4];
+      coord4(r.w, wv);
+      w.insert(w.end(), wv, wv + 4);
+      chain.push_back(r.chain);
+      window.push_back(r.window);
+      p1.insert(p1.end(), r.p1, r.p1 + p1_len);
+      for (int t = 0; t < r.n_targets; ++t) {
+        int yv[4];
+        coord4(r.targets[t], yv);
+        pw.insert(pw.end(), wv, wv + 4);
+        py.insert(py.end(), yv, yv + 4);
+        for (int k = 0; k < 4; ++k) {
+          // minimum image of y - w in [-L/2, L/2)
+          const int L = f->dims[k];
+          int d = ((yv[k] - wv[k]) % L + L) % L;
+          pyv.push_back(d >= L / 2 ? d - L : d);
+        }
+        p234.insert(p234.end(), r.p234 + (size_t)t * 320, r.p234 + (size_t)(t + 1) * 320);
+      }
+    }
+    const hsize_t first = (hsize_t)f->rows_2p2, m = (hsize_t)n;
+    f->w.write(w.data(), first, m);
+    f->chain_2p2.write(chain.data(), first, m);
+    f->window_2p2.write(window.data(), first, m);
+    f->p1.write(p1.data(), first, m);
+    const hsize_t npairs = (hsize_t)(pw.size() / 4);
+    if (npairs > 0) {
+      const hsize_t fp = (hsize_t)f->pairs_2p2;
+      f->pair_w.write(pw.data(), fp, npairs);
+      f->pair_y.write(py.data(), fp, npairs);
+      f->pair_yv.write(pyv.data(), fp, npairs);
+      f->p234.write(p234.data(), fp, npairs);
+    }
+    f->rows_2p2 += n;
+    f->pairs_2p2 += (long long)npairs;
     ok(H5Fflush(*f->file, H5F_SCOPE_GLOBAL), "flush");
   });
 }

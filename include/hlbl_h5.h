@@ -8,11 +8,13 @@
  *   chains[C][10]                           int     (start, direction, stride, steps)
  *   attributes lattice, procs, nconf, muon_mass_MeV, lattice_spacing_fm, xunit, kernel,
  *   flavour_order, coordinate_ranges, chains_columns; with one chain also direction,
- *   origin, stride.
+ *   origin, stride;
+ *   with with_2p2 the group 2p2: w[M][4], chain[M], window[M] int, P1[M][4][4][4][Lmax]
+ *   double (one row per source), pair_w, pair_y, pair_yv [R][4] int, P234[R][5][4][4][4]
+ *   double (one row per (source, target)), attributes rows, P1_index, P234_index, flavour.
  *
  * Row datasets are unlimited along N and chunked; every append is flushed, so a job
- * that is killed keeps the windows it finished. The reference's optional 2p2 group
- * (2+2 pieces) is not written: this library computes the connected kernel sum only.
+ * that is killed keeps the windows it finished.
  * Separate library (libhlbl_h5.so, host code, links HDF5) so that libhlbl_amd.so does
  * not depend on HDF5. Errors: NULL / nonzero return, message in hlbl_h5_last_error. */
 #ifndef HLBL_H5_H
@@ -30,7 +32,7 @@ typedef struct {
   int stride, steps;
 } hlbl_chain_info;
 
-/* RunInfo (include/hlbl/output.hpp:37-47) without with_2p2; kernel NULL: "LLambda0.4" */
+/* RunInfo (include/hlbl/output.hpp:37-47); kernel NULL: "LLambda0.4" */
 typedef struct {
   hlbl_coord lattice, procs;
   int nconf;
@@ -38,7 +40,20 @@ typedef struct {
   int n_chains;
   const char* kernel;
   double muon_mass_mev, lattice_spacing_fm, xunit;
+  int with_2p2;  /* write the 2p2 group */
 } hlbl_run_info;
+
+/* Result2p2 (include/hlbl/output.hpp:25-35): one 2+2 source and its targets, host
+ * arrays as hlbl_pieces_2p2 gives them after a copy: p1 [4][4][4][Lmax],
+ * p234 [n_targets][5][4][4][4]; pair_yv is computed (minimum image of target - w) */
+typedef struct {
+  hlbl_coord w;
+  int chain, window;
+  const double* p1;
+  const hlbl_coord* targets;
+  int n_targets;
+  const double* p234;
+} hlbl_source_2p2;
 
 typedef struct hlbl_result_file hlbl_result_file;
 
@@ -49,6 +64,9 @@ hlbl_result_file* hlbl_result_open(const char* path, const hlbl_run_info* info);
 int hlbl_result_append(hlbl_result_file* f, const double* rows, int n);
 /* the same with the argument types of hlbl_window_rows_fn, for hlbl_run_chain_each_window */
 int hlbl_result_append_rows(void* f, const double* rows, int n);
+/* ResultWriter::append_2p2 (src/output.cpp:249-297): n sources, then a flush; needs
+ * with_2p2 */
+int hlbl_result_append_2p2(hlbl_result_file* f, const hlbl_source_2p2* rows, int n);
 long long hlbl_result_rows(const hlbl_result_file* f);
 int hlbl_result_close(hlbl_result_file* f);
 const char* hlbl_h5_last_error(void);

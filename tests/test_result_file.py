@@ -194,3 +194,54 @@ def test_loads_before_the_main_library():
             "H.lib_h5(); H.lib(); print('ok')\n")
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr[-2000:]
+
+
+def test_2p2_group(tmp_path):
+    """The 2p2 group (src/output.cpp:146-172, 249-297) read back with h5dump: shapes,
+    values bitwise, pair rows per (source, target), pair_yv the minimum image of y - w
+    in [-L/2, L/2), the attribute texts; without with_2p2 there is no group and
+    appending 2+2 rows fails."""
+    g2 = GOLD["group_2p2"]
+    dims, lmax = (8, 4, 4, 4), 8
+    rng = np.random.default_rng(3)
+    srcs = [{"w": (1, 2, 3, 0), "chain": 0, "window": 1, "p1": rng.standard_normal(64 * lmax),
+             "targets": [(7, 0, 1, 3), (5, 2, 3, 1)], "p234": rng.standard_normal(2 * 320)},
+            {"w": (0, 0, 0, 0), "chain": 1, "window": 0, "p1": rng.standard_normal(64 * lmax),
+             "targets": [], "p234": []},
+            {"w": (2, 1, 0, 3), "chain": 1, "window": 2, "p1": rng.standard_normal(64 * lmax),
+             "targets": [(2, 3, 2, 1)], "p234": rng.standard_normal(320)}]
+    path = str(tmp_path / "r.h5")
+    f = H.ResultFile(path, dims, [CHAIN, ((0, 0, 0, 0), (1, 1, 1, 1), 1, 2)], 0.5, with_2p2=True)
+    f.append(rows_of(2, 0))
+    f.append_2p2(srcs[:1])
+    f.append_2p2(srcs[1:])
+    f.close()
+    shape, _, w, _ = dataset(path, "/2p2/w", tmp_path)
+    assert w.tolist() == [list(s["w"]) for s in srcs]
+    _, _, ch, _ = dataset(path, "/2p2/chain", tmp_path)
+    _, _, win, _ = dataset(path, "/2p2/window", tmp_path)
+    assert ch.tolist() == [0, 1, 1] and win.tolist() == [1, 0, 2]
+    shape, maxshape, p1, _ = dataset(path, "/2p2/P1", tmp_path)
+    assert shape == (3, 4, 4, 4, lmax) and maxshape[0] == "H5S_UNLIMITED"
+    assert np.array_equal(p1, np.stack([s["p1"] for s in srcs]).reshape(3, 4, 4, 4, lmax))
+    pairs = [(s["w"], t, s["p234"][320 * i:320 * (i + 1)]) for s in srcs for i, t in enumerate(s["targets"])]
+    shape, _, p234, _ = dataset(path, "/2p2/P234", tmp_path)
+    assert shape == (3, 5, 4, 4, 4)
+    assert np.array_equal(p234, np.stack([p for _, _, p in pairs]).reshape(3, 5, 4, 4, 4))
+    _, _, pw, _ = dataset(path, "/2p2/pair_w", tmp_path)
+    _, _, py, _ = dataset(path, "/2p2/pair_y", tmp_path)
+    _, _, pyv, _ = dataset(path, "/2p2/pair_yv", tmp_path)
+    assert pw.tolist() == [list(a) for a, _, _ in pairs] and py.tolist() == [list(b) for _, b, _ in pairs]
+    L = np.array(dims)
+    want_yv = (np.array([b for _, b, _ in pairs]) - np.array([a for a, _, _ in pairs]) + L // 2) % L - L // 2
+    assert np.array_equal(pyv, want_yv)
+    for name, text in g2["attribute_texts"].items():
+        assert attribute(path, "2p2/" + name) == text, name
+    for name in g2["datasets"]:
+        assert f'DATASET "{name.split("/")[-1]}"' in header(path)
+    plain = str(tmp_path / "plain.h5")
+    f = H.ResultFile(plain, dims, [CHAIN], 0.5)
+    with pytest.raises(H.HlblError, match="with_2p2"):
+        f.append_2p2(srcs[:1])
+    f.close()
+    assert 'GROUP "2p2"' not in header(plain)
