@@ -476,6 +476,99 @@ def run_chains(contractor, schedules, seed, xunit_, residency=0, form=0, batch=0
     return rows, total
 
 
+def plan_2p2(schedules, sources, lattice_dims):
+    """plan_2p2 (src/schedule.cpp:284-317): which solves are followed by the 2+2 pieces,
+    the first solve in run order (chains in input order, windows, solve order) of each
+    2+2 source, sites compared wrapped onto the lattice.
+
+    Returns {"at": {source index: (chain, window)}, "not_sources": [(wrapped site, chain,
+    window), ...] (first solves of sites that are no 2+2 source), "unsolved": [source
+    indices no chain solves]}."""
+    wrap = lambda v: tuple(int(a) % int(L) for a, L in zip(v, lattice_dims))
+    index = {}
+    for i, src in enumerate(sources):
+        index.setdefault(wrap(src), i)
+    at, not_sources, solved = {}, [], set()
+    for c, sch in enumerate(schedules):
+        for window, pos, _parts in sch.sources():
+            p = wrap(pos)
+            if p in solved:
+                continue
+            solved.add(p)
+            if p in index:
+                at[index[p]] = (c, window)
+            else:
+                not_sources.append((p, c, window))
+    unsolved = [i for i, src in enumerate(sources) if wrap(src) not in solved]
+    return {"at": at, "not_sources": not_sources, "unsolved": unsolved}
+
+
+def report_2p2(plan, sources, max_sites=10):
+    """report_2p2 (src/schedule.cpp:315-347): the warning lines of a 2+2 plan, solved
+    sites that are no 2+2 source, then 2+2 sources no chain solves, at most max_sites
+    sites listed each and the rest counted; no lines without 2+2 sources."""
+    lines = []
+    if not sources:
+        return lines
+    ns = plan["not_sources"]
+    if ns:
+        lines.append(f"WARNING: {len(ns)} solved source positions are not 2+2 sources "
+                     "(no 2+2 computed for them)")
+        lines += [f"  ({p[0]},{p[1]},{p[2]},{p[3]}) in window {w} of chain {c}" for p, c, w in ns[:max_sites]]
+        if len(ns) > max_sites:
+            lines.append(f"  ... and {len(ns) - max_sites} more")
+    un = plan["unsolved"]
+    if un:
+        lines.append(f"WARNING: {len(un)} 2+2 sources are not solved by any chain "
+                     "(no 2+2 computed for them)")
+        lines += ["  ({},{},{},{})".format(*sources[i]) for i in un[:max_sites]]
+        if len(un) > max_sites:
+            lines.append(f"  ... and {len(un) - max_sites} more")
+    return lines
+
+
+def run_2p2(contractor, lattice, seed, schedules, sources_2p2, xunit_, writer=None, iflavor=0):
+    """The 2+2 step of a run (src/driver.cpp:58-95 with plan_2p2): at the first solve of
+    each 2+2 source (site, targets), P1 and the P234 rows of its targets from that
+    source's block, the RandomProvider field of the run's seed as the chain solves it.
+    Sources are visited in run order (chains, windows, solve order); each row carries
+    the chain and window of its solve, and with a ResultFile opened with with_2p2 each
+    window's rows are appended together (the reference appends at the window's end).
+    Returns (rows, plan)."""
+    import torch
+    dims = tuple(lattice.global_)
+    wrap = lambda v: tuple(int(a) % int(L) for a, L in zip(v, dims))
+    plan = plan_2p2(schedules, [s for s, _t in sources_2p2], dims)
+    by_site = {wrap(sources_2p2[i][0]): i for i in plan["at"]}
+    lmax = max(dims)
+    prov = RandomProvider(lattice, seed)
+    blk = torch.empty(block_length(lattice), dtype=torch.float64, device="cuda")
+    rows, pending, done = [], [], set()
+    for c, sch in enumerate(schedules):
+        for window, pos, _parts in sch.sources():
+            w = wrap(pos)
+            i = by_site.get(w)
+            if i is None or i in done or plan["at"][i] != (c, window):
+                continue
+            done.add(i)
+            targets = [wrap(t) for t in sources_2p2[i][1]]
+            prov.compute(w, blk.data_ptr())
+            d1 = torch.empty(64 * lmax, dtype=torch.float64, device="cuda")
+            d234 = torch.empty(max(1, len(targets)) * 320, dtype=torch.float64, device="cuda")
+            contractor.pieces_2p2(blk.data_ptr(), w, targets, xunit_, d1.data_ptr(), d234.data_ptr(),
+                                  iflavor)
+            row = {"w": w, "chain": c, "window": window, "p1": d1.cpu().numpy(), "targets": targets,
+                   "p234": d234.cpu().numpy()[:320 * len(targets)]}
+            if writer is not None and pending and (pending[-1]["chain"], pending[-1]["window"]) != (c, window):
+                writer.append_2p2(pending)
+                pending = []
+            pending.append(row)
+            rows.append(row)
+    if writer is not None and pending:
+        writer.append_2p2(pending)
+    return rows, plan
+
+
 # ---- result file (include/hlbl_h5.h; the reference's ResultWriter, src/output.cpp) ----
 
 H5_LIB_PATH = os.path.join(_HERE, "libhlbl_h5.so")

@@ -418,3 +418,41 @@ def test_config_32x64_two_chains_source_batches(qed):
     log("32^3x64 form 1, sampled pairs: max rel err", max(errs), "median", np.median(errs))
     assert len(errs) == 30 and max(errs) < 1e-3
     con.close()
+
+
+@pytest.mark.skipif(not os.path.exists(H.H5_LIB_PATH), reason="libhlbl_h5.so absent (no HDF5 at build time)")
+def test_run_2p2_against_direct_pieces(qed, tmp_path):
+    """The 2+2 step of a run (src/driver.cpp:58-95 with plan_2p2) on 6^4, two chains:
+    one row per planned source in run order, each bitwise equal to pieces_2p2 on that
+    source's RandomProvider block; the unsolved source gets none; the 2p2 group of the
+    file holds the same rows."""
+    from test_result_file import dataset
+    dims, xunit, seed = (6, 6, 6, 6), 0.8, 5
+    lat = H.Lattice(dims)
+    con = H.Contractor(lat, qed)
+    schedules = [H.Schedule((1, 1, 1, 1), 1, 1), H.Schedule((1, 1, 1, 1), 1, 1, (1, 1, 1, 1))]
+    sources = [((5, 5, 5, 5), [(0, 0, 0, 0), (2, 3, 4, 5)]), ((2, 2, 2, 2), []),
+               ((1, 1, 1, 1), [(1, 1, 1, 1)]), ((3, 0, 0, 0), [(1, 2, 3, 4)])]
+    path = str(tmp_path / "r2p2.h5")
+    info = [((0, 0, 0, 0), (1, 1, 1, 1), 1, 1), ((1, 1, 1, 1), (1, 1, 1, 1), 1, 1)]
+    with H.ResultFile(path, dims, info, xunit, with_2p2=True) as f:
+        rows, plan = H.run_2p2(con, lat, seed, schedules, sources, xunit, writer=f)
+    assert plan["unsolved"] == [3]
+    assert [r["w"] for r in rows] == [(5, 5, 5, 5), (1, 1, 1, 1), (2, 2, 2, 2)]  # run order
+    assert [(r["chain"], r["window"]) for r in rows] == [plan["at"][i] for i in (0, 2, 1)]
+    lmax = max(dims)
+    for r, (site, targets) in zip(rows, [sources[0], sources[2], sources[1]]):
+        blk = torch.empty(H.block_length(lat), dtype=torch.float64, device="cuda")
+        H.fill_random_block_device(lat, H.source_seed(seed, site), blk.data_ptr())
+        d1 = torch.empty(64 * lmax, dtype=torch.float64, device="cuda")
+        d234 = torch.empty(max(1, len(targets)) * 320, dtype=torch.float64, device="cuda")
+        con.pieces_2p2(blk.data_ptr(), site, targets, xunit, d1.data_ptr(), d234.data_ptr())
+        assert np.array_equal(r["p1"], d1.cpu().numpy())
+        assert np.array_equal(r["p234"], d234.cpu().numpy()[:320 * len(targets)])
+    _, _, w, _ = dataset(path, "/2p2/w", tmp_path)
+    assert w.tolist() == [list(r["w"]) for r in rows]
+    _, _, p1, _ = dataset(path, "/2p2/P1", tmp_path)
+    assert np.array_equal(p1.reshape(len(rows), -1), np.stack([r["p1"] for r in rows]))
+    _, _, p234, _ = dataset(path, "/2p2/P234", tmp_path)
+    assert np.array_equal(p234.reshape(3, -1), np.concatenate([r["p234"] for r in rows]).reshape(3, -1))
+    con.close()

@@ -129,3 +129,69 @@ def test_bad_inputs():
         H.Schedule((1, 2, 1, 1), 1, 1)
     with pytest.raises(H.HlblError):
         H.Schedule((1, 1, 1, 1), 0, 1)
+
+
+def test_plan_2p2_known_answers():
+    """plan_2p2 on the cases of sanity_check/test_schedule.cpp:134-187: two chains from
+    (0,0,0,0) and (1,1,1,1), h = (1,1,1,1), one step each, on 6^4: (1,1,1,1) at chain 0
+    window 1, (5,5,5,5) at chain 0 as (-1,-1,-1,-1), (2,2,2,2) at chain 1, (3,0,0,0)
+    unsolved; every other solved site listed once; the report's 14 lines. A chain that
+    wraps onto its own sites gets one 2+2 per site."""
+    L = (6, 6, 6, 6)
+    chains = [H.Schedule((1, 1, 1, 1), 1, 1, (0, 0, 0, 0)), H.Schedule((1, 1, 1, 1), 1, 1, (1, 1, 1, 1))]
+    sources = [(1, 1, 1, 1), (2, 2, 2, 2), (5, 5, 5, 5), (3, 0, 0, 0)]
+    plan = H.plan_2p2(chains, sources, L)
+    assert plan["at"][0] == (0, 1) and plan["at"][2][0] == 0 and plan["at"][1][0] == 1
+    assert sorted(plan["at"]) == [0, 1, 2]
+    assert plan["unsolved"] == [3]
+    # (-1,-1,-1,-1) is solved by chain 0 (unwrapped), so (5,5,5,5) is found there
+    assert any(tuple(p) == (-1, -1, -1, -1) and w == plan["at"][2][1]
+               for w, p, _ in chains[0].sources())
+    sites = {tuple(a % 6 for a in p) for s in chains for _w, p, _ in s.sources()}
+    listed = [p for p, _c, _w in plan["not_sources"]]
+    assert len(listed) == len(set(listed)) == len(sites) - 3
+    assert not {(1, 1, 1, 1), (2, 2, 2, 2), (5, 5, 5, 5)} & set(listed)
+    lines = H.report_2p2(plan, sources)
+    n = len(plan["not_sources"])
+    assert len(lines) == 14
+    assert lines[0] == (f"WARNING: {n} solved source positions are not 2+2 sources "
+                        "(no 2+2 computed for them)")
+    assert lines[11] == f"  ... and {n - 10} more"
+    assert lines[12] == "WARNING: 1 2+2 sources are not solved by any chain (no 2+2 computed for them)"
+    assert lines[13] == "  (3,0,0,0)"
+    assert H.report_2p2(plan, []) == []
+    wrapped = H.plan_2p2([H.Schedule((1, 1, 1, 1), 1, 6)], [(0, 0, 0, 0)], (4, 4, 4, 4))
+    assert wrapped["at"] == {0: (0, 0)} and wrapped["unsolved"] == []
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_plan_2p2_first_solve(seed):
+    """Every 2+2 source is placed at the earliest solve of its wrapped site in run order
+    (brute force over the flattened solve list), duplicates keep the first index, and
+    not_sources holds exactly the first solves of the other sites, in run order."""
+    rng = np.random.default_rng(seed)
+    L = (4, 6, 4, 6)
+    chains = []
+    for _ in range(3):
+        h = tuple(int(v) for v in rng.choice([-1, 1], 4))
+        origin = tuple(int(v) for v in rng.integers(-3, 4, 4))
+        chains.append(H.Schedule(h, int(rng.integers(1, 3)), int(rng.integers(1, 4)), origin))
+    solves = [(tuple(a % l for a, l in zip(p, L)), c, w) for c, s in enumerate(chains) for w, p, _ in s.sources()]
+    sources = [tuple(int(v) for v in rng.integers(-6, 7, 4)) for _ in range(12)]
+    sources.append(sources[0])
+    plan = H.plan_2p2(chains, sources, L)
+    first = {}
+    for k, (p, c, w) in enumerate(solves):
+        first.setdefault(p, (k, c, w))
+    for i, src in enumerate(sources):
+        p = tuple(a % l for a, l in zip(src, L))
+        dup = any(tuple(a % l for a, l in zip(sources[j], L)) == p for j in range(i))
+        if p not in first:
+            assert i in plan["unsolved"] and i not in plan["at"]
+        elif dup:
+            assert i not in plan["at"]
+        else:
+            assert plan["at"][i] == first[p][1:]
+    src_sites = {tuple(a % l for a, l in zip(s, L)) for s in sources}
+    want = sorted((k, p, c, w) for p, (k, c, w) in first.items() if p not in src_sites)
+    assert plan["not_sources"] == [(p, c, w) for _k, p, c, w in want]
