@@ -92,6 +92,49 @@ def time_form(con, descs, form, warmup, steps):
     return (time.time() - t0) / steps, res
 
 
+def bench_2p2(H, torch, lat, con, args):
+    """The 2+2 pieces of one source (Contractor::pieces_2p2, src/contract.cpp:116-150):
+    P1 alone (pi_2p2 + p1_2p2), then P1 and four targets (p234_sites + p234_reduce per
+    target). Both calls are synchronous, so the host clock brackets them. pi_2p2's
+    algorithmic bytes per site: two 12 x 24 double spinor sets read (4608 B) and 16
+    doubles of Pi written (128 B)."""
+    V = 1
+    for d in DIMS:
+        V *= d
+    blk = torch.empty(H.block_length(lat), dtype=torch.float64, device="cuda")
+    H.fill_random_block_device(lat, 1, blk.data_ptr())
+    w = (3, 5, 7, 11)
+    targets = [(0, 0, 0, 0), (24, 12, 12, 12), (5, 1, 2, 3), (47, 23, 0, 9)]
+    lmax = max(DIMS)
+    d1 = torch.empty(64 * lmax, dtype=torch.float64, device="cuda")
+    d234 = torch.empty(len(targets) * 320, dtype=torch.float64, device="cuda")
+
+    def run(ts, k):
+        for _ in range(args.warmup):
+            con.pieces_2p2(blk.data_ptr(), w, ts, XUNIT, d1.data_ptr(), d234.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            con.pieces_2p2(blk.data_ptr(), w, ts, XUNIT, d1.data_ptr(), d234.data_ptr())
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k
+
+    t_p1 = run([], max(args.steps, 10))
+    log(f"P1 alone {t_p1 * 1e3:.3f} ms per source")
+    t_all = run(targets, args.steps)
+    per_target = (t_all - t_p1) / len(targets)
+    log(f"P1 + {len(targets)} targets {t_all * 1e3:.3f} ms; {per_target * 1e3:.3f} ms per target, "
+        f"{V / per_target:.3e} sites/s (four kernel evaluations each)")
+    line = {"metric": "2+2 sources per second (P1 + 4 targets)", "value": 1.0 / t_all, "unit": "sources/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_all * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (device RandomProvider block, seed 1)",
+            "config": {"workload": "pieces_2p2 of one source, %dx%dx%dx%d" % DIMS, "targets": len(targets)},
+            "p1_ms": t_p1 * 1e3, "ms_per_target": per_target * 1e3,
+            "pi_2p2_bytes_per_launch": V * (4608 + 128)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +145,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 1 GiB device buffer three times (reads and writes 1 GiB "
                          "each): the reference point for the HBM byte counters, then exit")
+    ap.add_argument("--part", choices=["kernel_sum", "2p2"], default="kernel_sum",
+                    help="2p2: time the 2+2 pieces of one source (P1 and four targets) on the same "
+                         "lattice and print their own line instead of the kernel-sum line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
@@ -124,6 +170,9 @@ def main():
     lat = H.Lattice(DIMS)
     qed = H.QedKernel(0)
     con = H.Contractor(lat, qed)
+    if args.part == "2p2":
+        bench_2p2(H, torch, lat, con, args)
+        return
 
     # surrogate: trained on the GPU by the library's own kernels (not timed), over
     # the distances this lattice reaches (outside: the table)

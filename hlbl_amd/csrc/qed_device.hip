@@ -170,7 +170,9 @@ __global__ void qed_channels_points(const double* __restrict__ table, const doub
 void launch_p234_2p2(hlbl_contractor* c, const double* pi, hlbl_coord gsw, const hlbl_coord* targets,
                      int n_targets, double xunit, double* d_p234, hipStream_t st) {
   const size_t V = c->volume;
-  constexpr size_t kChunk = 16384;  // sites per launch: 4 x 384 doubles of scratch each
+  // sites per launch, one thread each with 4 x 384 doubles (12 KiB) of scratch: 2^19
+  // sites are 8192 waves, 8 per SIMD, for 6 GiB of scratch
+  constexpr size_t kChunk = size_t(1) << 19;
   double *scratch = nullptr, *contrib = nullptr;
   check_hip(hipMallocAsync((void**)&scratch, std::min(V, kChunk) * 4 * qed::kNL * sizeof(double), st),
             "hipMallocAsync");
