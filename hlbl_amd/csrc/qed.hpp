@@ -242,9 +242,9 @@ HLBL_HD void subtraction_matrix(const R x[4], R d[16]) {
 }
 
 // L(x,y)[o] = L0xy[o] - sum_al Dx[mu][al] L0(0,y)[k al nu lam] - sum_al Dy[nu][al] L0(x,0)[k mu al lam]
-template <class R>
-HLBL_HD R lambda_subtract(int o, const R* l0xy, const R* l00y, const R* l0x0, const R dxm[16],
-                          const R dyn[16]) {
+// (P: a pointer, or any type indexed by [] as one, such as a lane-strided view)
+template <class R, class P>
+HLBL_HD R lambda_subtract(int o, P l0xy, P l00y, P l0x0, const R dxm[16], const R dyn[16]) {
   const int lam = o & 3, nu = (o >> 2) & 3, mu = (o >> 4) & 3, k = o >> 6;
   R r = l0xy[o];
   for (int al = 0; al < 4; ++al) {

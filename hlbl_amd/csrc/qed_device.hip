@@ -19,7 +19,18 @@ void upload_terms_qed(const qed::TermTable& t) {
 }
 
 // L0(u, v) of one argument pair, FP64, into l0[384]
-__device__ void l0_point(const double* table, const double u[4], const double v[4], double* l0) {
+// Element j of a lane's scratch at p[j * s]: with p = base + lane and s the number of
+// lanes, a wave's accesses to one element are contiguous.
+struct LaneView {
+  double* p;
+  size_t s;
+  __device__ double& operator[](size_t j) const { return p[j * s]; }
+  __device__ LaneView operator+(size_t j) const { return {p + j * s, s}; }
+};
+
+// P: double* or LaneView
+template <class P>
+__device__ void l0_point(const double* table, const double u[4], const double v[4], P l0) {
   double a, c, b, ch[qed::kNCh], t[qed::kNT];
   qed::acb(u, v, a, c, b);
   const qed::Cell<double> cell = qed::locate(a, c, b);
@@ -31,17 +42,17 @@ __device__ void l0_point(const double* table, const double u[4], const double v[
 
 // L^(Lambda)(x, y) into out[384] (plain form, FP64; 0 beyond the radius), with
 // 3 x 384 doubles of scratch
-__device__ void kernel_point(const double* table, const double x[4], const double y[4], double* scratch,
-                             double* out) {
+template <class P>
+__device__ void kernel_point(const double* table, const double x[4], const double y[4], P scratch, P out) {
   const double zero[4] = {0, 0, 0, 0};
   const double rr = qed::kRadius * qed::kRadius;
   if (norm2(x) > rr || norm2(y) > rr) {
     for (int o = 0; o < qed::kNL; ++o) out[o] = 0.0;
     return;
   }
-  double* l0xy = scratch;
-  double* l00y = l0xy + qed::kNL;
-  double* l0x0 = l00y + qed::kNL;
+  const P l0xy = scratch;
+  const P l00y = l0xy + qed::kNL;
+  const P l0x0 = l00y + qed::kNL;
   l0_point(table, x, y, l0xy);
   l0_point(table, zero, y, l00y);
   l0_point(table, x, zero, l0x0);
@@ -67,7 +78,7 @@ __global__ void qed_kernel_points(const double* __restrict__ table, const double
 
 // The 2+2 pieces P2_0, P2_1, P3, P4_0, P4_1 (kernel_p20, kernel_p21, kernel_p3,
 // kernel_p4 of compute_2p2_gpu, kernels/hlbl_kernels/kernels.cu:374-760) of sites
-// [s0, s0 + n) for one target: contrib[site][5][rho][sigma][nu]. With x = z - w and
+// [s0, s0 + n) for one target: contrib[5][rho][sigma][nu][site]. With x = z - w and
 // y = target - w (mod L; y given in [0, L)), each argument difference min-imaged on its
 // own and scaled by xunit, k over (r, s) = (0,1),(0,2),(0,3),(1,2),(1,3),(2,3):
 //   P2_0[r][s][nu] = L[k](x, y)[mu][nu][lam] Pi_{mu lam}(z)
@@ -75,10 +86,11 @@ __global__ void qed_kernel_points(const double* __restrict__ table, const double
 //   P3  [r][s][nu] = L[k](x - y, -y)[mu][lam][nu] Pi_{mu lam}(z)
 //   P4_0[r][s][nu] = L[k](y - x, -x)[nu][lam][mu] Pi_{mu lam}(z)
 //   P4_1[r][s][nu] = xv_r P4_0 term, P4_1[s][r][nu] = -xv_s P4_0 term (xv the integer
-//   min-image of x). One thread per site; scratch 4 x 384 doubles per thread.
+//   min-image of x). One thread per site; scratch 4 x 384 doubles per thread, lane-strided
+//   (element j of thread i at scratch[j n + i]); contrib[j][V], site-minor.
 __global__ void p234_sites(const double* __restrict__ table, const double* __restrict__ pi,
                            hlbl_geom local, hlbl_geom global, hlbl_coord proc, hlbl_coord gsw,
-                           hlbl_coord y, double xunit, size_t s0, size_t n, double* __restrict__ scratch,
+                           hlbl_coord y, double xunit, size_t s0, size_t n, size_t V, double* __restrict__ scratch,
                            double* __restrict__ contrib) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -95,9 +107,9 @@ __global__ void p234_sites(const double* __restrict__ table, const double* __res
     xm[r] = xv[r] * xunit;
     ym[r] = coord_map(yi[r], L) * xunit;
   }
-  double* K = scratch + i * 4 * qed::kNL;
-  double* tmp = K + qed::kNL;
-  double* out = contrib + z * 320;
+  const LaneView K{scratch + i, n};
+  const LaneView tmp = K + qed::kNL;
+  const LaneView out{contrib + z, V};
   for (int j = 0; j < 320; ++j) out[j] = 0.0;
   double p[16];
   for (int j = 0; j < 16; ++j) p[j] = pi[z * 16 + j];
@@ -142,14 +154,14 @@ __global__ void p234_sites(const double* __restrict__ table, const double* __res
   }
 }
 
-// out[j] = sum over the V sites of contrib[site][j], j < 320: one workgroup per j,
+// out[j] = sum over the V sites of contrib[j][site], j < 320: one workgroup per j,
 // per-thread strided sums and a fixed tree
 __global__ __launch_bounds__(256) void p234_reduce(const double* __restrict__ contrib, size_t V,
                                                    double* __restrict__ out) {
   __shared__ double red[256];
   const int j = blockIdx.x;
   double s = 0.0;
-  for (size_t z = threadIdx.x; z < V; z += 256) s += contrib[z * 320 + j];
+  for (size_t z = threadIdx.x; z < V; z += 256) s += contrib[(size_t)j * V + z];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int h = 128; h > 0; h >>= 1) {
@@ -170,8 +182,8 @@ __global__ void qed_channels_points(const double* __restrict__ table, const doub
 void launch_p234_2p2(hlbl_contractor* c, const double* pi, hlbl_coord gsw, const hlbl_coord* targets,
                      int n_targets, double xunit, double* d_p234, hipStream_t st) {
   const size_t V = c->volume;
-  // sites per launch, one thread each with 4 x 384 doubles (12 KiB) of scratch: 2^19
-  // sites are 8192 waves, 8 per SIMD, for 6 GiB of scratch
+  // sites per launch, one thread each with 4 x 384 doubles (12 KiB) of scratch, strided
+  // by the launch's site count: 2^19 sites are 8192 waves, 8 per SIMD, for 6 GiB
   constexpr size_t kChunk = size_t(1) << 19;
   double *scratch = nullptr, *contrib = nullptr;
   check_hip(hipMallocAsync((void**)&scratch, std::min(V, kChunk) * 4 * qed::kNL * sizeof(double), st),
@@ -185,7 +197,7 @@ void launch_p234_2p2(hlbl_contractor* c, const double* pi, hlbl_coord gsw, const
     for (size_t s0 = 0; s0 < V && e == hipSuccess; s0 += kChunk) {
       const size_t n = std::min(kChunk, V - s0);
       hipLaunchKernelGGL(p234_sites, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, c->qed->table64.p,
-                         pi, c->local, c->global, c->proc, gsw, y, xunit, s0, n, scratch, contrib);
+                         pi, c->local, c->global, c->proc, gsw, y, xunit, s0, n, V, scratch, contrib);
       e = hipGetLastError();
     }
     if (e == hipSuccess) {
