@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 DIMS = (48, 24, 24, 24)
 XUNIT = 105.658 / 197.3269804 * 0.1  # m_mu a, a = 0.1 fm
 MI355X_HBM = 8.0e12
+MI355X_FP16_MFMA = 2.5e15  # dense
 
 
 def log(*a):
@@ -46,35 +47,89 @@ def measured_traffic(kernel):
     return None
 
 
-def cpu_baseline(seconds_target=12.0):
-    """The oracle (CPU restatement, FP64, OpenMP) on a bounded sample: whole pairs
-    on an 8^4 lattice, repeated for about `seconds_target`, scaled per site to 24^3x48."""
+def _oracle():
     import numpy as np
     o = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
-    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     o.oracle_kernel_sum.argtypes = [ctypes.c_void_p] * 5 + [ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+    o.oracle_source_seed.restype = ctypes.c_uint64
+    o.oracle_source_seed.argtypes = [ctypes.c_uint64, ctypes.c_void_p]
+    o.oracle_fill_block.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_uint64]
+    return o, (lambda a: a.ctypes.data_as(ctypes.c_void_p)), np
+
+
+def cpu_baseline(H, one_core_seconds=8.0):
+    """The oracle (CPU restatement, FP64, OpenMP over sites) timed on this host:
+    config 1 whole (8^4, chain h=(1,1,1,1), n=1, K=1: 31 blocks, 80 pairs, RandomProvider
+    seed 0, xunit = m_mu a at a = 0.1 fm) on all cores, then pairs of it on one core for
+    about one_core_seconds. Per-site cost scaled to 24^3x48 for `value` (pass 1 and
+    pass 2 are both per-site loops)."""
+    o, P, np = _oracle()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    dims = (8, 8, 8, 8)
+    V = 8 ** 4
+    sch = H.Schedule((1, 1, 1, 1), 1, 1)
+    pairs = [(q, p) for _w, p, parts in sch.sources() for q in parts]
+    assert len(pairs) == 80
+    wrap = lambda v: tuple(a % L for a, L in zip(v, dims))
+    blocks = {}
+    d4 = np.array(dims, np.int32)
+    for q, p in pairs:
+        for pos in (q, p):
+            if pos not in blocks:
+                b = np.empty(2 * 12 * 24 * V)
+                seed = o.oracle_source_seed(0, P(np.array(wrap(pos), np.int32)))
+                o.oracle_fill_block(P(b), P(d4), None, ctypes.c_uint64(seed))
+                blocks[pos] = b
+    lat = H.Lattice(dims)
+    out = np.zeros(2)
+
+    def run(pr):
+        o.oracle_kernel_sum(P(blocks[pr[0]]), P(blocks[pr[1]]), P(d4), P(np.array(wrap(pr[0]), np.int32)),
+                            P(np.array(lat.site_map_diff(pr[1], pr[0]), np.int32)), ctypes.c_double(XUNIT), 0,
+                            P(out))
+
+    o.oracle_set_threads(threads)
+    t0 = time.perf_counter()
+    for pr in pairs:
+        run(pr)
+    t_all = (time.perf_counter() - t0) / len(pairs)
+    o.oracle_set_threads(1)
+    n1, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < one_core_seconds:
+        run(pairs[n1 % len(pairs)])
+        n1 += 1
+    t_one = (time.perf_counter() - t0) / n1
+    o.oracle_set_threads(threads)
+    big = 48 * 24 ** 3 / V
+    return {"value": 1.0 / (t_all * big), "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"config 1 whole (8^4, 80 pairs, {t_all:.3f} s per pair on {threads} threads) and "
+                      f"{n1} of its pairs on 1 thread ({t_one:.2f} s each); per-site cost scaled by "
+                      f"sites to 24^3x48",
+            "value_1core": 1.0 / (t_one * big), "config1_pairs_per_s": 1.0 / t_all,
+            "config1_pairs_per_s_1core": 1.0 / t_one}
+
+
+def cpu_full_pair(H):
+    """One whole 24^3x48 pair through the oracle on all host cores (blocks of seed 2 at
+    two window-1 positions); prints its own JSON line."""
+    o, P, np = _oracle()
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     o.oracle_set_threads(threads)
-    dims = np.array((8, 8, 8, 8), np.int32)
-    V = int(np.prod(dims))
-    S = np.empty(2 * 12 * 24 * V)
-    Y = np.empty_like(S)
-    o.oracle_fill_block(P(S), P(dims), None, ctypes.c_uint64(11))
-    o.oracle_fill_block(P(Y), P(dims), None, ctypes.c_uint64(12))
+    V = 48 * 24 ** 3
+    d4 = np.array(DIMS, np.int32)
+    lat = H.Lattice(DIMS)
+    q, p = (3, 3, 3, 3), (6, 6, 6, 6)
+    S, Y = np.empty(2 * 12 * 24 * V), np.empty(2 * 12 * 24 * V)
+    for pos, b in ((q, S), (p, Y)):
+        seed = o.oracle_source_seed(2, P(np.array(pos, np.int32)))
+        o.oracle_fill_block(P(b), P(d4), None, ctypes.c_uint64(seed))
     out = np.zeros(2)
-    n, t0 = 0, time.time()
-    while True:
-        o.oracle_kernel_sum(P(S), P(Y), P(dims), P(np.zeros(4, np.int32)), P(np.ones(4, np.int32)),
-                            ctypes.c_double(XUNIT), 0, P(out))
-        n += 1
-        if time.time() - t0 > seconds_target:
-            break
-    dt = (time.time() - t0) / n
-    per_site = dt / V
-    v_big = 48 * 24 ** 3
-    return {"value": 1.0 / (per_site * v_big), "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": f"{n} whole pairs (both flavours) on an 8^4 lattice, {dt:.2f} s each, "
-                      f"scaled by sites to 24^3x48"}
+    t0 = time.perf_counter()
+    o.oracle_kernel_sum(P(S), P(Y), P(d4), P(np.array(q, np.int32)), P(np.array(lat.site_map_diff(p, q), np.int32)),
+                        ctypes.c_double(XUNIT), 0, P(out))
+    dt = time.perf_counter() - t0
+    print(json.dumps({"metric": "oracle seconds per 24^3x48 pair (both flavours)", "value": dt, "unit": "s",
+                      "cores": threads, "kernel_sum": list(out)}), flush=True)
 
 
 def time_form(con, descs, form, warmup, steps):
@@ -141,6 +196,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-full-pair", action="store_true",
+                    help="only time one whole 24^3x48 pair through the oracle on the host cores, then exit")
     ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 1 GiB device buffer three times (reads and writes 1 GiB "
@@ -155,8 +212,13 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
     import hlbl_amd as H
+
+    if args.cpu_full_pair:
+        cpu_full_pair(H)
+        return
+
+    import torch
 
     if args.copy_calibration:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
@@ -231,27 +293,37 @@ def main():
     head = min(ok, key=lambda f: forms[f][0]) if ok else 1
     dt, err, ms = forms[head]
     value = 65.0 / dt
-    # roofline of the dominant kernel, pass 2, against HBM (its highest floor):
-    # algorithmic bytes per launch = per pair, both FP64 blocks once (2 x 2 x 12 x 24
-    # doubles per site) and the pair's coefficient field once (640 floats per site)
+    # roofline of the dominant kernel, pass 2. HBM: the window's compulsory bytes, each of
+    # its 31 FP64 blocks read once (31 x 4608 V bytes). Compute: pass 2's nominal flops,
+    # per pair 2 V (8 x 6912 + 4 x 144 x 640) = 847,872 V (the dxu products and the 640
+    # real 12x12 traces of both flavours), against the dense FP16 MFMA peak the traces
+    # run on (each product is issued as three FP16 products; counted once). The whole
+    # window: 958,464 V per pair (pass 1 included).
     V = lat.volume()
-    bytes_pass2 = 65 * (2 * 2 * 12 * 24 * 8 + 640 * 4) * V
-    achieved = bytes_pass2 / (ms[3] * 1e-3)
+    bytes_window = 31 * 4608 * V
+    achieved = bytes_window / (ms[3] * 1e-3)
+    flops_pass2 = 65 * 847872.0 * V
+    flops_window = 65 * 958464.0 * V
     traffic = measured_traffic("pass2_window")
-    cpu = None if args.no_cpu else cpu_baseline()
+    cpu = None if args.no_cpu else cpu_baseline(H)
     names = {1: "FP32 window-batched (pass-2 products as scaled FP16 hi/lo x3), plain QED kernel",
              2: "FP32 window-batched (pass-2 products as scaled FP16 hi/lo x3), MLP surrogate channels"}
     line = {
         "metric": "pairs/sec at 24^3x48 on 1 GPU, surrogate+FP32 vs plain FP64, rel err <=1e-3",
         "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32", "data": "synthetic",
+        "dtype": "fp16x3/fp32", "data": "synthetic",
         "config": {"workload": "24^3x48 window 1 of chain K=8 n=3 h=(1,1,1,1): 65 pairs x 2 flavours, "
                                f"30 resident FP64 blocks, seed 2; {names[head]}",
                    "lattice": list(DIMS), "pairs_per_step": 65, "xunit": XUNIT, "form": head},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": MI355X_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / MI355X_HBM, "traffic": traffic,
-                     "kernel": "pass2_window", "ms": ms[3]},
+                     "kernel": "pass2_window", "ms": ms[3], "bytes": bytes_window},
+        "compute": {"bound": "mfma", "kernel": "pass2_window", "achieved": flops_pass2 / (ms[3] * 1e-3) / 1e12,
+                    "peak": MI355X_FP16_MFMA / 1e12, "unit": "TFLOP/s",
+                    "frac": flops_pass2 / (ms[3] * 1e-3) / MI355X_FP16_MFMA, "flops": flops_pass2,
+                    "window_tflops": flops_window / dt / 1e12},
+        "surrogate_within_bound": bool(err2.max() <= 1e-3),
         "max_pair_rel_err": float(err),
         "plain_fp64_pairs_per_s": 1.0 / dt64,
         "speedup_vs_plain_fp64": value * dt64,

@@ -17,6 +17,8 @@ import ctypes as C
 import math
 import os
 
+import numpy as np
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libhlbl_amd.so")
 
@@ -60,6 +62,10 @@ _SIGS = {
     "hlbl_fill_random_block_host": (C.c_int, [C.c_void_p, Geom, Coord, C.c_uint64, C.c_int]),
     "hlbl_fill_random_block_device": (C.c_int, [C.c_void_p, Geom, Coord, C.c_uint64, C.c_void_p]),
     "hlbl_qed_init": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    "hlbl_window_set_judge_bound": (C.c_int, [C.c_void_p, C.c_double]),
+    "hlbl_window_fallback": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "hlbl_qed_init_file": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_char_p]),
+    "hlbl_qed_exact": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
     "hlbl_qed_free": (C.c_int, [C.c_void_p]),
     "hlbl_qed_table_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -112,6 +118,9 @@ _SIGS = {
     "hlbl_max_steps_without_wrap": (C.c_int, [Coord, C.c_int, Coord, Coord, C.c_int]),
     "hlbl_run_chain": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_double, C.c_int,
                                  C.c_int, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "hlbl_run_chain_2p2": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_double, C.c_int, C.c_int,
+                                     C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                     C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "hlbl_run_chain_each_window": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_double, C.c_int,
                                              C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                              C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
@@ -213,12 +222,30 @@ class RandomProvider:
         return 0
 
 
-class QedKernel:
-    """L^(Lambda), Lambda = 0.4 (KQED's initialise / QED_Mkernel_L2 replacement)."""
+TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "qed_kernel_table.bin")
 
-    def __init__(self, device=0):
+
+def qed_exact(a, beta, b, quad=None):
+    """The 42 frame components of T^I, T^II, T^III at |x| = a, angle beta, |y| = b from
+    the kernel's defining integrals (host; quad = (nt, nr, ntheta, npsi) or None for the
+    table's own quadrature)."""
+    out = (C.c_double * 42)()
+    q = None if quad is None else (C.c_int * 4)(*quad)
+    check(lib().hlbl_qed_exact(float(a), float(beta), float(b), q, out))
+    return list(out)
+
+
+class QedKernel:
+    """L^(Lambda), Lambda = 0.4 (KQED's initialise / QED_Mkernel_L2 replacement). The
+    plain kernel interpolates the tabulated T tensors of table_path (default: the
+    library's data/qed_kernel_table.bin)."""
+
+    def __init__(self, device=0, table_path=None):
         h = C.c_void_p()
-        check(lib().hlbl_qed_init(C.byref(h), device))
+        if table_path is None:
+            check(lib().hlbl_qed_init(C.byref(h), device))
+        else:
+            check(lib().hlbl_qed_init_file(C.byref(h), device, os.fsencode(table_path)))
         self.handle = h
 
     def close(self):
@@ -332,6 +359,16 @@ class Contractor:
         check(lib().hlbl_window_judge(self.handle, C.byref(e), C.byref(n)))
         return e.value, n.value
 
+    def set_judge_bound(self, bound):
+        """Form 2 returns the plain kernel's results (form 1) when the judge exceeds bound."""
+        check(lib().hlbl_window_set_judge_bound(self.handle, float(bound)))
+
+    def fell_back(self):
+        """Whether the last form-2 window returned the plain kernel's results."""
+        f = C.c_int()
+        check(lib().hlbl_window_fallback(self.handle, C.byref(f)))
+        return bool(f.value)
+
     # seam 2: the launchers of kernels/cuda_lattice.h
     def dzu_dzsu(self, d_dzu, d_dzsu, fwd_src, fwd_y, iflavor, gsx, stream=None):
         L = self.lattice
@@ -427,8 +464,12 @@ def max_steps_without_wrap(h, stride, lattice, origin=(0, 0, 0, 0), cap=10000):
                                              cap)
 
 
+class Plan2p2(C.Structure):
+    _fields_ = [("w", Coord), ("window", C.c_int), ("targets", C.POINTER(Coord)), ("n_targets", C.c_int)]
+
+
 def run_chain(contractor, schedule, seed, xunit_, residency=0, form=0, batch=0, writer=None,
-              chain=0):
+              chain=0, plan_2p2=None, iflavor_2p2=0):
     """The schedule loop on random fields (src/driver.cpp:23-165).
 
     batch 0: one pair per call; 1: one call per window (its sources solved first);
@@ -436,7 +477,12 @@ def run_chain(contractor, schedule, seed, xunit_, residency=0, form=0, batch=0, 
     With a ResultFile as writer, each window's rows are appended to it (and flushed)
     when the window is done, as the reference's driver does (src/driver.cpp:134-137);
     chain is the chain index stored in the rows.
-    Returns (rows, stats); rows are dicts {x, y, yv, window, chain, kernel_sum}."""
+    plan_2p2: [(site in [0, L), window, [targets]), ...], the chain's planned 2+2
+    sources: computed inside the solve loop from the solved block (src/driver.cpp:58-95),
+    appended to the writer (opened with_2p2) after each window's kernel-sum rows.
+    Returns (rows, stats), rows dicts {x, y, yv, window, chain, kernel_sum}; with
+    plan_2p2 (rows, stats, rows_2p2), rows_2p2 dicts {w, chain, window, p1, targets, p234}
+    in run order."""
     n_pairs = len(schedule.pairs())
     buf = (C.c_double * (16 * max(1, n_pairs)))()
     st = (C.c_double * 5)()
@@ -444,9 +490,40 @@ def run_chain(contractor, schedule, seed, xunit_, residency=0, form=0, batch=0, 
     if writer is not None:
         fn = C.cast(lib_h5().hlbl_result_append_rows, C.c_void_p)
         user = writer.handle
-    check(lib().hlbl_run_chain_each_window(contractor.handle, schedule.handle, C.c_uint64(seed),
-                                           xunit_, residency, form, batch, chain, fn, user, buf,
-                                           n_pairs, st))
+    if plan_2p2 is None:
+        check(lib().hlbl_run_chain_each_window(contractor.handle, schedule.handle, C.c_uint64(seed),
+                                               xunit_, residency, form, batch, chain, fn, user, buf,
+                                               n_pairs, st))
+    else:
+        keep = []
+        plan = (Plan2p2 * max(1, len(plan_2p2)))()
+        for i, (w, window, targets) in enumerate(plan_2p2):
+            tg = (Coord * max(1, len(targets)))(*[_coord(t) for t in targets])
+            keep.append(tg)
+            plan[i] = Plan2p2(_coord(w), int(window), tg, len(targets))
+        rows_2p2 = []
+
+        def on_2p2(_user, rows, n):
+            try:
+                for i in range(n):
+                    r = rows[i]
+                    nt = r.n_targets
+                    rows_2p2.append({"w": (r.w.t, r.w.x, r.w.y, r.w.z), "chain": r.chain, "window": r.window,
+                                     "p1": np.ctypeslib.as_array(r.p1, (64 * max(contractor.lattice.global_),)).copy(),
+                                     "targets": [(r.targets[k].t, r.targets[k].x, r.targets[k].y, r.targets[k].z)
+                                                 for k in range(nt)],
+                                     "p234": np.ctypeslib.as_array(r.p234, (320 * nt,)).copy() if nt else
+                                     np.zeros(0)})
+                if writer is not None and lib_h5().hlbl_result_append_2p2(writer.handle, rows, n) != 0:
+                    return 1
+                return 0
+            except Exception:  # noqa: BLE001 (reported as a failed callback)
+                return 1
+
+        cb = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(Source2p2), C.c_int)(on_2p2)
+        check(lib().hlbl_run_chain_2p2(contractor.handle, schedule.handle, C.c_uint64(seed), xunit_,
+                                       residency, form, batch, chain, fn, user, plan, len(plan_2p2),
+                                       iflavor_2p2, C.cast(cb, C.c_void_p), None, buf, n_pairs, st))
     rows = []
     for i in range(n_pairs):
         r = buf[16 * i:16 * i + 16]
@@ -454,7 +531,9 @@ def run_chain(contractor, schedule, seed, xunit_, residency=0, form=0, batch=0, 
                      "yv": tuple(int(v) for v in r[8:12]), "window": int(r[12]),
                      "chain": int(r[13]), "kernel_sum": [r[14], r[15]]})
     keys = ("pairs", "positions_solved", "cache_peak", "seconds_contract", "seconds_solve")
-    return rows, dict(zip(keys, list(st)))
+    if plan_2p2 is None:
+        return rows, dict(zip(keys, list(st)))
+    return rows, dict(zip(keys, list(st))), rows_2p2
 
 
 def run_chains(contractor, schedules, seed, xunit_, residency=0, form=0, batch=0, writer=None):
@@ -527,45 +606,24 @@ def report_2p2(plan, sources, max_sites=10):
     return lines
 
 
-def run_2p2(contractor, lattice, seed, schedules, sources_2p2, xunit_, writer=None, iflavor=0):
-    """The 2+2 step of a run (src/driver.cpp:58-95 with plan_2p2): at the first solve of
-    each 2+2 source (site, targets), P1 and the P234 rows of its targets from that
-    source's block, the RandomProvider field of the run's seed as the chain solves it.
-    Sources are visited in run order (chains, windows, solve order); each row carries
-    the chain and window of its solve, and with a ResultFile opened with with_2p2 each
-    window's rows are appended together (the reference appends at the window's end).
-    Returns (rows, plan)."""
-    import torch
+def run_2p2(contractor, lattice, seed, schedules, sources_2p2, xunit_, writer=None, iflavor=0,
+            residency=0, form=0, batch=0):
+    """A run with its 2+2 step (src/driver.cpp:58-95 with plan_2p2): the chains in turn
+    through the library's solve loop, which computes P1 and the P234 rows of its targets
+    at the first solve in run order of each 2+2 source (site, targets), from that solved
+    block (the RandomProvider field of the run's seed). Each row carries the chain and
+    window of its solve; with a ResultFile opened with_2p2 each window's 2+2 rows are
+    appended after its kernel-sum rows. Returns (rows_2p2 in run order, plan)."""
     dims = tuple(lattice.global_)
     wrap = lambda v: tuple(int(a) % int(L) for a, L in zip(v, dims))
     plan = plan_2p2(schedules, [s for s, _t in sources_2p2], dims)
-    by_site = {wrap(sources_2p2[i][0]): i for i in plan["at"]}
-    lmax = max(dims)
-    prov = RandomProvider(lattice, seed)
-    blk = torch.empty(block_length(lattice), dtype=torch.float64, device="cuda")
-    rows, pending, done = [], [], set()
+    rows = []
     for c, sch in enumerate(schedules):
-        for window, pos, _parts in sch.sources():
-            w = wrap(pos)
-            i = by_site.get(w)
-            if i is None or i in done or plan["at"][i] != (c, window):
-                continue
-            done.add(i)
-            targets = [wrap(t) for t in sources_2p2[i][1]]
-            prov.compute(w, blk.data_ptr())
-            d1 = torch.empty(64 * lmax, dtype=torch.float64, device="cuda")
-            d234 = torch.empty(max(1, len(targets)) * 320, dtype=torch.float64, device="cuda")
-            contractor.pieces_2p2(blk.data_ptr(), w, targets, xunit_, d1.data_ptr(), d234.data_ptr(),
-                                  iflavor)
-            row = {"w": w, "chain": c, "window": window, "p1": d1.cpu().numpy(), "targets": targets,
-                   "p234": d234.cpu().numpy()[:320 * len(targets)]}
-            if writer is not None and pending and (pending[-1]["chain"], pending[-1]["window"]) != (c, window):
-                writer.append_2p2(pending)
-                pending = []
-            pending.append(row)
-            rows.append(row)
-    if writer is not None and pending:
-        writer.append_2p2(pending)
+        entries = [(wrap(sources_2p2[i][0]), w, [wrap(t) for t in sources_2p2[i][1]])
+                   for i, (ci, w) in plan["at"].items() if ci == c]
+        _r, _st, r2 = run_chain(contractor, sch, seed, xunit_, residency, form, batch, writer, chain=c,
+                                plan_2p2=entries, iflavor_2p2=iflavor)
+        rows += r2
     return rows, plan
 
 
@@ -621,11 +679,11 @@ class ResultFile:
     run attributes; every append is flushed.
 
     chains: [(start, direction, stride, steps)], steps resolved. The kernel attribute
-    names the QED kernel used; this build's plain kernel has stand-in weight functions,
-    so the default says so."""
+    names the QED kernel used; the default is the reference's, L^(Lambda) at
+    Lambda = 0.4."""
 
     def __init__(self, path, lattice_dims, chains, xunit_, muon_mass_mev=0.0, lattice_spacing_fm=0.0,
-                 procs=(1, 1, 1, 1), nconf=-1, kernel="LLambda0.4 (stand-in weight functions)",
+                 procs=(1, 1, 1, 1), nconf=-1, kernel="LLambda0.4",
                  with_2p2=False):
         L = lib_h5()
         arr = (ChainInfo * len(chains))(*[ChainInfo(_coord(s), _coord(d), int(st), int(n))

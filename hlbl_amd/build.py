@@ -60,7 +60,24 @@ def build(verbose=True):
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
     build_h5(verbose)
+    build_qedgen()
     return LIB
+
+
+# The table generator (host C++, OpenMP): writes data/qed_kernel_table.bin from the
+# kernel's defining integrals (about 50 minutes on 8 cores); not run by the build.
+QEDGEN_SRC = os.path.join(HERE, "qedgen", "gen_qed_table.cpp")
+QEDGEN = os.path.join(HERE, "qedgen", "gen_qed_table")
+
+
+def build_qedgen():
+    cmd = [os.environ.get("CXX", "g++"), "-O3", "-march=x86-64-v2", "-std=c++17", "-fopenmp",
+           "-Wall", QEDGEN_SRC, "-o", QEDGEN + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("gen_qed_table build failed:\n" + r.stdout + r.stderr)
+    os.replace(QEDGEN + ".tmp", QEDGEN)
+    return QEDGEN
 
 
 # The result-file library (libhlbl_h5.so): host C++ linked against HDF5, kept apart so

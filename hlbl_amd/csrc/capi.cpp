@@ -2,7 +2,10 @@
 // the per-pair contraction (Contractor::kernel_sum, src/contract.cpp:104-114).
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <string>
@@ -13,7 +16,9 @@
 #include "context.hpp"
 #include "contract_kernels.hpp"
 #include "provider.hpp"
+#include "qed_exact.hpp"
 #include "qed_host.hpp"
+#include "driver.hpp"
 #include "schedule.hpp"
 
 namespace hlbl {
@@ -60,6 +65,25 @@ int device_cu_count(int device) {
 }  // namespace hlbl
 
 using namespace hlbl;
+
+namespace hlbl {
+namespace {
+
+// The committed table next to the library: <dir of libhlbl_amd.so>/data/qed_kernel_table.bin,
+// unless HLBL_QED_TABLE names another file.
+std::string default_table_path() {
+  if (const char* env = std::getenv("HLBL_QED_TABLE")) return env;
+  Dl_info info;
+  if (dladdr(reinterpret_cast<void*>(&hlbl_qed_init), &info) == 0 || !info.dli_fname)
+    throw Error("hlbl_qed_init: cannot locate the library's directory");
+  std::string dir = info.dli_fname;
+  const size_t slash = dir.find_last_of('/');
+  dir = slash == std::string::npos ? std::string(".") : dir.substr(0, slash);
+  return dir + "/data/qed_kernel_table.bin";
+}
+
+}  // namespace
+}  // namespace hlbl
 
 extern "C" {
 
@@ -121,9 +145,10 @@ int hlbl_fill_random_block_device(double* d_out, hlbl_geom local, hlbl_coord pro
 
 // ---- QED kernel -------------------------------------------------------------------
 
-int hlbl_qed_init(hlbl_qed** out, int device) {
+int hlbl_qed_init_file(hlbl_qed** out, int device, const char* table_path) {
   return guarded([&] {
     check_ptr(out, "out");
+    check_ptr(table_path, "table_path");
     *out = nullptr;
     check_hip(hipSetDevice(device), "hipSetDevice");
     auto q = new hlbl_qed();
@@ -134,7 +159,11 @@ int hlbl_qed_init(hlbl_qed** out, int device) {
       upload_terms_fp64(tt);
       upload_terms_fp32(tt);
       upload_terms_qed(tt);
-      qed::build_weight_table(q->host_table);
+      try {
+        qed::load_table(table_path, q->host_table);
+      } catch (const std::runtime_error& e) {
+        throw Error(e.what());
+      }
       const size_t n = q->host_table.size();
       q->table64.ensure(n);
       check_hip(hipMemcpy(q->table64.p, q->host_table.data(), n * sizeof(double),
@@ -148,6 +177,31 @@ int hlbl_qed_init(hlbl_qed** out, int device) {
       throw;
     }
     *out = q;
+  });
+}
+
+int hlbl_qed_init(hlbl_qed** out, int device) {
+  std::string path;
+  const int rc = guarded([&] { path = hlbl::default_table_path(); });
+  if (rc != 0) return rc;
+  return hlbl_qed_init_file(out, device, path.c_str());
+}
+
+int hlbl_qed_exact(double a, double beta, double b, const int* quad, double* out) {
+  return guarded([&] {
+    check_ptr(out, "out");
+    if (!(a >= 0 && b >= 0 && beta >= 0 && beta <= M_PI))
+      throw Error("hlbl_qed_exact: need |x|, |y| >= 0 and beta in [0, pi]");
+    qed_exact::Config cfg;
+    cfg.nt = 16, cfg.nr = 28, cfg.nth = 8, cfg.nps = 16;
+    if (quad) {
+      if (quad[0] < 4 || quad[0] > 64 || quad[1] < 4 || quad[2] < 2 || quad[3] < 4)
+        throw Error("hlbl_qed_exact: quadrature sizes out of range");
+      cfg.nt = quad[0], cfg.nr = quad[1], cfg.nth = quad[2], cfg.nps = quad[3];
+    }
+    const qed_exact::Tframe f = qed_exact::Evaluator(cfg).eval(a, b, beta);
+    for (int A = 0; A < 3; ++A)
+      for (int c = 0; c < qed::kFrame; ++c) out[A * qed::kFrame + c] = f.c[A][c];
   });
 }
 
@@ -335,18 +389,7 @@ int hlbl_insert_g5_gamma(const double* d_dzu, const double* d_dzsu, double* d_g_
     check_ptr(d_g_dzu, "d_g_dzu");
     check_ptr(d_g_dzsu, "d_g_dzsu");
     hipStream_t st = as_stream(stream);
-    // stage one flavour into the two-flavour layout of form_gdz
-    double *dz = nullptr, *gdz = nullptr;
-    check_hip(hipMallocAsync((void**)&dz, 2 * 10 * 144 * 2 * sizeof(double), st), "hipMallocAsync");
-    check_hip(hipMallocAsync((void**)&gdz, 2 * 40 * 144 * 2 * sizeof(double), st), "hipMallocAsync");
-    check_hip(hipMemsetAsync(dz, 0, 2 * 10 * 144 * 2 * sizeof(double), st), "memset");
-    check_hip(hipMemcpyAsync(dz, d_dzu, 6 * 144 * 2 * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-    check_hip(hipMemcpyAsync(dz + 6 * 144 * 2, d_dzsu, 4 * 144 * 2 * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-    launch_gamma(dz, gdz, st);
-    check_hip(hipMemcpyAsync(d_g_dzu, gdz, 6 * 4 * 144 * 2 * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-    check_hip(hipMemcpyAsync(d_g_dzsu, gdz + 6 * 4 * 144 * 2, 4 * 4 * 144 * 2 * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-    (void)hipFreeAsync(dz, st);
-    (void)hipFreeAsync(gdz, st);
+    launch_gamma_one(d_dzu, d_dzsu, d_g_dzu, d_g_dzsu, st);
     check_hip(hipStreamSynchronize(st), "insert_g5_gamma");
   });
 }
@@ -471,15 +514,9 @@ namespace hlbl {
 void window_run(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit, int form,
                 double* out, hipStream_t st);
 void window_judge_result(const hlbl_contractor* c, double* err, int* n);
+void window_set_judge_bound(hlbl_contractor* c, double bound);
+void window_fallback_result(const hlbl_contractor* c, int* fell_back);
 
-struct PairRow {
-  Vec4 x, y, yv;
-  int window;
-  double ks[2];
-};
-void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xunit, bool host_tier,
-               int form, int batch, std::vector<PairRow>& rows, double stats[5],
-               const std::function<void(size_t, size_t)>& on_window);
 
 // a row as the C ABI hands it out: x[4], y[4], yv[4], window, chain, ks0, ks1
 void pack_row(const PairRow& p, int chain, double* r) {
@@ -535,6 +572,21 @@ int hlbl_window_judge(const hlbl_contractor* c, double* max_rel_err, int* n_samp
     check_ptr(max_rel_err, "max_rel_err");
     check_ptr(n_samples, "n_samples");
     window_judge_result(c, max_rel_err, n_samples);
+  });
+}
+
+int hlbl_window_set_judge_bound(hlbl_contractor* c, double bound) {
+  return guarded([&] {
+    check_ptr(c, "contractor");
+    window_set_judge_bound(c, bound);
+  });
+}
+
+int hlbl_window_fallback(const hlbl_contractor* c, int* fell_back) {
+  return guarded([&] {
+    check_ptr(c, "contractor");
+    check_ptr(fell_back, "fell_back");
+    window_fallback_result(c, fell_back);
   });
 }
 
@@ -623,10 +675,11 @@ int hlbl_mlp_loss_grad(hlbl_qed* q, const float* d_in, const float* d_target, in
   });
 }
 
-int hlbl_run_chain_each_window(hlbl_contractor* c, const hlbl_schedule* s, uint64_t seed,
-                               double xunit, int residency, int form, int batch, int chain,
-                               hlbl_window_rows_fn fn, void* user, double* rows, int max_rows,
-                               double stats[5]) {
+int hlbl_run_chain_2p2(hlbl_contractor* c, const hlbl_schedule* s, uint64_t seed, double xunit,
+                       int residency, int form, int batch, int chain, hlbl_window_rows_fn fn,
+                       void* user, const hlbl_plan_2p2* plan, int n_plan, int iflavor,
+                       hlbl_window_2p2_fn fn2, void* user2, double* rows, int max_rows,
+                       double stats[5]) {
   return guarded([&] {
     check_ptr(c, "contractor");
     check_ptr(s, "schedule");
@@ -634,6 +687,25 @@ int hlbl_run_chain_each_window(hlbl_contractor* c, const hlbl_schedule* s, uint6
     if (residency != 0 && residency != 1) throw Error("residency must be 0 (device) or 1 (host)");
     if (chain < 0) throw Error("chain must be non-negative");
     if (batch < 0 || batch > 2) throw Error("batch must be 0 (pairs), 1 (windows) or 2 (sources)");
+    if (n_plan < 0) throw Error("n_plan < 0");
+    if (n_plan > 0) check_ptr(plan, "plan");
+    if (iflavor != 0 && iflavor != 1) throw Error("iflavor must be 0 or 1");
+    std::map<Vec4, Row2p2> plan2;
+    const hlbl_geom& g = c->global;
+    for (int i = 0; i < n_plan; ++i) {
+      const hlbl_plan_2p2& e = plan[i];
+      const Vec4 w = {e.w.t, e.w.x, e.w.y, e.w.z};
+      if (w[0] < 0 || w[0] >= g.T || w[1] < 0 || w[1] >= g.LX || w[2] < 0 || w[2] >= g.LY || w[3] < 0 ||
+          w[3] >= g.LZ)
+        throw Error("hlbl_run_chain_2p2: a planned site is outside [0, L)");
+      if (e.n_targets < 0 || (e.n_targets > 0 && !e.targets)) throw Error("hlbl_run_chain_2p2: bad targets");
+      if (plan2.count(w)) throw Error("hlbl_run_chain_2p2: a site is planned twice");
+      Row2p2 r;
+      r.w = w;
+      r.window = e.window;
+      r.targets.assign(e.targets, e.targets + e.n_targets);
+      plan2.emplace(w, std::move(r));
+    }
     std::vector<PairRow> out;
     std::vector<double> packed;
     std::function<void(size_t, size_t)> on_window;
@@ -644,11 +716,30 @@ int hlbl_run_chain_each_window(hlbl_contractor* c, const hlbl_schedule* s, uint6
         if (fn(user, packed.data(), (int)n) != 0)
           throw Error("hlbl_run_chain: the window callback failed");
       };
-    run_chain(c, s->steps, seed, xunit, residency == 1, form, batch, out, stats, on_window);
+    std::function<void(std::vector<Row2p2>&)> on_2p2;
+    if (fn2)
+      on_2p2 = [&](std::vector<Row2p2>& rs) {
+        std::vector<hlbl_source_2p2> v(rs.size());
+        for (size_t i = 0; i < rs.size(); ++i)
+          v[i] = {{rs[i].w[0], rs[i].w[1], rs[i].w[2], rs[i].w[3]}, chain, rs[i].window, rs[i].p1.data(),
+                  rs[i].targets.data(), (int)rs[i].targets.size(), rs[i].p234.data()};
+        if (fn2(user2, v.data(), (int)v.size()) != 0)
+          throw Error("hlbl_run_chain_2p2: the 2+2 callback failed");
+      };
+    run_chain(c, s->steps, seed, xunit, residency == 1, form, batch, out, stats, on_window, plan2, iflavor,
+              on_2p2);
     if ((int)out.size() > max_rows) throw Error("hlbl_run_chain: rows buffer too small");
     if (!out.empty()) check_ptr(rows, "rows");
     for (size_t i = 0; i < out.size(); ++i) pack_row(out[i], chain, rows + 16 * i);
   });
+}
+
+int hlbl_run_chain_each_window(hlbl_contractor* c, const hlbl_schedule* s, uint64_t seed,
+                               double xunit, int residency, int form, int batch, int chain,
+                               hlbl_window_rows_fn fn, void* user, double* rows, int max_rows,
+                               double stats[5]) {
+  return hlbl_run_chain_2p2(c, s, seed, xunit, residency, form, batch, chain, fn, user, nullptr, 0, 0,
+                            nullptr, nullptr, rows, max_rows, stats);
 }
 
 int hlbl_run_chain(hlbl_contractor* c, const hlbl_schedule* s, uint64_t seed, double xunit,

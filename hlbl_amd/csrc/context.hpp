@@ -85,7 +85,7 @@ constexpr int kMlpIn = 3;
 constexpr int kMlpHidden = 128;
 constexpr int kMlpLayers = 4;  // hidden layers
 constexpr int kMlpOut = qed::kNChUsed;
-// parameter blocks: W0[128][3] b0[128], W1..W3[128][128] b[128], W4[31][128] b4[31]
+// parameter blocks: W0[128][3] b0[128], W1..W3[128][128] b[128], W4[42][128] b4[42]
 constexpr int kMlpParams = kMlpHidden * kMlpIn + kMlpHidden +
                            (kMlpLayers - 1) * (kMlpHidden * kMlpHidden + kMlpHidden) +
                            kMlpOut * kMlpHidden + kMlpOut;
@@ -128,6 +128,9 @@ struct hlbl_contractor {
   hlbl::DevBuf<double> kcst;    // the pair constants L0(0,y), L0(y,0) [2][384]
   hlbl::DevBuf<double> part2;   // pass-2 partials [nwg2][12]
   hlbl::DevBuf<double> pi2p2;   // the 2+2 pieces' Pi_{mu nu}(z) [V][16]
+  hlbl::DevBuf<double> part234; // P2..P4 workgroup partials [workgroups][320]
+  hlbl::DevBuf<double> p1buf;   // the 2+2 step of a run: P1 [4][4][4][Lmax]
+  hlbl::DevBuf<double> p234buf; //   and P2..P4 [targets][320]
   hlbl::DevBuf<double> result;  // [2]
   hlbl::PinBuf<double> host_result;
   hlbl::WindowCache* window = nullptr;

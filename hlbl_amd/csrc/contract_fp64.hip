@@ -93,6 +93,30 @@ __global__ void form_gdz(const double* __restrict__ dz, double* __restrict__ gdz
   gdz[(size_t)i * 2 + 1] = im;
 }
 
+// One flavour straight from and to the caller's arrays (the seam-2 insertion):
+// g_dzu[j][nu][a][b] (j < 6) and g_dzsu[j-6][nu][a][b] from dzu[j][a][.], dzsu[j-6][a][.],
+// the same arithmetic as form_gdz.
+__global__ void form_gdz_one(const double* __restrict__ dzu, const double* __restrict__ dzsu,
+                             double* __restrict__ g_dzu, double* __restrict__ g_dzsu) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (j, nu, a, b)
+  if (i >= 10 * 4 * 144) return;
+  const int b = i % 12, a = (i / 12) % 12, nu = (i / 144) % 4, j = i / 576;
+  const int s = b / 3, colour = b % 3;
+  const int sc = gamma_col(nu, s);
+  const double* src = j < 6 ? dzu + (size_t)j * 288 : dzsu + (size_t)(j - 6) * 288;
+  const double* v = src + ((size_t)a * 12 + 3 * sc + colour) * 2;
+  double re = v[0], im = v[1];
+  if (sc >= 2) {
+    re = -re;
+    im = -im;
+  }
+  mul_ipow(re, im, gamma_phase(nu, s));
+  double* dst = j < 6 ? g_dzu + (size_t)j * 4 * 288 : g_dzsu + (size_t)(j - 6) * 4 * 288;
+  const size_t o = (((size_t)nu * 12 + a) * 12 + b) * 2;
+  dst[o] = re;
+  dst[o + 1] = im;
+}
+
 // B operand of the pass-2 traces: bmat[f][r*24 + j][col] = bmat_value (pad columns 0)
 __global__ void form_bmat(const double* __restrict__ gdz, double* __restrict__ bmat) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (f, kidx, col)
@@ -296,6 +320,11 @@ void launch_forms_batched(const double* xi, double* dz, double* gdz, int n_pairs
 void launch_gamma(const double* dz, double* gdz, hipStream_t st) {
   hipLaunchKernelGGL(form_gdz, dim3(div_up(2 * 10 * 4 * 144, 256), 1), dim3(256), 0, st, dz, gdz);
   check_hip(hipGetLastError(), "form_gdz launch");
+}
+
+void launch_gamma_one(const double* dzu, const double* dzsu, double* g_dzu, double* g_dzsu, hipStream_t st) {
+  hipLaunchKernelGGL(form_gdz_one, dim3(div_up(10 * 4 * 144, 256)), dim3(256), 0, st, dzu, dzsu, g_dzu, g_dzsu);
+  check_hip(hipGetLastError(), "form_gdz_one launch");
 }
 
 void launch_bmat(const double* gdz, double* bmat, hipStream_t st) {

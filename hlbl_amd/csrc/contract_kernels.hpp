@@ -17,6 +17,9 @@ void launch_pass1(hlbl_contractor* c, const double* S, const double* Y, hlbl_coo
                   hipStream_t st);
 // gdz = gamma_nu gamma_5 dz, both flavours, reference layout per flavour
 void launch_gamma(const double* dz, double* gdz, hipStream_t st);
+// one flavour, from the caller's dzu [6][12][12] / dzsu [4][12][12] into g_dzu [6][4][12][12] /
+// g_dzsu [4][4][12][12] (complex)
+void launch_gamma_one(const double* dzu, const double* dzsu, double* g_dzu, double* g_dzsu, hipStream_t st);
 // pass-2 B operands from gdz
 void launch_bmat(const double* gdz, double* bmat, hipStream_t st);
 // pass 2; out[f] (or out[0] with single_flavour) = or += the kernel sum
@@ -39,7 +42,10 @@ void launch_p234_2p2(hlbl_contractor* c, const double* pi, hlbl_coord gsw, const
 void launch_fill_random(double* d_out, hlbl_geom local, hlbl_coord proc, uint64_t seed,
                         hipStream_t st);
 
-// QED kernel at n argument pairs (test entry) and channel interpolation
+// QED kernel at n argument pairs (test entry) and channel interpolation; the point
+// kernel with the caller's scratch of n * 3 * 384 doubles
+void launch_qed_points(const double* table, const double* x, const double* y, int n, double* kerv,
+                       double* scratch, hipStream_t st);
 void launch_qed_kernel(const hlbl_qed* q, int mode, const double* x, const double* y, int n,
                        double* kerv, hipStream_t st);
 void launch_qed_channels(const hlbl_qed* q, int mode, const double* acb, int n, double* out,

@@ -13,6 +13,7 @@
 #include "../../include/hlbl_amd.h"
 #include "context.hpp"
 #include "contract_kernels.hpp"
+#include "driver.hpp"
 #include "schedule.hpp"
 
 namespace hlbl {
@@ -115,21 +116,14 @@ Vec4 min_image(const Vec4& a, const Vec4& b, const hlbl_geom& g) {
 
 }  // namespace
 
-struct PairRow {
-  Vec4 x, y, yv;
-  int window;
-  double ks[2];
-};
 
-// rows in the reference's order: windows, new sources in solve order, partners;
-// on_window(first, n) after each window with its rows [first, first + n), as the
-// reference writes each window's rows when it is done (src/driver.cpp:134-137)
-// batch 0: one pair per call; 1: one call per window (its sources solved first);
-// 2: one call per new source with all its partners (the solve order and cache peak of
-// batch 0, for lattices where a whole window's blocks do not fit)
+
+
 void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xunit, bool host_tier,
                int form, int batch, std::vector<PairRow>& rows, double stats[5],
-               const std::function<void(size_t, size_t)>& on_window) {
+               const std::function<void(size_t, size_t)>& on_window,
+               const std::map<Vec4, Row2p2>& plan_2p2, int iflavor_2p2,
+               const std::function<void(std::vector<Row2p2>&)>& on_2p2) {
   if (c->local.T != c->global.T || c->local.LX != c->global.LX || c->local.LY != c->global.LY ||
       c->local.LZ != c->global.LZ)
     throw Error("hlbl_run_chain: one process holds the whole lattice here");
@@ -147,16 +141,41 @@ void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xuni
   int solved = 0;
   using clk = std::chrono::steady_clock;
 
-  auto solve = [&](const Vec4& p) {
+  std::vector<Row2p2> window_2p2;  // the current window's 2+2 rows
+  const int lmax = std::max(std::max(c->global.T, c->global.LX), std::max(c->global.LY, c->global.LZ));
+  auto pieces_2p2 = [&](int window, const Vec4& site, const double* d_fwd) {
+    const auto it = plan_2p2.find(site);
+    if (it == plan_2p2.end() || it->second.window != window) return;
+    Row2p2 r = it->second;
+    const int nt = (int)r.targets.size();
+    c->p1buf.ensure((size_t)64 * lmax);
+    c->p234buf.ensure((size_t)std::max(1, nt) * 320);
+    launch_p1_2p2(c, d_fwd, iflavor_2p2, to_c(site), c->p1buf.p, stm);
+    if (nt > 0) launch_p234_2p2(c, c->pi2p2.p, to_c(site), r.targets.data(), nt, xunit, c->p234buf.p, stm);
+    r.p1.resize((size_t)64 * lmax);
+    r.p234.resize((size_t)nt * 320);
+    check_hip(hipMemcpyAsync(r.p1.data(), c->p1buf.p, r.p1.size() * sizeof(double), hipMemcpyDeviceToHost, stm),
+              "copy P1");
+    if (nt > 0)
+      check_hip(hipMemcpyAsync(r.p234.data(), c->p234buf.p, r.p234.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, stm), "copy P234");
+    check_hip(hipStreamSynchronize(stm), "2+2 pieces");
+    window_2p2.push_back(std::move(r));
+  };
+
+  auto solve = [&](int window, const Vec4& p) {
     const auto t0 = clk::now();
     double* dst = cache.begin_insert(p);
     const Vec4 site = wrap4(p, c->global);
     const uint64_t sd = hlbl_source_seed(seed, to_c(site));
     launch_fill_random(dst, c->local, c->proc, sd, stm);
     check_hip(hipStreamSynchronize(stm), "solve");
-    cache.end_insert(p, stm);
     ++solved;
     t_solve += std::chrono::duration<double>(clk::now() - t0).count();
+    // 2+2 before the block is stored: dst is the solve target (the cache entry, or
+    // the host tier's device scratch block), as src/driver.cpp:88-93
+    pieces_2p2(window, site, dst);
+    cache.end_insert(p, stm);
   };
 
   for (const Step& st : s) {
@@ -164,7 +183,7 @@ void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xuni
     if (batch == 2) {
       if (host_tier) throw Error("hlbl_run_chain: source batches read every block in place (device tier)");
       for (const Source& src : st.sources) {
-        solve(src.position);
+        solve(st.index, src.position);
         if (src.partners.empty()) continue;
         const double* d_y = cache.device_ptr(src.position, 0, stm);
         const size_t first = rows.size();
@@ -187,7 +206,7 @@ void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xuni
       }
     } else if (batch == 0) {
       for (const Source& src : st.sources) {
-        solve(src.position);
+        solve(st.index, src.position);
         if (src.partners.empty()) continue;
         const double* d_y = cache.device_ptr(src.position, 0, stm);
         for (const Vec4& q : src.partners) {
@@ -210,7 +229,7 @@ void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xuni
     } else {
       // the whole window: solve its sources, contract its pairs in one call
       if (host_tier) throw Error("hlbl_run_chain: window batches read every block in place (device tier)");
-      for (const Source& src : st.sources) solve(src.position);
+      for (const Source& src : st.sources) solve(st.index, src.position);
       std::vector<hlbl_pair_desc> descs;
       std::vector<std::pair<Vec4, Vec4>> done;
       const size_t first = rows.size();
@@ -236,6 +255,10 @@ void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xuni
       for (const auto& d : done) cache.pair_done(d.first, d.second);
     }
     if (on_window) on_window(window_first, rows.size() - window_first);
+    if (!window_2p2.empty()) {
+      if (on_2p2) on_2p2(window_2p2);
+      window_2p2.clear();
+    }
   }
   if (cache.size() != 0) throw Error("run_chain: propagators left in the cache");
   stats[0] = (double)rows.size();

@@ -74,23 +74,24 @@ HLBL_HD void eval_args(int e, const SiteGeom<R>& g, R u[4], R v[4]) {
 // Phase 1: channel ch of evaluation e (interpolated in the table).
 template <class R, class TT>
 __device__ inline R kp_channel(int e, int ch, const SiteGeom<R>& g, const TT* table) {
-  R u[4], v[4], a, c, b;
+  R u[4], v[4], a, be, b;
   eval_args(e, g, u, v);
-  qed::acb(u, v, a, c, b);
-  const qed::Cell<R> cell = qed::locate(a, c, b);
+  qed::grid_coords(u, v, a, be, b);
+  const qed::Cell<R> cell = qed::locate(a, be, b);
   return qed::interp(table, cell, ch);
 }
 
 // Phase 2: the T tensors of evaluation e, components j, 64 + j and 128 + j of lane j
-// (they share their coordinates and invariants) into T.
+// (they share the frame entries) into T, from the 42 frame components ch_e.
 template <class R>
 __device__ inline void kp_tensors(int e, int j, const SiteGeom<R>& g, const R* ch_e, R* T) {
   static_assert(qed::kNT == 3 * 64, "three components per lane");
-  R u[4], v[4];
+  R u[4], v[4], e0[4], e1[4];
   eval_args(e, g, u, v);
-  const qed::TArgs<R> t = qed::t_args(j, u, v);
+  qed::frame(u, v, e0, e1);
+  const qed::TArgs<R> t = qed::t_args(j, e0, e1);
 #pragma unroll
-  for (int A = 0; A < 3; ++A) T[64 * A + j] = qed::t_value(A, t, ch_e);
+  for (int A = 0; A < 3; ++A) T[64 * A + j] = qed::t_value(t, ch_e + qed::kFrame * A);
 }
 
 // v[i] as a blend of the four values: the compiler cannot turn it into a load
@@ -115,7 +116,7 @@ constexpr int kCoefUsed = 40;
 constexpr int kCoefField = 16 * kCoefUsed;  // coefficients per site
 
 // Where the channels of the six site evaluations come from when they are not
-// interpolated: p[e] points at the 32 channels of evaluation e of this site
+// interpolated: p[e] points at the kNCh channels of evaluation e of this site
 // (none: interpolate). Passed by value and read through selects, so that it
 // stays in registers.
 struct ChannelRows {
@@ -135,17 +136,20 @@ struct ChannelRows {
 // A wave computes the 16 x 40 coefficients of one site in its own LDS area, with
 // no workgroup barrier: a wave's LDS operations complete in order, and
 // wave_sync() keeps the compiler from moving accesses across phases.
-//   ch[6][32]  channels of the six evaluations, all fetched first (one latency)
+//   ch[6][kNCh]  channels of the six evaluations, all fetched first (one latency)
 //   T[192]  its T tensors
 //   L[3][384]  L0 slots: 0 = (x, 0), 1 = the partner (0, x) / (0, d), 2 = the main
 //           evaluation, turned in place into K_m (each lane updates its own entries)
 //   C[640]  coefficients; the K_m scatter into the 24 G columns as they complete
 //           (one coefficient per K element), the H columns gather from K3 at the end
 //   D[48]   subtraction matrices of xm, ym, dm
-constexpr int kKwLen = 6 * 32 + qed::kNT + 3 * qed::kNL + kCoefField + 48;
+constexpr int kChS = qed::kNChUsed;  // channel stride of one evaluation in LDS
+constexpr int kKwLen = 6 * kChS + qed::kNT + 3 * qed::kNL + kCoefField + 48;
 // the same for the FP32 window field (kfield_wave_axes, kfield_wave_axis_rows), which
-// keeps the channels of three evaluations: ch[3][32]
-constexpr int kKwLen3 = 3 * 32 + qed::kNT + 3 * qed::kNL + kCoefField + 48;
+// keeps the channels of three evaluations at stride kChS3, ch[3][42], and forms each
+// lane's rows of the subtraction matrices in registers (no D area)
+constexpr int kChS3 = qed::kNChUsed;
+constexpr int kKwLen3 = 3 * kChS3 + qed::kNT + 3 * qed::kNL + kCoefField;
 
 // L0 sparse terms, staged in LDS by the workgroup, one layout per precision.
 // FP32: each output's terms padded to kMaxTerms with zero terms: an output is a fixed
@@ -233,24 +237,25 @@ __device__ inline double l0_lds(const TermLds<double>& tt, int o, const double* 
 __device__ inline int l0_output(const TermLds<float>&, int t) { return t; }
 __device__ inline int l0_output(const TermLds<double>& tt, int t) { return tt.order[t]; }
 
-// the channels of the six site evaluations into ch[6][32]: from the rows when
+// the channels of the six site evaluations into ch[6][kNCh]: from the rows when
 // given, else interpolated in the table
 template <class R, class TT>
 __device__ __forceinline__ void wave_channels(const TT* __restrict__ table, const SiteGeom<R>& g,
                                               const ChannelRows& rows, R* ch) {
   const int lane = threadIdx.x & 63;
+  static_assert(6 * qed::kNChUsed <= 4 * 64, "four rounds of the wave");
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 4; ++k) {
     const int t = lane + 64 * k;
     if (t < 6 * qed::kNChUsed) {
       const int e = t / qed::kNChUsed, c = t - e * qed::kNChUsed;
-      ch[e * 32 + c] = rows.given ? (R)rows.row(e)[c] : kp_channel(e, c, g, table);
+      ch[e * kChS + c] = rows.given ? (R)rows.row(e)[c] : kp_channel(e, c, g, table);
     }
   }
   wave_sync();
 }
 
-// T tensors and L0 of evaluation e (0..7) into dst[384] from its channels ch[32]
+// T tensors and L0 of evaluation e (0..7) into dst[384] from its channels ch[kNCh]
 template <class R>
 __device__ __forceinline__ void wave_l0_from(int e, const TermLds<R>& tt, const SiteGeom<R>& g,
                                              const R* ch, R* T, R* dst) {
@@ -314,19 +319,26 @@ __device__ __forceinline__ void wave_kernel(int m, R* main, const R* l00y, const
 // lane's L0 outputs are o = lane + 64 kk, the same outputs it turns into K_m, so the
 // L0 value stays in a register (wave_l0_from followed by wave_kernel, without the
 // round trip of `main` through LDS and one wave_sync).
+// The lane's rows of the subtraction matrices, D_{a al}(pu) and D_{b al}(pv), are
+// formed from the argument vectors (qed::subtraction_matrix, entry by entry).
 __device__ __forceinline__ void wave_l0_kernel(int e, int m, const TermLds<float>& tt, const SiteGeom<float>& g,
                                                const float* ch, float* T, float* main, const float* l00y,
-                                               const float* l0x0, const float* du, const float* dv, bool in,
+                                               const float* l0x0, const float pu[4], const float pv[4], bool in,
                                                float* C) {
   const int lane = threadIdx.x & 63;
   kp_tensors(e, lane, g, ch, T);
   wave_sync();
   const int c = lane & 3, b = (lane >> 2) & 3, a = (lane >> 4) & 3;
   float dm[4], dn[4];
+  {
+    const float lam = (float)qed::kLambda;
+    const float eu = expf(-lam * norm2(pu) * 0.5f), ev = expf(-lam * norm2(pv) * 0.5f);
+    const float ua = blend4(pu, a), vb = blend4(pv, b);
 #pragma unroll
-  for (int al = 0; al < 4; ++al) {
-    dm[al] = du[a * 4 + al];
-    dn[al] = dv[b * 4 + al];
+    for (int al = 0; al < 4; ++al) {
+      dm[al] = ((a == al ? 1.f : 0.f) - lam * ua * pu[al]) * eu;
+      dn[al] = ((b == al ? 1.f : 0.f) - lam * vb * pv[al]) * ev;
+    }
   }
 #pragma unroll 2
   for (int kk = 0; kk < qed::kNL / 64; ++kk) {
@@ -371,7 +383,7 @@ __device__ __forceinline__ void kfield_wave(const TT* __restrict__ table, const 
                                             const R* cst, R* w, O* __restrict__ out) {
   const int lane = threadIdx.x & 63;
   R* ch = w;
-  R* T = ch + 6 * 32;
+  R* T = ch + 6 * kChS;
   R* L = T + qed::kNT;
   R* C = L + 3 * qed::kNL;
   R* D = C + kCoefField;
@@ -383,14 +395,14 @@ __device__ __forceinline__ void kfield_wave(const TT* __restrict__ table, const 
   const R* Dx = D;
   const R* Dy = D + 16;
   const R* Dd = D + 32;
-  wave_l0_from(1, tt, g, ch + 1 * 32, T, s0);                  // (x, 0)
-  wave_l0_from(0, tt, g, ch + 0 * 32, T, s2);                  // (x, y)
+  wave_l0_from(1, tt, g, ch + 1 * kChS, T, s0);                  // (x, 0)
+  wave_l0_from(0, tt, g, ch + 0 * kChS, T, s2);                  // (x, y)
   wave_kernel(0, s2, cst, s0, Dx, Dy, g.in12, C);              // K1 = L(x, y)
-  wave_l0_from(3, tt, g, ch + 3 * 32, T, s1);                  // (0, x)
-  wave_l0_from(2, tt, g, ch + 2 * 32, T, s2);                  // (y, x)
+  wave_l0_from(3, tt, g, ch + 3 * kChS, T, s1);                  // (0, x)
+  wave_l0_from(2, tt, g, ch + 2 * kChS, T, s2);                  // (y, x)
   wave_kernel(1, s2, s1, cst + qed::kNL, Dy, Dx, g.in12, C);   // K2 = L(y, x)
-  wave_l0_from(5, tt, g, ch + 5 * 32, T, s1);                  // (0, d)
-  wave_l0_from(4, tt, g, ch + 4 * 32, T, s2);                  // (x, d)
+  wave_l0_from(5, tt, g, ch + 5 * kChS, T, s1);                  // (0, d)
+  wave_l0_from(4, tt, g, ch + 4 * kChS, T, s2);                  // (x, d)
   wave_kernel(2, s2, s1, s0, Dx, Dd, g.in3, C);                // K3 = L(x, d)
   for (int t = lane; t < 16 * 16; t += 64) {
     const int row = t >> 4, hc = t & 15;
@@ -410,19 +422,19 @@ __device__ __forceinline__ void kfield_wave_axis_rows(const TT* __restrict__ tab
                                                       const SiteGeom<R>& g, const ChannelRows rows, R* w,
                                                       O* __restrict__ x0, O* __restrict__ ox) {
   const int lane = threadIdx.x & 63;
-  R* ch = w;  // [2][32]: (x, 0), (0, x)
-  R* T = ch + 3 * 32;
+  R* ch = w;  // [2][kChS3]: (x, 0), (0, x)
+  R* T = ch + 3 * kChS3;
   R* s0 = T + qed::kNT;
   if (lane < qed::kNChUsed) {
-    ch[0 * 32 + lane] = rows.given ? (R)rows.row(1)[lane] : kp_channel(1, lane, g, table);
-    ch[1 * 32 + lane] = rows.given ? (R)rows.row(3)[lane] : kp_channel(3, lane, g, table);
+    ch[0 * kChS3 + lane] = rows.given ? (R)rows.row(1)[lane] : kp_channel(1, lane, g, table);
+    ch[1 * kChS3 + lane] = rows.given ? (R)rows.row(3)[lane] : kp_channel(3, lane, g, table);
   }
   wave_sync();
-  wave_l0_from(1, tt, g, ch + 0 * 32, T, s0);  // (x, 0)
+  wave_l0_from(1, tt, g, ch + 0 * kChS3, T, s0);  // (x, 0)
 #pragma unroll
   for (int k = 0; k < qed::kNL / 64; ++k) x0[64 * k + lane] = (O)s0[64 * k + lane];
   wave_sync();
-  wave_l0_from(3, tt, g, ch + 1 * 32, T, s0);  // (0, x)
+  wave_l0_from(3, tt, g, ch + 1 * kChS3, T, s0);  // (0, x)
 #pragma unroll
   for (int k = 0; k < qed::kNL / 64; ++k) ox[64 * k + lane] = (O)s0[64 * k + lane];
   wave_sync();
@@ -439,11 +451,10 @@ __device__ __forceinline__ void kfield_wave_axes(const TT* __restrict__ table, c
                                                  const float* __restrict__ ox, const float* __restrict__ od,
                                                  float* w, O* __restrict__ out) {
   const int lane = threadIdx.x & 63;
-  float* ch = w;  // [3][32]: (x, y), (y, x), (x, d)
-  float* T = ch + 3 * 32;
+  float* ch = w;  // [3][kChS3]: (x, y), (y, x), (x, d)
+  float* T = ch + 3 * kChS3;
   float* L = T + qed::kNT;
   float* C = L + 3 * qed::kNL;
-  float* D = C + kCoefField;
   float* s0 = L;
   float* s1 = L + qed::kNL;
   float* s2 = L + 2 * qed::kNL;
@@ -454,14 +465,13 @@ __device__ __forceinline__ void kfield_wave_axes(const TT* __restrict__ table, c
     rx0[k] = x0[64 * k + lane];
     rox[k] = ox[64 * k + lane];
   }
-  if (lane < 16) kp_dmat(lane, g, D);
   // channels of the evaluations 0 (x, y), 2 (y, x), 4 (x, d)
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int t = lane + 64 * k;
     if (t < 3 * qed::kNChUsed) {
       const int j = t / qed::kNChUsed, c = t - j * qed::kNChUsed, e = 2 * j;
-      ch[j * 32 + c] = rows.given ? rows.row(e)[c] : kp_channel(e, c, g, table);
+      ch[j * kChS3 + c] = rows.given ? rows.row(e)[c] : kp_channel(e, c, g, table);
     }
   }
 #pragma unroll
@@ -470,16 +480,13 @@ __device__ __forceinline__ void kfield_wave_axes(const TT* __restrict__ table, c
     s1[64 * k + lane] = rox[k];
   }
   wave_sync();
-  const float* Dx = D;
-  const float* Dy = D + 16;
-  const float* Dd = D + 32;
-  wave_l0_kernel(0, 0, tt, g, ch + 0 * 32, T, s2, cst, s0, Dx, Dy, g.in12, C);             // K1 = L(x, y)
-  wave_l0_kernel(2, 1, tt, g, ch + 1 * 32, T, s2, s1, cst + qed::kNL, Dy, Dx, g.in12, C);  // K2 = L(y, x)
+  wave_l0_kernel(0, 0, tt, g, ch + 0 * kChS3, T, s2, cst, s0, g.xm, g.ym, g.in12, C);             // K1 = L(x, y)
+  wave_l0_kernel(2, 1, tt, g, ch + 1 * kChS3, T, s2, s1, cst + qed::kNL, g.ym, g.xm, g.in12, C);  // K2 = L(y, x)
   // the (0, d) row into s1, loaded now so that its registers are not live above
 #pragma unroll
   for (int k = 0; k < qed::kNL / 64; ++k) s1[64 * k + lane] = od[64 * k + lane];
   wave_sync();
-  wave_l0_kernel(4, 2, tt, g, ch + 2 * 32, T, s2, s1, s0, Dx, Dd, g.in3, C);               // K3 = L(x, d)
+  wave_l0_kernel(4, 2, tt, g, ch + 2 * kChS3, T, s2, s1, s0, g.xm, g.dm, g.in3, C);               // K3 = L(x, d)
   for (int t = lane; t < 16 * 16; t += 64) {
     const int row = t >> 4, hc = t & 15;
     C[row * kCoefUsed + 24 + hc] = coef_h(row, hc, g, s2);
@@ -496,7 +503,7 @@ template <class R, class TT>
 __device__ __forceinline__ void kfield_wave_constants(const TT* __restrict__ table, const TermLds<R>& tt,
                                                       const SiteGeom<R>& g, R* w, R* cst) {
   R* ch = w;
-  R* T = ch + 6 * 32;
+  R* T = ch + 6 * kChS;
   wave_l0(6, table, tt, g, ch, T, cst);
   wave_l0(7, table, tt, g, ch, T, cst + qed::kNL);
 }

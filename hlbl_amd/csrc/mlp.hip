@@ -1,6 +1,6 @@
-// MLP surrogate of the QED kernel's weight functions: 3 inputs (|x|/R, |y|/R,
-// cos beta), 4 hidden layers of 128 GELU, 31 outputs (the tabulated channels,
-// divided by an envelope and standardised per channel). Weights are stored in
+// MLP surrogate of the tabulated QED kernel: 3 inputs (|x|/R, |y|/R, beta/pi),
+// 4 hidden layers of 128 GELU, 42 outputs (the frame components of the T tensors,
+// standardised per channel). Weights are stored in
 // BF16 and used as BF16 values in every forward pass (the FP32 master copy is
 // what the optimiser updates: quantisation-aware training); products are
 // accumulated in FP32. Training (forward, backward, Adam) and inference run on
@@ -258,29 +258,22 @@ double mlp_loss_grad(hlbl_qed* q, const float* x, const float* t, int n, float* 
 
 // ---------------------------------------------------- training-set and stats --
 
-// envelope dividing the channels: the exponential decay the six weight
-// functions share, exp(3/2 - (sqrt(1+a^2) + sqrt(1+b^2))/2 - sqrt(4+|x-y|^2)/4),
-// |x-y|^2 = a^2 + b^2 - 2abc
-__host__ __device__ inline double mlp_envelope(double a, double b, double c) {
-  const double d2 = a * a + b * b - 2.0 * a * b * c;
-  return exp(1.5 - 0.5 * (sqrt(1.0 + a * a) + sqrt(1.0 + b * b)) - 0.25 * sqrt(4.0 + (d2 > 0 ? d2 : 0.0)));
-}
-
 // counter-based uniform in [0, 1)
 __host__ __device__ inline double mlp_uniform(uint64_t seed, uint64_t i, int k) {
   return (double)(hash64(seed * 0x9E3779B97F4A7C15ULL + i * 3 + k) >> 11) * (1.0 / 9007199254740992.0);
 }
 
-void mlp_sample_inputs(uint64_t seed, size_t i, double radius, double& a, double& c, double& b) {
+// |x|, |y| uniform in [0, radius], cos beta uniform in [-1, 1]; beta = acos(cos beta)
+void mlp_sample_inputs(uint64_t seed, size_t i, double radius, double& a, double& be, double& b) {
   a = radius * mlp_uniform(seed, i, 0);
   b = radius * mlp_uniform(seed, i, 1);
-  c = 2.0 * mlp_uniform(seed, i, 2) - 1.0;
+  be = std::acos(2.0 * mlp_uniform(seed, i, 2) - 1.0);
 }
 
 // Training set of n samples (host threads): the channels of the plain evaluation
 // (the table, interpolated as the plain kernel does) at |x|, |y| uniform in
-// [0, radius], cos beta uniform in [-1, 1]; x [n][3] normalised inputs, t [n][31]
-// channels / envelope.
+// [0, radius], cos beta uniform in [-1, 1]; x [n][3] normalised inputs (|x|/radius,
+// |y|/radius, beta/pi), t [n][42] channels.
 void mlp_build_set(uint64_t seed, int n, double radius, const std::vector<double>& table,
                    std::vector<float>& x, std::vector<double>& t) {
   x.resize((size_t)n * 3);
@@ -293,15 +286,14 @@ void mlp_build_set(uint64_t seed, int n, double radius, const std::vector<double
     pool.emplace_back([&, th] {
       double ch[qed::kNCh];
       for (int i = th; i < n; i += nt) {
-        double a, c, b;
-        mlp_sample_inputs(seed, (size_t)i, radius, a, c, b);
-        const qed::Cell<double> cell = qed::locate(a, c, b);
+        double a, be, b;
+        mlp_sample_inputs(seed, (size_t)i, radius, a, be, b);
+        const qed::Cell<double> cell = qed::locate(a, be, b);
         for (int k = 0; k < kMlpOut; ++k) ch[k] = qed::interp(table.data(), cell, k);
-        const double env = mlp_envelope(a, b, c);
         x[(size_t)i * 3 + 0] = (float)(a / radius);
         x[(size_t)i * 3 + 1] = (float)(b / radius);
-        x[(size_t)i * 3 + 2] = (float)c;
-        for (int k = 0; k < kMlpOut; ++k) t[(size_t)i * kMlpOut + k] = ch[k] / env;
+        x[(size_t)i * 3 + 2] = (float)(be / M_PI);
+        for (int k = 0; k < kMlpOut; ++k) t[(size_t)i * kMlpOut + k] = ch[k];
       }
     });
   for (auto& p : pool) p.join();
@@ -385,7 +377,7 @@ __device__ __forceinline__ void mlp_layer(const unsigned short* __restrict__ w, 
 }
 
 // The wave's 64 rows through the network. Lane r supplies the inputs (x0, x1, x2)
-// of row r; out_fn(row, col, value) receives the 31 outputs. act: the wave's
+// of row r; out_fn(row, col, value) receives the 42 outputs. act: the wave's
 // [64][kActStride] BF16 in LDS. Rows are private to the wave, and a wave's LDS
 // operations complete in order, so no barrier is needed.
 template <class OutFn>
@@ -412,10 +404,11 @@ __device__ __forceinline__ void mlp_wave64(const unsigned short* __restrict__ w1
   mlp_layer<8, false>(w16 + L.w[1], w16 + L.b[1], kMlpHidden, act, out_fn);
   mlp_layer<8, false>(w16 + L.w[2], w16 + L.b[2], kMlpHidden, act, out_fn);
   mlp_layer<8, false>(w16 + L.w[3], w16 + L.b[3], kMlpHidden, act, out_fn);
-  mlp_layer<2, true>(w16 + L.w[4], w16 + L.b[4], kMlpOut, act, out_fn);
+  static_assert(kMlpOut <= 48, "three 16-column output blocks");
+  mlp_layer<3, true>(w16 + L.w[4], w16 + L.b[4], kMlpOut, act, out_fn);
 }
 
-// plain forward of n rows x [n][3] -> y [n][31] (normalised), BF16
+// plain forward of n rows x [n][3] -> y [n][42] (normalised), BF16
 __global__ __launch_bounds__(kInfThreads, 2) void mlp_infer_rows(const unsigned short* __restrict__ w16,
                                                               const float* __restrict__ x, int n,
                                                               float* __restrict__ y) {
@@ -437,8 +430,8 @@ __global__ __launch_bounds__(kInfThreads, 2) void mlp_infer_rows(const unsigned 
   });
 }
 
-// Surrogate channels of a window (gsx = 0), chbuf[block][site][32] =
-// envelope * (mean + sd * net), for the blocks 0: (x, 0) and 1: (0, x), shared
+// Surrogate channels of a window (gsx = 0), chbuf[block][site][kNCh] =
+// mean + sd * net, for the blocks 0: (x, 0) and 1: (0, x), shared
 // by every yv, and 2 + 3u + j: (x, y), (y, x), (x, x - y) of the distinct yv u.
 // Arguments outside the trained domain (|x| or |y| above radius) take the
 // channels from the table instead.
@@ -447,7 +440,7 @@ __global__ __launch_bounds__(kInfThreads, 2) void mlp_surrogate_channels(
     const float* __restrict__ table32, float radius, const hlbl_coord* __restrict__ yvs,
     int n_unique, hlbl_geom global, double xunit, float* __restrict__ chbuf) {
   __shared__ __attribute__((aligned(16))) unsigned short act[kInfThreads / 64][kInfWaveRows * kActStride];
-  __shared__ float s_env[kInfThreads / 64][kInfWaveRows];
+  __shared__ float s_in[kInfThreads / 64][kInfWaveRows];  // 1: inside the trained domain
   __shared__ float s_acb[kInfThreads / 64][kInfWaveRows][3];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const size_t V = geom_volume(global);
@@ -458,7 +451,7 @@ __global__ __launch_bounds__(kInfThreads, 2) void mlp_surrogate_channels(
     const size_t r0 = (size_t)blockIdx.x * kInfRows + (size_t)wave * kInfWaveRows;
     if (r0 >= rows) return;
     const size_t row = r0 + lane;
-    float a = 0.f, c = 0.f, b = 0.f, env = 0.f;
+    float a = 0.f, be = 0.f, b = 0.f, in = 0.f;
     if (row < rows) {
       const int blk = (int)(row / V);
       const size_t site = row - (size_t)blk * V;
@@ -468,21 +461,21 @@ __global__ __launch_bounds__(kInfThreads, 2) void mlp_surrogate_channels(
       site_geom(site, global, global, zero, zero, yvs[u], xunit, g);
       float pu[4], pv[4];
       eval_args(e, g, pu, pv);
-      qed::acb(pu, pv, a, c, b);
-      env = (float)mlp_envelope(a, b, c);
-      if (a > radius || b > radius) env = -1.f;  // outside the trained domain
+      qed::grid_coords(pu, pv, a, be, b);
+      in = (a > radius || b > radius) ? 0.f : 1.f;  // outside the trained domain: table
     }
-    s_env[wave][lane] = env;
+    s_in[wave][lane] = in;
     s_acb[wave][lane][0] = a;
-    s_acb[wave][lane][1] = c;
+    s_acb[wave][lane][1] = be;
     s_acb[wave][lane][2] = b;
-    const float* envw = s_env[wave];
+    const float* inw = s_in[wave];
     const float(*acbw)[3] = s_acb[wave];
-    mlp_wave64(w16, fminf(a * inv_r, 1.f), fminf(b * inv_r, 1.f), c, act[wave], [=](int r, int col, float v) {
+    mlp_wave64(w16, fminf(a * inv_r, 1.f), fminf(b * inv_r, 1.f), be * (float)M_1_PI, act[wave],
+               [=](int r, int col, float v) {
       if (r0 + r >= rows) return;
       float ch;
-      if (envw[r] >= 0.f) {
-        ch = envw[r] * (scale[col] + scale[kMlpOut + col] * v);
+      if (inw[r] > 0.f) {
+        ch = scale[col] + scale[kMlpOut + col] * v;
       } else {
         const qed::Cell<float> cell = qed::locate(acbw[r][0], acbw[r][1], acbw[r][2]);
         ch = qed::interp(table32, cell, col);
@@ -502,15 +495,15 @@ void launch_surrogate_channels(hlbl_contractor* c, WindowPlan& w, hipStream_t st
   check_hip(hipGetLastError(), "mlp_surrogate_channels launch");
 }
 
-// surrogate channels at points (a, cos beta, b): inputs, network, envelope and
-// scale; points outside the trained domain take the table's channels
+// surrogate channels at points (a, beta, b): inputs, network and scale; points
+// outside the trained domain take the table's channels
 __global__ void k_surrogate_inputs(const double* __restrict__ acb, int n, double radius,
                                    float* __restrict__ x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   x[3 * i + 0] = (float)fmin(acb[3 * i + 0] / radius, 1.0);
   x[3 * i + 1] = (float)fmin(acb[3 * i + 2] / radius, 1.0);
-  x[3 * i + 2] = (float)acb[3 * i + 1];
+  x[3 * i + 2] = (float)(acb[3 * i + 1] * M_1_PI);
 }
 __global__ void k_surrogate_outputs(const double* __restrict__ acb, const float* __restrict__ y,
                                     const float* __restrict__ scale, const double* __restrict__ table,
@@ -523,8 +516,7 @@ __global__ void k_surrogate_outputs(const double* __restrict__ acb, const float*
     out[i] = qed::interp(table, qed::locate(a, cb, b), c);
     return;
   }
-  const double env = mlp_envelope(a, b, cb);
-  out[i] = env * ((double)scale[c] + (double)scale[kMlpOut + c] * (double)y[i]);
+  out[i] = (double)scale[c] + (double)scale[kMlpOut + c] * (double)y[i];
 }
 
 void launch_surrogate_points(hlbl_qed* q, const double* acb, int n, double* out, hipStream_t st) {
@@ -592,7 +584,7 @@ void mlp_train(hlbl_qed* q, int n_samples, int epochs, int batch, float lr, uint
   std::vector<float> x;
   std::vector<double> t;
   mlp_build_set(seed, n_samples, q->mlp_radius, q->host_table, x, t);
-  // per-channel standardisation of the enveloped targets
+  // per-channel standardisation of the targets
   std::vector<float> sc(2 * kMlpOut);
   std::vector<float> tn((size_t)n_samples * kMlpOut);
   for (int k = 0; k < kMlpOut; ++k) {

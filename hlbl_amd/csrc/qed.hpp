@@ -2,59 +2,49 @@
 // Lambda = 0.4, arguments in muon units, as used by ker_4pt_contraction
 // (kernels/cuda_lattice.cu:266-275, 403-476) through KQED's QED_Mkernel_L2.
 //
-// Structure (position-space Mainz kernel):
 //   L0_{k mu nu lam}(x,y) = sum_{A, al, be, de} G^A_{k mu nu lam; al be de} T^A_{al be de}(x,y)
 // with k the antisymmetric pair (rho, sigma) and
 //   G^I   = 1/8 Tr(P_{de rho sigma} g_mu g_al g_nu g_be g_lam)
 //   G^II  = -1/4 Tr(P_{de rho sigma} g_mu g_al g_nu) delta_{be lam}
 //   G^III = -1/4 Tr(P_{de rho sigma} g_nu g_be g_lam) delta_{al mu}
 //   P_{de rho sigma} = g_de [g_rho, g_sigma] + 2 (delta_{sigma de} g_rho - delta_{rho de} g_sigma)
-//   T^I_{al be de}   = d^x_al (d^x_be + d^y_be) V_de
-//   T^II_{al be de}  = d^x_al (T_{be de} + delta_{be de} S / 4)
-//   T^III_{al be de} = (d^x_be + d^y_be)(T_{al de} + delta_{al de} S / 4)
-//   S = g0,  V_de = x_de g1 + y_de g2,
-//   T_{al be} = (x_al x_be - delta x^2/4) l1 + (y_al y_be - delta y^2/4) l2
-//             + (x_al y_be + y_al x_be - delta x.y/2) l3
-// and the Lambda subtraction, which makes the kernel vanish at x = 0 and y = 0:
+// T^I, T^II, T^III the tensors of qed_exact.hpp (computed there from the defining
+// integrals, tabulated on the grid of qed_grid.hpp), and the Lambda subtraction,
+// which makes the kernel vanish at x = 0 and y = 0:
 //   L(x,y) = L0(x,y) - d^x_mu(x_al e^{-Lambda x^2/2}) L0_{k al nu lam}(0,y)
 //                    - d^y_nu(y_al e^{-Lambda y^2/2}) L0_{k mu al lam}(x,0)
 // The kernel is 0 when |x| or |y| exceeds kRadius (12 fm).
 //
-// The scalar weight functions g0, g1, g2, l1, l2, l3 of (u, v, w) = (x^2, y^2, x.y)
-// are tabulated with the derivatives the tensors need, on a grid in
-// (|x|, cos beta, |y|), and the plain kernel interpolates the table trilinearly.
-// Functions of (u, v, w) keep every derivative regular at x = 0 and y = 0.
+// The table holds the T tensors in the frame of the argument pair: e0 = x^,
+// e1 = the unit part of y orthogonal to x, P = the projector on the plane
+// orthogonal to both; per tensor 14 components (channel A*14 + c): t_{ijk},
+// i,j,k in {0,1}, at c = (i*2+j)*2+k, and t_{PPd}, t_{PdP}, t_{dPP} at 8+d, 10+d, 12+d:
+//   T_{ijk} = sum t_{abd} e_a,i e_b,j e_d,k
+//           + sum_d (t_{PPd} P_ij e_d,k + t_{PdP} P_ik e_d,j + t_{dPP} e_d,i P_jk).
+// The components are interpolated cubically in (|x|, beta, sqrt|y|), beta the
+// angle between x and y.
 #pragma once
 
 #include <cmath>
 
 #include "lattice.hpp"
+#include "qed_grid.hpp"
 
 namespace hlbl {
 namespace qed {
 
 constexpr double kLambda = 0.4;
-constexpr double kHbarC = 197.3269804;        // MeV fm
-constexpr double kMuonMass = 105.6583745;     // MeV
-constexpr double kRadiusFm = 12.0;
-constexpr double kRadius = kRadiusFm * kMuonMass / kHbarC;  // 6.4253... muon units
+constexpr double kHbarC = 197.3269804;     // MeV fm
+constexpr double kMuonMass = 105.6583745;  // MeV
+constexpr double kRadius = qed_grid::kRadius;  // 12 fm in muon units (6.4253...)
 
-// table grid
-constexpr int kNA = 49;   // |x| in [0, kRadius]
-constexpr int kNC = 25;   // cos beta in [-1, 1]
-constexpr int kNB = 49;   // |y| in [0, kRadius]
-constexpr int kNCh = 32;  // channels per grid point (31 used, 1 pad)
-constexpr int kNChUsed = 31;
-
-// channels
-enum Channel {
-  G0U = 0, G0V, G0W,
-  L1 = 3, L1U, L1V, L1W,
-  L2 = 7, L2U, L2V, L2W,
-  L3 = 11, L3U, L3V, L3W,
-  G1U = 15, G1V, G1W, G1UU, G1UV, G1UW, G1VW, G1WW,
-  G2U = 23, G2V, G2W, G2UU, G2UV, G2UW, G2VW, G2WW,
-};
+// table grid (qed_grid.hpp)
+constexpr int kNA = qed_grid::kNA;        // |x|
+constexpr int kNC = qed_grid::kNBeta;     // beta
+constexpr int kNB = qed_grid::kNB;        // |y|, clustered at 0
+constexpr int kNChUsed = qed_grid::kNChan;  // 42 frame components
+constexpr int kNCh = 48;                  // channel stride of a grid point (6 zero pad)
+constexpr int kFrame = 14;                // components per tensor
 
 constexpr int kNT = 192;   // T^I, T^II, T^III: 3 x 64 components (al*4+be)*4+de
 constexpr int kNL = 384;   // L: [6][4][4][4]
@@ -76,60 +66,85 @@ HLBL_HD int term_coef(unsigned short t) { return (int)(signed char)(t >> 8); }
 
 // ---- table interpolation --------------------------------------------------
 
+// Cubic Lagrange weights of the four nodes i0..i0+3 around the position p (in node
+// units) on a grid of n nodes: i0 = floor(p) - 1, clamped to [0, n-4].
+template <class R>
+HLBL_HD int lagrange4(R p, int n, R w[4]) {
+  int i0 = (int)floor(p) - 1;
+  i0 = i0 < 0 ? 0 : (i0 > n - 4 ? n - 4 : i0);
+  const R t = p - R(i0);
+  const R t1 = t - R(1), t2 = t - R(2), t3 = t - R(3);
+  w[0] = -t1 * t2 * t3 * R(1.0 / 6.0);
+  w[1] = t * t2 * t3 * R(0.5);
+  w[2] = -t * t1 * t3 * R(0.5);
+  w[3] = t * t1 * t2 * R(1.0 / 6.0);
+  return i0;
+}
+
 template <class R>
 struct Cell {
-  int base;  // flat index of the (ia, ic, ib) corner, in grid points
-  R fa, fc, fb;
+  int base;  // flat index of the first stencil node (ia0, ic0, ib0), in grid points
+  R wa[4], wc[4], wb[4];
 };
 
-// Locate (a, c, b) on the grid. a, b >= 0 (clamped to the last cell), c in [-1, 1].
+// Stencil of the grid coordinates (a, beta, b): a, b in [0, kRadius] (clamped),
+// beta in [0, pi].
 template <class R>
-HLBL_HD Cell<R> locate(R a, R c, R b) {
-  const R ha = R(kNA - 1) / R(kRadius);
-  const R hb = R(kNB - 1) / R(kRadius);
-  const R hc = R(kNC - 1) / R(2);
-  R ta = a * ha, tb = b * hb, tc = (c + R(1)) * hc;
-  int ia = (int)floor(ta), ib = (int)floor(tb), ic = (int)floor(tc);
-  ia = ia < 0 ? 0 : (ia > kNA - 2 ? kNA - 2 : ia);
-  ib = ib < 0 ? 0 : (ib > kNB - 2 ? kNB - 2 : ib);
-  ic = ic < 0 ? 0 : (ic > kNC - 2 ? kNC - 2 : ic);
+HLBL_HD Cell<R> locate(R a, R beta, R b) {
+  const R ra = R(kNA - 1) / R(kRadius);
+  const R rc = R(kNC - 1) / R(3.14159265358979323846);
+  const R rb = R(1) / R(kRadius);
+  a = a < R(0) ? R(0) : (a > R(kRadius) ? R(kRadius) : a);
+  b = b < R(0) ? R(0) : (b > R(kRadius) ? R(kRadius) : b);
   Cell<R> cell;
+  const int ia = lagrange4<R>(a * ra, kNA, cell.wa);
+  const int ic = lagrange4<R>(beta * rc, kNC, cell.wc);
+  const int ib = lagrange4<R>(R(sqrt(b * rb)) * R(kNB - 1), kNB, cell.wb);
   cell.base = (ia * kNC + ic) * kNB + ib;
-  cell.fa = ta - R(ia);
-  cell.fc = tc - R(ic);
-  cell.fb = tb - R(ib);
   return cell;
 }
 
-// Trilinear interpolation of channel ch. table is [kNA][kNC][kNB][kNCh].
+// Tricubic interpolation of channel ch. table is [kNA][kNC][kNB][kNCh].
 template <class R, class TT>
 HLBL_HD R interp(const TT* __restrict__ table, const Cell<R>& q, int ch) {
   const int sa = kNC * kNB * kNCh, sc = kNB * kNCh, sb = kNCh;
-  const TT* p = table + (size_t)q.base * kNCh + ch;
-  R v000 = (R)p[0], v001 = (R)p[sb], v010 = (R)p[sc], v011 = (R)p[sc + sb];
-  R v100 = (R)p[sa], v101 = (R)p[sa + sb], v110 = (R)p[sa + sc], v111 = (R)p[sa + sc + sb];
-  R v00 = v000 + q.fb * (v001 - v000);
-  R v01 = v010 + q.fb * (v011 - v010);
-  R v10 = v100 + q.fb * (v101 - v100);
-  R v11 = v110 + q.fb * (v111 - v110);
-  R v0 = v00 + q.fc * (v01 - v00);
-  R v1 = v10 + q.fc * (v11 - v10);
-  return v0 + q.fa * (v1 - v0);
+  const TT* p0 = table + (size_t)q.base * kNCh + ch;
+  R r = R(0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    R ri = R(0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const TT* p = p0 + i * sa + j * sc;
+      const R s = q.wb[0] * (R)p[0] + q.wb[1] * (R)p[sb] + q.wb[2] * (R)p[2 * sb] + q.wb[3] * (R)p[3 * sb];
+      ri += q.wc[j] * s;
+    }
+    r += q.wa[i] * ri;
+  }
+  return r;
 }
 
-// (|x|, cos beta, |y|) of an argument pair; cos beta = 0 when either is 0
+// (|x|, beta, |y|) of an argument pair, beta = atan2(|x ^ y|, x.y) (accurate for
+// nearly parallel arguments); beta = 0 when either argument is 0
 template <class R>
-HLBL_HD void acb(const R x[4], const R y[4], R& a, R& c, R& b) {
-  R u = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
-  R v = y[0] * y[0] + y[1] * y[1] + y[2] * y[2] + y[3] * y[3];
-  R w = x[0] * y[0] + x[1] * y[1] + x[2] * y[2] + x[3] * y[3];
+HLBL_HD void grid_coords(const R x[4], const R y[4], R& a, R& beta, R& b) {
+  const R u = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
+  const R v = y[0] * y[0] + y[1] * y[1] + y[2] * y[2] + y[3] * y[3];
+  const R w = x[0] * y[0] + x[1] * y[1] + x[2] * y[2] + x[3] * y[3];
+  R s2 = R(0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = i + 1; j < 4; ++j) {
+      const R m = x[i] * y[j] - x[j] * y[i];
+      s2 += m * m;
+    }
   a = sqrt(u);
   b = sqrt(v);
-  c = (a > R(0) && b > R(0)) ? w / (a * b) : R(0);
-  c = c > R(1) ? R(1) : (c < R(-1) ? R(-1) : c);
+  beta = (u > R(0) && v > R(0)) ? atan2(sqrt(s2), w) : R(0);
 }
 
-// ---- T tensors -------------------------------------------------------------
+// ---- T tensors from the frame components ------------------------------------
 
 // v[i] for a runtime i without indexing the array (keeps v in registers)
 template <class R>
@@ -137,87 +152,111 @@ HLBL_HD R pick4(const R v[4], int i) {
   return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3]));
 }
 
-// d^x_al F = 2 F_u x_al + F_w y_al
+// The frame of an argument pair: e0 = x^ (y^ when x = 0, axis 0 when both are 0),
+// e1 = the unit part of y orthogonal to e0, or, when y is parallel to e0 (|x ^ y|
+// <= 1e-12 |x| |y|) or 0, the unit part orthogonal to e0 of the axis on which e0
+// is smallest (the first such axis).
 template <class R>
-HLBL_HD R dx(R fu, R fw, R xa, R ya) {
-  return R(2) * fu * xa + fw * ya;
-}
-// (d^x_be + d^y_be) F = (2 F_u + F_w) x_be + (2 F_v + F_w) y_be
-template <class R>
-HLBL_HD R dsum(R fu, R fv, R fw, R xb, R yb) {
-  return (R(2) * fu + fw) * xb + (R(2) * fv + fw) * yb;
-}
-// d^x_al (d^x_be + d^y_be) F, from F's channels starting at base (U V W UU UV UW VW WW)
-template <class R>
-HLBL_HD R dxdsum(const R* f, R xa, R ya, R xb, R yb, bool ab) {
-  R fu = f[0], fw = f[2], fuu = f[3], fuv = f[4], fuw = f[5], fvw = f[6], fww = f[7];
-  R r = ab ? (R(2) * fu + fw) : R(0);
-  r += (R(2) * (R(2) * fuu + fuw) * xa + (R(2) * fuw + fww) * ya) * xb;
-  r += (R(2) * (R(2) * fuv + fuw) * xa + (R(2) * fvw + fww) * ya) * yb;
-  return r;
+HLBL_HD void frame(const R x[4], const R y[4], R e0[4], R e1[4]) {
+  const R u = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
+  const R v = y[0] * y[0] + y[1] * y[1] + y[2] * y[2] + y[3] * y[3];
+  if (u > R(0)) {
+    const R inv = R(1) / sqrt(u);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) e0[i] = x[i] * inv;
+  } else if (v > R(0)) {
+    const R inv = R(1) / sqrt(v);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) e0[i] = y[i] * inv;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) e0[i] = i == 0 ? R(1) : R(0);
+  }
+  R s2 = R(0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = i + 1; j < 4; ++j) {
+      const R m = x[i] * y[j] - x[j] * y[i];
+      s2 += m * m;
+    }
+  const bool parallel = !(u > R(0) && v > R(0)) || s2 <= R(1e-24) * u * v;
+  R c = R(0);
+  R t[4];
+  if (!parallel) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c += y[i] * e0[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) t[i] = y[i] - c * e0[i];
+  } else {
+    int m = 0;
+    R am = fabs(e0[0]);
+#pragma unroll
+    for (int i = 1; i < 4; ++i)
+      if (fabs(e0[i]) < am) {
+        am = fabs(e0[i]);
+        m = i;
+      }
+    const R em = pick4(e0, m);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) t[i] = (i == m ? R(1) : R(0)) - em * e0[i];
+  }
+  const R n = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2] + t[3] * t[3]);
+  const R inv = R(1) / n;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) e1[i] = t[i] * inv;
 }
 
-// What the three T components of one index j = (al*4+be)*4+de share at the argument
-// pair (x, y): the picked coordinates, the Kronecker deltas and the invariants.
+// What the three T components of one index j = (al*4+be)*4+de share: the frame
+// vectors' entries at al, be, de and the projector entries.
 template <class R>
 struct TArgs {
-  R xa, xb, xd, ya, yb, yd, dab, dad, dbd, u, v, w;
-  bool ab;
+  R p0[3], p1[3];  // e0 and e1 at al, be, de
+  R Pab, Pad, Pbd;
 };
 template <class R>
-HLBL_HD TArgs<R> t_args(int j, const R x[4], const R y[4]) {
+HLBL_HD TArgs<R> t_args(int j, const R e0[4], const R e1[4]) {
   const int al = (j >> 4) & 3, be = (j >> 2) & 3, de = j & 3;
   TArgs<R> t;
-  t.ab = al == be;
-  t.dab = al == be ? R(1) : R(0);
-  t.dad = al == de ? R(1) : R(0);
-  t.dbd = be == de ? R(1) : R(0);
-  t.xa = pick4(x, al), t.xb = pick4(x, be), t.xd = pick4(x, de);
-  t.ya = pick4(y, al), t.yb = pick4(y, be), t.yd = pick4(y, de);
-  t.u = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
-  t.v = y[0] * y[0] + y[1] * y[1] + y[2] * y[2] + y[3] * y[3];
-  t.w = x[0] * y[0] + x[1] * y[1] + x[2] * y[2] + x[3] * y[3];
+  t.p0[0] = pick4(e0, al), t.p0[1] = pick4(e0, be), t.p0[2] = pick4(e0, de);
+  t.p1[0] = pick4(e1, al), t.p1[1] = pick4(e1, be), t.p1[2] = pick4(e1, de);
+  const R dab = al == be ? R(1) : R(0), dad = al == de ? R(1) : R(0), dbd = be == de ? R(1) : R(0);
+  t.Pab = dab - t.p0[0] * t.p0[1] - t.p1[0] * t.p1[1];
+  t.Pad = dad - t.p0[0] * t.p0[2] - t.p1[0] * t.p1[2];
+  t.Pbd = dbd - t.p0[1] * t.p0[2] - t.p1[1] * t.p1[2];
   return t;
 }
 
-// Component A*64 + j of the T tensors from the channels ch, with t = t_args(j, x, y).
+// Component A*64 + j of the T tensors from the 14 frame components c of tensor A
+// (c = ch + 14 A), with t = t_args(j, e0, e1).
 template <class R>
-HLBL_HD R t_value(int A, const TArgs<R>& t, const R* ch) {
-  const R xa = t.xa, xb = t.xb, xd = t.xd, ya = t.ya, yb = t.yb, yd = t.yd;
-  const R dab = t.dab, dad = t.dad, dbd = t.dbd;
-  if (A == 0) {
-    R r = dbd * (dx(ch[G1U], ch[G1W], xa, ya) + dx(ch[G2U], ch[G2W], xa, ya));
-    r += dad * dsum(ch[G1U], ch[G1V], ch[G1W], xb, yb);
-    r += xd * dxdsum(ch + G1U, xa, ya, xb, yb, t.ab);
-    r += yd * dxdsum(ch + G2U, xa, ya, xb, yb, t.ab);
-    return r;
+HLBL_HD R t_value(const TArgs<R>& t, const R* c) {
+  // sum_{a,b,d} c[(a*2+b)*2+d] E_a[al] E_b[be] E_d[de]
+  R r = R(0);
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const R ea = a == 0 ? t.p0[0] : t.p1[0];
+    R rb = R(0);
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const R eb = b == 0 ? t.p0[1] : t.p1[1];
+      rb += eb * (c[(a * 2 + b) * 2] * t.p0[2] + c[(a * 2 + b) * 2 + 1] * t.p1[2]);
+    }
+    r += ea * rb;
   }
-  const R u = t.u, v = t.v, w = t.w;
-  const R q = R(0.25), h = R(0.5);
-  if (A == 1) {
-    R r = (dab * xd + xb * dad - h * dbd * xa) * ch[L1];
-    r += (xb * xd - q * dbd * u) * dx(ch[L1U], ch[L1W], xa, ya);
-    r += (yb * yd - q * dbd * v) * dx(ch[L2U], ch[L2W], xa, ya);
-    r += (dab * yd + yb * dad - h * dbd * ya) * ch[L3];
-    r += (xb * yd + yb * xd - h * dbd * w) * dx(ch[L3U], ch[L3W], xa, ya);
-    r += q * dbd * dx(ch[G0U], ch[G0W], xa, ya);
-    return r;
-  }
-  R r = (dab * xd + xa * dbd - h * dad * xb) * ch[L1];
-  r += (xa * xd - q * dad * u) * dsum(ch[L1U], ch[L1V], ch[L1W], xb, yb);
-  r += (dab * yd + ya * dbd - h * dad * yb) * ch[L2];
-  r += (ya * yd - q * dad * v) * dsum(ch[L2U], ch[L2V], ch[L2W], xb, yb);
-  r += (dab * (xd + yd) + dbd * (xa + ya) - h * dad * (xb + yb)) * ch[L3];
-  r += (xa * yd + ya * xd - h * dad * w) * dsum(ch[L3U], ch[L3V], ch[L3W], xb, yb);
-  r += q * dad * dsum(ch[G0U], ch[G0V], ch[G0W], xb, yb);
+  r += t.Pab * (c[8] * t.p0[2] + c[9] * t.p1[2]);
+  r += t.Pad * (c[10] * t.p0[1] + c[11] * t.p1[1]);
+  r += t.Pbd * (c[12] * t.p0[0] + c[13] * t.p1[0]);
   return r;
 }
 
-// Component i = A*64 + (al*4+be)*4+de of the T tensors, from the channels ch
+// Component i = A*64 + (al*4+be)*4+de of the T tensors from the channels ch[42]
 // at the argument pair (x, y).
 template <class R>
 HLBL_HD R t_component(int i, const R x[4], const R y[4], const R* ch) {
-  return t_value(i >> 6, t_args(i & 63, x, y), ch);
+  R e0[4], e1[4];
+  frame(x, y, e0, e1);
+  return t_value(t_args(i & 63, e0, e1), ch + kFrame * (i >> 6));
 }
 
 // L0 output o from the T tensors

@@ -1,17 +1,16 @@
-// Host-only parts of the QED kernel: table builders.
+// Host-only parts of the QED kernel.
 #pragma once
 
+#include <string>
 #include <vector>
 
 #include "qed.hpp"
 
 namespace hlbl {
 namespace qed {
-
-// the 31 channels (plus pad) at (u, v, w) = (x^2, y^2, x.y)
-void weight_channels(double u, double v, double w, double ch[kNCh]);
-// the full table [kNA][kNC][kNB][kNCh]
-void build_weight_table(std::vector<double>& table);
-
+// the tabulated T tensors (qed_grid.hpp file format) as [kNA][kNC][kNB][kNCh] doubles
+// (channels kNChUsed..kNCh-1 zero); throws std::runtime_error on a missing, truncated
+// or differently gridded file
+void load_table(const std::string& path, std::vector<double>& table);
 }  // namespace qed
 }  // namespace hlbl

@@ -1,10 +1,12 @@
 // Host side of the QED kernel: the G^A term table from gamma-matrix traces,
-// and the weight-function table.
+// and the reader of the tabulated T tensors.
 #include <cmath>
 #include <complex>
 #include <algorithm>
 #include <cstring>
 #include <stdexcept>
+#include <fstream>
+#include <string>
 #include <vector>
 
 #include "qed.hpp"
@@ -120,141 +122,25 @@ void build_term_table(TermTable* out) {
                    [&](unsigned short x, unsigned short y) { return out->nterm[x] > out->nterm[y]; });
 }
 
-// ---- weight functions ------------------------------------------------------
-// Second-order jet in (u, v, w): value, gradient, Hessian (uu uv uw vv vw ww).
-struct Jet {
-  double f, g[3], h[6];
-};
+// ---- the tabulated T tensors --------------------------------------------------
 
-namespace {
-
-inline int hidx(int i, int j) {
-  if (i > j) std::swap(i, j);
-  static const int t[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-  return t[i][j];
-}
-
-Jet jconst(double c) {
-  Jet r{};
-  r.f = c;
-  return r;
-}
-Jet jvar(double x, int i) {
-  Jet r{};
-  r.f = x;
-  r.g[i] = 1.0;
-  return r;
-}
-Jet operator+(const Jet& a, const Jet& b) {
-  Jet r;
-  r.f = a.f + b.f;
-  for (int i = 0; i < 3; ++i) r.g[i] = a.g[i] + b.g[i];
-  for (int i = 0; i < 6; ++i) r.h[i] = a.h[i] + b.h[i];
-  return r;
-}
-Jet operator*(double s, const Jet& a) {
-  Jet r;
-  r.f = s * a.f;
-  for (int i = 0; i < 3; ++i) r.g[i] = s * a.g[i];
-  for (int i = 0; i < 6; ++i) r.h[i] = s * a.h[i];
-  return r;
-}
-Jet operator+(double s, const Jet& a) {
-  Jet r = a;
-  r.f += s;
-  return r;
-}
-Jet operator*(const Jet& a, const Jet& b) {
-  Jet r;
-  r.f = a.f * b.f;
-  for (int i = 0; i < 3; ++i) r.g[i] = a.f * b.g[i] + a.g[i] * b.f;
-  for (int i = 0; i < 3; ++i)
-    for (int j = i; j < 3; ++j) {
-      int k = hidx(i, j);
-      r.h[k] = a.f * b.h[k] + a.h[k] * b.f + a.g[i] * b.g[j] + a.g[j] * b.g[i];
-    }
-  return r;
-}
-// apply a scalar function with derivatives (f0, f1, f2) at a.f
-Jet chain(const Jet& a, double f0, double f1, double f2) {
-  Jet r;
-  r.f = f0;
-  for (int i = 0; i < 3; ++i) r.g[i] = f1 * a.g[i];
-  for (int i = 0; i < 3; ++i)
-    for (int j = i; j < 3; ++j) {
-      int k = hidx(i, j);
-      r.h[k] = f1 * a.h[k] + f2 * a.g[i] * a.g[j];
-    }
-  return r;
-}
-Jet jexp(const Jet& a) {
-  double e = std::exp(a.f);
-  return chain(a, e, e, e);
-}
-Jet jsqrt(const Jet& a) {
-  double s = std::sqrt(a.f);
-  return chain(a, s, 0.5 / s, -0.25 / (s * a.f));
-}
-Jet jinv(const Jet& a) {
-  double i = 1.0 / a.f;
-  return chain(a, i, -i * i, 2.0 * i * i * i);
-}
-
-}  // namespace
-
-// The weight functions g0, g1, g2, l1, l2, l3 at (u, v, w) = (x^2, y^2, x.y).
-void weight_jets(double u, double v, double w, Jet out[6]) {
-  Jet U = jvar(u, 0), V = jvar(v, 1), W = jvar(w, 2);
-  Jet sx = jsqrt(1.0 + U), sy = jsqrt(1.0 + V);
-  Jet sd = jsqrt(4.0 + (U + V + (-2.0) * W));
-  Jet E = jexp(1.5 + ((-0.5) * (sx + sy) + (-0.25) * sd));
-  out[0] = E * (1.0 + 0.25 * W) * jinv(1.0 + 0.5 * (U + V));         // g0
-  out[1] = 0.5 * E * jinv(sx * sd);                                    // g1
-  out[2] = (-0.3) * E * jinv(sy * sd);                                 // g2
-  out[3] = 0.2 * E * jinv(1.0 + U);                                    // l1
-  out[4] = (-0.15) * E * jinv(1.0 + V);                                // l2
-  out[5] = 0.1 * E * (1.0 + (-0.1) * W) * jinv(sx * sy * sd);         // l3
-}
-
-void weight_channels(double u, double v, double w, double ch[kNCh]) {
-  Jet j[6];
-  weight_jets(u, v, w, j);
-  const Jet &g0 = j[0], &g1 = j[1], &g2 = j[2], &l1 = j[3], &l2 = j[4], &l3 = j[5];
-  for (int i = 0; i < kNCh; ++i) ch[i] = 0.0;
-  ch[G0U] = g0.g[0]; ch[G0V] = g0.g[1]; ch[G0W] = g0.g[2];
-  const Jet* ls[3] = {&l1, &l2, &l3};
-  for (int q = 0; q < 3; ++q) {
-    ch[L1 + 4 * q] = ls[q]->f;
-    ch[L1 + 4 * q + 1] = ls[q]->g[0];
-    ch[L1 + 4 * q + 2] = ls[q]->g[1];
-    ch[L1 + 4 * q + 3] = ls[q]->g[2];
-  }
-  const Jet* gs[2] = {&g1, &g2};
-  for (int q = 0; q < 2; ++q) {
-    double* c = ch + G1U + 8 * q;
-    const Jet& g = *gs[q];
-    c[0] = g.g[0]; c[1] = g.g[1]; c[2] = g.g[2];
-    c[3] = g.h[0];  // uu
-    c[4] = g.h[1];  // uv
-    c[5] = g.h[2];  // uw
-    c[6] = g.h[4];  // vw
-    c[7] = g.h[5];  // ww
-  }
-}
-
-void build_weight_table(std::vector<double>& table) {
-  table.assign((size_t)kNA * kNC * kNB * kNCh, 0.0);
-  for (int ia = 0; ia < kNA; ++ia) {
-    double a = kRadius * ia / (kNA - 1);
-    for (int ic = 0; ic < kNC; ++ic) {
-      double c = -1.0 + 2.0 * ic / (kNC - 1);
-      for (int ib = 0; ib < kNB; ++ib) {
-        double b = kRadius * ib / (kNB - 1);
-        double* p = &table[(((size_t)ia * kNC + ic) * kNB + ib) * kNCh];
-        weight_channels(a * a, b * b, a * b * c, p);
-      }
-    }
-  }
+void load_table(const std::string& path, std::vector<double>& table) {
+  std::ifstream in(path, std::ios::binary);
+  if (!in) throw std::runtime_error("QED kernel table: cannot open " + path);
+  qed_grid::FileHeader h;
+  in.read(reinterpret_cast<char*>(&h), sizeof(h));
+  if (!in || h.magic != qed_grid::kMagic || h.version != qed_grid::kVersion)
+    throw std::runtime_error("QED kernel table: " + path + " is not a version-1 table");
+  if ((int)h.na != kNA || (int)h.nbeta != kNC || (int)h.nb != kNB || (int)h.nch != kNChUsed ||
+      h.radius != kRadius)
+    throw std::runtime_error("QED kernel table: " + path + " has another grid");
+  const size_t npts = (size_t)kNA * kNC * kNB;
+  std::vector<float> raw(npts * kNChUsed);
+  in.read(reinterpret_cast<char*>(raw.data()), (std::streamsize)(raw.size() * sizeof(float)));
+  if (!in) throw std::runtime_error("QED kernel table: " + path + " is truncated");
+  table.assign(npts * kNCh, 0.0);
+  for (size_t p = 0; p < npts; ++p)
+    for (int c = 0; c < kNChUsed; ++c) table[p * kNCh + c] = (double)raw[p * kNChUsed + c];
 }
 
 }  // namespace qed

@@ -107,11 +107,19 @@ void pass1_group_units(const WinPair* pairs, int n_pairs, int* units, int unit_s
     if (seen[u] != 1) throw Error("pass1_group_units: a unit is not listed exactly once");
 }
 
+// Default gate of the surrogate form: a form-2 window whose judge (the largest
+// deviation of the surrogate's coefficients from the plain kernel's at the sampled
+// sites, relative to the site's largest coefficient) exceeds it is recomputed with
+// the plain kernel (form 1).
+constexpr double kDefaultJudgeBound = 1e-3;
+
 struct WindowCache {
   std::vector<std::unique_ptr<WindowPlan>> plans;
   hipStream_t stream = nullptr;
   double last_judge = -1.0;
   int last_judge_n = 0;
+  double judge_bound = kDefaultJudgeBound;
+  int last_fallback = 0;  // the last form-2 window returned form 1's results
   ~WindowCache() {
     plans.clear();
     if (stream) (void)hipStreamDestroy(stream);
@@ -129,6 +137,16 @@ void window_judge_result(const hlbl_contractor* c, double* err, int* n) {
   if (!c->window || c->window->last_judge < 0) throw Error("hlbl_window_judge: no surrogate window has run");
   *err = c->window->last_judge;
   *n = c->window->last_judge_n;
+}
+
+void window_set_judge_bound(hlbl_contractor* c, double bound) {
+  if (!(bound > 0)) throw Error("hlbl_window_set_judge_bound: the bound must be positive");
+  window_cache(c)->judge_bound = bound;
+}
+
+void window_fallback_result(const hlbl_contractor* c, int* fell_back) {
+  if (!c->window || c->window->last_judge < 0) throw Error("hlbl_window_fallback: no surrogate window has run");
+  *fell_back = c->window->last_fallback;
 }
 
 // the window's plan (reused or built) with the pair table filled in
@@ -266,6 +284,19 @@ void window_run(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double x
   check_hip(hipGraphLaunch(w->exec, st), "hipGraphLaunch");
   check_hip(hipStreamSynchronize(st), "window");
   window_finish(c, w, out);
+  if (form == 2) {
+    // the judge as a gate: past the bound, the plain kernel's results instead
+    WindowCache* cache = c->window;
+    cache->last_fallback = cache->last_judge > cache->judge_bound ? 1 : 0;
+    if (cache->last_fallback) {
+      const double judge = cache->last_judge;
+      const int judge_n = cache->last_judge_n;
+      window_run(c, pairs, n, xunit, 1, out, user_stream);
+      cache->last_judge = judge;
+      cache->last_judge_n = judge_n;
+      cache->last_fallback = 1;
+    }
+  }
 }
 
 void window_profile(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit, int form,

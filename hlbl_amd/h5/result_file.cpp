@@ -371,6 +371,10 @@ int hlbl_result_append_rows(void* f, const double* rows, int n) {
   return hlbl_result_append(static_cast<hlbl_result_file*>(f), rows, n);
 }
 
+int hlbl_result_append_2p2_rows(void* f, const hlbl_source_2p2* rows, int n) {
+  return hlbl_result_append_2p2(static_cast<hlbl_result_file*>(f), rows, n);
+}
+
 long long hlbl_result_rows(const hlbl_result_file* f) { return f ? f->rows : -1; }
 
 int hlbl_result_close(hlbl_result_file* f) {

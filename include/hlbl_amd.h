@@ -57,11 +57,23 @@ int hlbl_fill_random_block_device(double* d_out, hlbl_geom local, hlbl_coord pro
 /* ---- QED kernel L^(Lambda), Lambda = 0.4 -----------------------------------
  * Replaces KQED's initialise(&kqed_t) / free_QED_temps (bin/hlbl_hypercube.cpp:175,226)
  * and the device call QED_Mkernel_L2(0.4, x, y, kqed_t, kerv) (kernels/cuda_lattice.cu:266-275).
- * hlbl_qed_init builds the weight-function tables on `device`. */
+ * hlbl_qed_init_file loads the tabulated kernel (the T tensors of the Mainz
+ * representation on a grid in |x|, beta, |y|, written by hlbl_amd/qedgen from the
+ * kernel's defining integrals) from table_path onto `device`; hlbl_qed_init does the
+ * same with the library's own table, data/qed_kernel_table.bin beside
+ * libhlbl_amd.so, or the file the environment variable HLBL_QED_TABLE names. */
 int hlbl_qed_init(hlbl_qed** out, int device);
+int hlbl_qed_init_file(hlbl_qed** out, int device, const char* table_path);
 int hlbl_qed_free(hlbl_qed* q);
-/* weight-function table geometry: channels, points along |x|, cos(beta), |y|, and the
- * outer radius in muon units (the kernel is 0 beyond it) */
+/* The tabulated quantity itself, from the kernel's defining integrals (host, one
+ * thread, no device): the 42 frame components of T^I, T^II, T^III at |x| = a,
+ * |y| = b, angle beta in [0, pi] between x and y, into out[42]. quad = {nt, nr,
+ * ntheta, npsi} quadrature sizes (Feynman-parameter nodes per axis; radial, polar and
+ * azimuthal nodes per centre), or NULL for the table's own (16, 28, 8, 16).
+ * No counterpart in the reference (KQED ships its tables precomputed). */
+int hlbl_qed_exact(double a, double beta, double b, const int* quad, double* out);
+/* table geometry: channel stride of a grid point, points along |x|, beta, |y|, and
+ * the outer radius in muon units (the kernel is 0 beyond it) */
 int hlbl_qed_table_info(const hlbl_qed* q, int* n_channels, int* n_a, int* n_c, int* n_b,
                         double* radius);
 /* copy the FP64 table [n_a][n_c][n_b][n_channels] to host memory */
@@ -74,8 +86,9 @@ int hlbl_qed_table_copy(const hlbl_qed* q, double* host_out, size_t n_doubles);
  * plain outside the trained domain). */
 int hlbl_qed_kernel(hlbl_qed* q, int mode, const double* d_x, const double* d_y, int n,
                     double* d_kerv, hlbl_stream stream);
-/* The interpolated weight-function channels at n points (a=|x|, c=cos beta, b=|y|):
- * d_acb is [n][3], d_out [n][n_channels]. mode as above. */
+/* The interpolated table channels (the 42 frame components of T^I, T^II, T^III,
+ * then zero pad) at n points (a=|x|, beta = the angle between x and y in [0, pi],
+ * b=|y|): d_acb is [n][3], d_out [n][n_channels]. mode as above. */
 int hlbl_qed_channels(hlbl_qed* q, int mode, const double* d_acb, int n, double* d_out,
                       hlbl_stream stream);
 
@@ -157,6 +170,12 @@ int hlbl_window_kernel_sum(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
 /* the surrogate's judge for the last form-2 window: max relative deviation of the
  * surrogate kernel from the plain one over the sampled sites */
 int hlbl_window_judge(const hlbl_contractor* c, double* max_rel_err, int* n_samples);
+/* The judge gates form 2: a window whose judge exceeds `bound` (default 1e-3) is
+ * recomputed with the plain kernel (form 1), and those results are returned;
+ * hlbl_window_fallback tells whether the last form-2 window did so. No counterpart
+ * in the reference (the surrogate is this library's addition). */
+int hlbl_window_set_judge_bound(hlbl_contractor* c, double bound);
+int hlbl_window_fallback(const hlbl_contractor* c, int* fell_back);
 /* Measurement hook (no counterpart in the reference): runs a window of form 1 or 2
  * once without the graph, with HIP events around each stage on the launch stream;
  * ms = {pass 1 incl. its reductions, surrogate channels (0 for form 1),
@@ -229,6 +248,39 @@ int hlbl_run_chain_each_window(hlbl_contractor* c, const hlbl_schedule* s, uint6
                                double xunit, int residency, int form, int batch, int chain,
                                hlbl_window_rows_fn fn, void* user, double* rows, int max_rows,
                                double stats[5]);
+
+/* ---- the 2+2 step of a run (src/driver.cpp:58-95) ----------------------------
+ * Result2p2 (include/hlbl/output.hpp:25-35): one 2+2 source and its targets, host
+ * arrays: p1 [4][4][4][Lmax], p234 [n_targets][5][4][4][4]. */
+typedef struct {
+  hlbl_coord w;
+  int chain, window;
+  const double* p1;
+  const hlbl_coord* targets;
+  int n_targets;
+  const double* p234;
+} hlbl_source_2p2;
+/* A 2+2 source whose pieces this chain computes: its site w (in [0, L)), the window of
+ * its first solve in the run (plan_2p2, src/schedule.cpp:284-317) and its targets. */
+typedef struct {
+  hlbl_coord w;
+  int window;
+  const hlbl_coord* targets;
+  int n_targets;
+} hlbl_plan_2p2;
+typedef int (*hlbl_window_2p2_fn)(void* user, const hlbl_source_2p2* rows, int n);
+/* hlbl_run_chain_each_window with the 2+2 step inside the solve loop: when the chain
+ * solves a planned site in its planned window, P1 and P2..P4 of its targets
+ * (hlbl_pieces_2p2, flavour iflavor) are computed from the solved block before it is
+ * stored (the solve target, as src/driver.cpp:88-93), and fn2(user2, rows, n) receives
+ * each window's 2+2 rows after the window's kernel-sum rows (fn2 may be NULL). A
+ * nonzero return from fn2 stops the run with an error. hlbl_result_append_2p2_rows
+ * of libhlbl_h5.so is such an fn2. */
+int hlbl_run_chain_2p2(hlbl_contractor* c, const hlbl_schedule* s, uint64_t seed, double xunit,
+                       int residency, int form, int batch, int chain, hlbl_window_rows_fn fn,
+                       void* user, const hlbl_plan_2p2* plan, int n_plan, int iflavor,
+                       hlbl_window_2p2_fn fn2, void* user2, double* rows, int max_rows,
+                       double stats[5]);
 
 #ifdef __cplusplus
 }

@@ -43,17 +43,7 @@ typedef struct {
   int with_2p2;  /* write the 2p2 group */
 } hlbl_run_info;
 
-/* Result2p2 (include/hlbl/output.hpp:25-35): one 2+2 source and its targets, host
- * arrays as hlbl_pieces_2p2 gives them after a copy: p1 [4][4][4][Lmax],
- * p234 [n_targets][5][4][4][4]; pair_yv is computed (minimum image of target - w) */
-typedef struct {
-  hlbl_coord w;
-  int chain, window;
-  const double* p1;
-  const hlbl_coord* targets;
-  int n_targets;
-  const double* p234;
-} hlbl_source_2p2;
+/* The 2+2 row type hlbl_source_2p2, Result2p2 of include/hlbl/output.hpp:25-35, is declared in hlbl_amd.h */
 
 typedef struct hlbl_result_file hlbl_result_file;
 
@@ -67,6 +57,8 @@ int hlbl_result_append_rows(void* f, const double* rows, int n);
 /* ResultWriter::append_2p2 (src/output.cpp:249-297): n sources, then a flush; needs
  * with_2p2 */
 int hlbl_result_append_2p2(hlbl_result_file* f, const hlbl_source_2p2* rows, int n);
+/* the same with the argument types of hlbl_window_2p2_fn, for hlbl_run_chain_2p2 */
+int hlbl_result_append_2p2_rows(void* f, const hlbl_source_2p2* rows, int n);
 long long hlbl_result_rows(const hlbl_result_file* f);
 int hlbl_result_close(hlbl_result_file* f);
 const char* hlbl_h5_last_error(void);

@@ -37,8 +37,8 @@ def oracle():
     o.oracle_4pt.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int] + [ctypes.c_void_p] * 5 + \
         [ctypes.c_double, ctypes.c_int]
     o.oracle_kernel_sum.argtypes = [ctypes.c_void_p] * 5 + [ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
-    o.oracle_channels_exact.argtypes = [ctypes.c_double] * 3 + [ctypes.c_void_p]
     o.oracle_channels_interp.argtypes = [ctypes.c_double] * 3 + [ctypes.c_void_p]
+    o.oracle_load_table.argtypes = [ctypes.c_char_p]
     o.oracle_table_size.restype = ctypes.c_size_t
     return o
 

@@ -158,73 +158,76 @@ def test_qed_trace_tensors_textbook(oracle):
     assert np.array_equal(got, want)
 
 
-def channels(oracle, u, v, w):
-    ch = np.zeros(32)
-    oracle.oracle_channels_exact(u, v, w, P(ch))
+R_MUON = 12.0 * 105.6583745 / 197.3269804  # 12 fm in muon units
+
+
+def interp(oracle, a, be, b):
+    ch = np.zeros(42)
+    oracle.oracle_channels_interp(a, be, b, P(ch))
     return ch
 
 
-def test_weight_channel_derivatives(oracle):
-    """The hyper-dual derivatives of the weight functions against central differences of
-    the values (l1, l2, l3 are channels; g0, g1, g2 enter through derivatives only)."""
-    u, v, w = 1.3, 0.7, 0.4
-    h = 1e-5
-    for base in (3, 7, 11):  # l1, l2, l3: (f, fu, fv, fw)
-        for d in range(3):
-            e = np.zeros(3)
-            e[d] = h
-            fp = channels(oracle, *(np.array([u, v, w]) + e))[base]
-            fm = channels(oracle, *(np.array([u, v, w]) - e))[base]
-            assert abs((fp - fm) / (2 * h) - channels(oracle, u, v, w)[base + 1 + d]) < 1e-8
-    # g1, g2: second derivatives from the first ones
-    names = {"uu": (0, 0), "uv": (0, 1), "uw": (0, 2), "vw": (1, 2), "ww": (2, 2)}
-    for base in (15, 23):
-        c0 = channels(oracle, u, v, w)
-        for k, (name, (a, b)) in enumerate(names.items()):
-            e = np.zeros(3)
-            e[b] = h
-            fp = channels(oracle, *(np.array([u, v, w]) + e))[base + a]
-            fm = channels(oracle, *(np.array([u, v, w]) - e))[base + a]
-            assert abs((fp - fm) / (2 * h) - c0[base + 3 + k]) < 1e-7, name
+def exact(a, be, b, quad=None):
+    import hlbl_amd as H
+    return np.array(H.qed_exact(a, be, b, quad))
 
 
-def test_t_tensors_against_finite_differences(oracle):
-    """T^I = d^x_al (d^x_be + d^y_be) V_de, T^II = d^x_al (T_be_de + delta S/4),
-    T^III = (d^x_be + d^y_be)(T_al_de + delta S/4) from exact weight functions."""
-    rng = np.random.default_rng(3)
-    x, y = rng.uniform(-1, 1, 4), rng.uniform(-1, 1, 4)
-    u, v, w = x @ x, y @ y, x @ y
-    ch = channels(oracle, u, v, w)
-    T = np.zeros(192)
-    oracle.oracle_t_tensors(P(x), P(y), P(ch), P(T))
-    T = T.reshape(3, 4, 4, 4)
+def symmetrized(ch):
+    """The O(3)-symmetric part of the frame components at beta = 0 or pi (x parallel to
+    y): t110 = tPP0, t101 = tP0P, t011 = t0PP (means), the odd components 0."""
+    out = ch.copy()
+    for A in range(3):
+        t = out[14 * A:14 * A + 14]
+        for i, j in ((6, 8), (5, 10), (3, 12)):
+            t[i] = t[j] = 0.5 * (t[i] + t[j])
+        t[[1, 2, 4, 7, 9, 11, 13]] = 0
+    return out
 
-    def svt(xx, yy):
-        S = np.zeros(1)
-        V = np.zeros(4)
-        Tt = np.zeros(16)
-        oracle.oracle_svt(P(np.ascontiguousarray(xx)), P(np.ascontiguousarray(yy)), P(S), P(V), P(Tt))
-        return S[0], V, Tt.reshape(4, 4)
 
-    h = 1e-4
+def test_table_nodes_are_the_integrals(oracle):
+    """At grid nodes the interpolated table is the defining-integral evaluation at the
+    table's own quadrature (up to the file's FP32 rounding); at beta = 0 and pi it is the
+    symmetric part of that evaluation, whose remainder is quadrature error (small)."""
+    for ia, ib_, ik in [(5, 7, 20), (16, 30, 3), (32, 0, 48), (1, 16, 40), (3, 32, 10), (0, 0, 30)]:
+        a, be, b = R_MUON * ia / 32, np.pi * ib_ / 32, R_MUON * (ik / 48) ** 2
+        got, raw = interp(oracle, a, be, b), exact(a, be, b)
+        want = symmetrized(raw) if ib_ in (0, 32) else raw
+        for A in range(3):
+            sc = np.abs(want[14 * A:14 * A + 14]).max()
+            assert np.abs(got[14 * A:14 * A + 14] - want[14 * A:14 * A + 14]).max() <= 1e-6 * sc
+            assert np.abs(raw[14 * A:14 * A + 14] - want[14 * A:14 * A + 14]).max() <= 1e-3 * sc
 
-    def d_x(f, xx, yy, al):
-        e = np.zeros(4)
-        e[al] = h
-        return (f(xx + e, yy) - f(xx - e, yy)) / (2 * h)
 
-    def d_sum(f, xx, yy, be):
-        e = np.zeros(4)
-        e[be] = h
-        return (f(xx + e, yy + e) - f(xx - e, yy - e)) / (2 * h)
+def test_table_between_nodes_against_finer_integrals(oracle):
+    """Off the nodes, the interpolated table against the integrals evaluated with a finer
+    quadrature (24 48 12 24): interpolation plus quadrature error, a few 1e-3 of each
+    tensor's largest component (the size measured when the grid was chosen)."""
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for _ in range(4):
+        a, b = rng.uniform(0.2, 5.5, 2)
+        be = rng.uniform(0.1, 3.0)
+        got, want = interp(oracle, a, be, b), exact(a, be, b, (24, 48, 12, 24))
+        for A in range(3):
+            sc = np.abs(want[14 * A:14 * A + 14]).max()
+            worst = max(worst, np.abs(got[14 * A:14 * A + 14] - want[14 * A:14 * A + 14]).max() / sc)
+    assert worst < 3e-3, worst
 
-    for al, be in itertools.product(range(4), repeat=2):
-        t1 = d_x(lambda a, b: d_sum(lambda c, d: svt(c, d)[1], a, b, be), x, y, al)
-        t2 = d_x(lambda a, b: svt(a, b)[2][be] + (np.eye(4)[be] * svt(a, b)[0] / 4), x, y, al)
-        t3 = d_sum(lambda a, b: svt(a, b)[2][al] + (np.eye(4)[al] * svt(a, b)[0] / 4), x, y, be)
-        assert np.allclose(T[0, al, be], t1, atol=1e-6), ("I", al, be)
-        assert np.allclose(T[1, al, be], t2, atol=1e-7), ("II", al, be)
-        assert np.allclose(T[2, al, be], t3, atol=1e-7), ("III", al, be)
+
+def test_t_frame_symmetries(oracle):
+    """T^II_{al be de} = d^x_al <eps_be eps_de I> is symmetric in (be, de), T^III in
+    (al, de): in frame components c[1] = c[2], c[5] = c[6], c[8] = c[10], c[9] = c[11]
+    (T^II) and c[1] = c[4], c[3] = c[6], c[8] = c[12], c[9] = c[13] (T^III)."""
+    t = np.zeros(oracle.oracle_table_size())
+    oracle.oracle_table(P(t))
+    t = t.reshape(-1, 42)
+    sc = np.abs(t).max()
+    for i, j in ((1, 2), (5, 6), (8, 10), (9, 11)):
+        assert np.abs(t[:, 14 + i] - t[:, 14 + j]).max() <= 1e-6 * sc
+    for i, j in ((1, 4), (3, 6), (8, 12), (9, 13)):
+        assert np.abs(t[:, 28 + i] - t[:, 28 + j]).max() <= 1e-6 * sc
+    # T^I has no such symmetry
+    assert np.abs(t[:, 1] - t[:, 2]).max() > 1e-3 * sc
 
 
 def qed(oracle, x, y):
@@ -235,33 +238,86 @@ def qed(oracle, x, y):
     return k
 
 
+def full_kernel(k384):
+    """[6][4][4][4] with k over (rho < sigma) -> the antisymmetric [4][4][4][4][4]."""
+    pairs = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    k = k384.reshape(6, 4, 4, 4)
+    L = np.zeros((4, 4, 4, 4, 4))
+    for i, (r, s) in enumerate(pairs):
+        L[r, s] = k[i]
+        L[s, r] = -k[i]
+    return L
+
+
+def test_qed_kernel_covariance(oracle):
+    """L(Rx, Ry) = R.L(x, y) on all five indices for rotations and reflections R of
+    O(4): the frame assembly, the traces and the interpolation in invariants."""
+    rng = np.random.default_rng(5)
+    for trial in range(4):
+        x, y = rng.uniform(-1.5, 1.5, 4), rng.uniform(-1.5, 1.5, 4)
+        Q, _ = np.linalg.qr(rng.normal(size=(4, 4)))
+        L0 = full_kernel(qed(oracle, x, y)[0])
+        L1 = full_kernel(qed(oracle, Q @ x, Q @ y)[0])
+        want = np.einsum("ai,bj,ck,dl,em,ijklm->abcde", Q, Q, Q, Q, Q, L0)
+        assert np.abs(L1 - want).max() <= 1e-10 * np.abs(L0).max(), trial
+    # a lattice symmetry with the arguments parallel (the frame's degenerate branch)
+    x = np.array([1.0, 1.0, 0.0, 0.0])
+    perm = np.eye(4)[[2, 3, 0, 1]]
+    L0 = full_kernel(qed(oracle, x, 2 * x)[0])
+    L1 = full_kernel(qed(oracle, perm @ x, perm @ (2 * x))[0])
+    want = np.einsum("ai,bj,ck,dl,em,ijklm->abcde", perm, perm, perm, perm, perm, L0)
+    assert np.abs(L1 - want).max() <= 1e-10 * np.abs(L0).max()
+
+
 def test_qed_kernel_vanishes_at_origin(oracle):
     rng = np.random.default_rng(0)
     for _ in range(5):
         v = rng.uniform(-2, 2, 4)
-        assert np.all(qed(oracle, np.zeros(4), v) == 0.0)
-        assert np.all(qed(oracle, v, np.zeros(4)) == 0.0)
-        assert np.abs(qed(oracle, v, rng.uniform(-2, 2, 4))).max() > 1e-3
+        assert np.abs(qed(oracle, np.zeros(4), v)).max() <= 1e-15
+        assert np.abs(qed(oracle, v, np.zeros(4))).max() <= 1e-15
+        assert np.abs(qed(oracle, v, rng.uniform(-2, 2, 4))).max() > 1e-7
 
 
 def test_qed_kernel_cutoff(oracle):
-    R = 12.0 * 105.6583745 / 197.3269804
     v = np.array([0.3, -0.2, 0.1, 0.5])
-    far = np.array([R * 1.01, 0, 0, 0])
+    far = np.array([R_MUON * 1.01, 0, 0, 0])
     assert np.all(qed(oracle, far, v) == 0.0) and np.all(qed(oracle, v, far) == 0.0)
-    near = np.array([R * 0.99, 0, 0, 0])
+    near = np.array([R_MUON * 0.99, 0, 0, 0])
     assert np.abs(qed(oracle, near, v)).max() > 0.0
 
 
-def test_table_nodes_exact(oracle):
-    """At grid nodes the interpolated channels are the exact ones."""
-    R = 12.0 * 105.6583745 / 197.3269804
-    for ia, ic, ib in [(0, 12, 0), (3, 5, 7), (48, 24, 48), (20, 0, 31)]:
-        a, c, b = R * ia / 48, -1 + 2 * ic / 24, R * ib / 48
-        got = np.zeros(32)
-        oracle.oracle_channels_interp(a, c, b, P(got))
-        want = channels(oracle, a * a, b * b, a * b * c)
-        assert np.allclose(got[:31], want[:31], rtol=1e-13, atol=1e-15)
+# The reference's golden sums involve KQED's tabulated kernel; this kernel, computed
+# from its defining integrals and tabulated independently, meets them to about 1%
+# (KQED's own interpolation and this table's differ at that level). Tolerance 2e-2.
+GOLDEN_TOL = 2e-2
+
+
+def test_golden_kernel_sum(oracle):
+    """sanity_check/test_contract_random.cpp:37-58: 8x4x4x4, fill seeds 1 (x) and 2 (y),
+    gsx = 0, yv = (1,1,1,1), xunit = 1."""
+    g = GOLD["test_contract_random"]
+    S = fill_block(oracle, g["lattice"], g["seed_x"])
+    Y = fill_block(oracle, g["lattice"], g["seed_y"])
+    got = oracle_pair(oracle, S, Y, g["lattice"], g["gsx"], g["yv"], g["xunit"])
+    want = np.array(g["kernel_sum"])
+    assert np.all(np.abs(got - want) <= GOLDEN_TOL * np.abs(want)), (got, want)
+
+
+def test_golden_2p2_sums(oracle):
+    """sanity_check/test_contract_random.cpp:72-127: 6^4, seed 3, w = (1,2,3,4),
+    targets (3,3,3,3) and (0,5,1,2), xunit = 1: sum of P2_0, P2_1, P3 and of P4_0, P4_1."""
+    g = GOLD["sum_p1_2p2"]
+    dims = np.array(g["lattice"], np.int32)
+    blk = fill_block(oracle, dims, g["seed"])
+    s23 = s4 = 0.0
+    for tgt in ((3, 3, 3, 3), (0, 5, 1, 2)):
+        out = np.zeros(320)
+        oracle.oracle_p234(P(blk), P(dims), g["flavour"], P(np.array((1, 2, 3, 4), np.int32)),
+                           P(np.array(tgt, np.int32)), ctypes.c_double(1.0), P(out))
+        s23 += out[:192].sum()
+        s4 += out[192:].sum()
+    for got, want in ((s23, g["sum_p23"]), (s4, g["sum_p4"])):
+        assert abs(got - want) <= GOLDEN_TOL * abs(want), (got, want)
 
 
 # ---- the contraction, restated independently with numpy on a tiny lattice --
@@ -374,7 +430,7 @@ def test_dzsu_of_point_fields(oracle):
 
 # ---- the MLP restatement ------------------------------------------------------
 
-MLP_IN, MLP_OUT = (3, 128, 128, 128, 128), (128, 128, 128, 128, 31)
+MLP_IN, MLP_OUT = (3, 128, 128, 128, 128), (128, 128, 128, 128, 42)
 
 
 def mlp_offsets():
@@ -416,15 +472,15 @@ def test_mlp_chain_known_answer(oracle):
         p[off[l][0] + 0 * 128 + 0] = 1.0    # Wl[0][0]
     p[off[4][1] + 5] = 0.375                # b4[5]
     x = np.array([[0.8125, -3.0, 7.0]], np.float32)
-    y = np.zeros((1, 31))
-    oracle.oracle_mlp(P(p), P(x), P(np.zeros((1, 31), np.float32)), 1, None, P(y))
+    y = np.zeros((1, 42))
+    oracle.oracle_mlp(P(p), P(x), P(np.zeros((1, 42), np.float32)), 1, None, P(y))
     v = 0.8125
     for _ in range(4):
         v = gelu_exact(v)
     assert abs(y[0, 0] - v) <= 1e-15
     assert y[0, 5] == 0.375 and np.count_nonzero(y[0]) == 2
     # the BF16 inference forward rounds each hidden activation to BF16
-    yb = np.zeros((1, 31))
+    yb = np.zeros((1, 42))
     oracle.oracle_mlp_forward_bf16(P(p), P(x), 1, P(yb))
     v = 0.8125
     for _ in range(4):
@@ -440,7 +496,7 @@ def test_mlp_gradient_finite_differences(oracle):
     rng = np.random.default_rng(3)
     p = bf16_np(rng.normal(0, 0.2, n).astype(np.float32))
     x = rng.uniform(-1, 1, (5, 3)).astype(np.float32)
-    t = rng.normal(0, 1, (5, 31)).astype(np.float32)
+    t = rng.normal(0, 1, (5, 42)).astype(np.float32)
     g = np.zeros(n)
     oracle.oracle_mlp(P(p), P(x), P(t), 5, P(g), None)
     # one weight and one bias of every layer; steps of one BF16 ulp-pair (exact)

@@ -129,9 +129,9 @@ def test_attributes(tmp_path):
     assert attribute(path, "direction") == list(CHAIN[1])
     assert attribute(path, "origin") == list(CHAIN[0])
     assert attribute(path, "stride") == [CHAIN[2]]
-    assert attribute(path, "kernel") == "LLambda0.4 (stand-in weight functions)"
-    write(path, [CHAIN], [], kernel="LLambda0.4")
     assert attribute(path, "kernel") == GOLD["kernel_default"]
+    write(path, [CHAIN], [], kernel="LSurrogate")
+    assert attribute(path, "kernel") == "LSurrogate"
     h = header(path)
     for a in GOLD["attributes_int4"] + GOLD["attributes_int1"]:
         assert re.search(rf'ATTRIBUTE "{a}" \{{\s*DATATYPE\s+H5T_STD_I32LE', h), a
