@@ -152,10 +152,20 @@ HLBL_HD R pick4(const R v[4], int i) {
   return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3]));
 }
 
+// |x ^ y|^2 / (|x|^2 |y|^2) at or below which x and y count as parallel: far above
+// the rounding of |x ^ y| for parallel arguments (with fused multiply-adds its
+// terms x_i y_j - x_j y_i are the rounding error of one product, not 0), so that
+// y - (y.e0) e0 is never a rounding remainder; the components that depend on e1
+// vanish linearly in the angle, so below the threshold its choice is immaterial.
+template <class R>
+HLBL_HD constexpr R parallel_sin2() { return R(1e-24); }  // FP64: |sin beta| <= 1e-12
+template <>
+HLBL_HD constexpr float parallel_sin2<float>() { return 1e-10f; }  // FP32: <= 1e-5
+
 // The frame of an argument pair: e0 = x^ (y^ when x = 0, axis 0 when both are 0),
-// e1 = the unit part of y orthogonal to e0, or, when y is parallel to e0 (|x ^ y|
-// <= 1e-12 |x| |y|) or 0, the unit part orthogonal to e0 of the axis on which e0
-// is smallest (the first such axis).
+// e1 = the unit part of y orthogonal to e0, or, when y is parallel to e0 (see
+// parallel_sin2) or 0, the unit part orthogonal to e0 of the axis on which e0 is
+// smallest (the first such axis).
 template <class R>
 HLBL_HD void frame(const R x[4], const R y[4], R e0[4], R e1[4]) {
   const R u = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
@@ -180,7 +190,7 @@ HLBL_HD void frame(const R x[4], const R y[4], R e0[4], R e1[4]) {
       const R m = x[i] * y[j] - x[j] * y[i];
       s2 += m * m;
     }
-  const bool parallel = !(u > R(0) && v > R(0)) || s2 <= R(1e-24) * u * v;
+  const bool parallel = !(u > R(0) && v > R(0)) || s2 <= parallel_sin2<R>() * u * v;
   R c = R(0);
   R t[4];
   if (!parallel) {
