@@ -75,6 +75,8 @@ _SIGS = {
                                   C.c_void_p, C.c_void_p]),
     "hlbl_qed_channels": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
                                     C.c_void_p]),
+    "hlbl_qed_kernel_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                          C.c_void_p, C.c_void_p]),
     "hlbl_mlp_param_count": (C.c_int, [C.c_void_p]),
     "hlbl_mlp_init": (C.c_int, [C.c_void_p, C.c_uint64]),
     "hlbl_mlp_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64,
@@ -262,6 +264,12 @@ class QedKernel:
     def kernel(self, d_x, d_y, n, d_kerv, mode=0, stream=None):
         check(lib().hlbl_qed_kernel(self.handle, mode, C.c_void_p(d_x), C.c_void_p(d_y), n,
                                     C.c_void_p(d_kerv), stream))
+
+    def kernel_noalloc(self, d_x, d_y, n, d_kerv, stream=None):
+        """kernel() in its plain form with the library's fixed scratch: at most 2048 pairs,
+        no allocation; calls that overlap on one GPU race."""
+        check(lib().hlbl_qed_kernel_noalloc(self.handle, C.c_void_p(d_x), C.c_void_p(d_y), n,
+                                            C.c_void_p(d_kerv), stream))
 
     def channels(self, d_acb, n, d_out, mode=0, stream=None):
         check(lib().hlbl_qed_channels(self.handle, mode, C.c_void_p(d_acb), n,

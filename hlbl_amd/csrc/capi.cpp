@@ -172,6 +172,7 @@ int hlbl_qed_init_file(hlbl_qed** out, int device, const char* table_path) {
       q->table32.ensure(n);
       check_hip(hipMemcpy(q->table32.p, t32.data(), n * sizeof(float), hipMemcpyHostToDevice),
                 "hipMemcpy table");
+      q->noalloc_scratch = noalloc_scratch_setup(device);  // with `device` current
     } catch (...) {
       delete q;
       throw;

@@ -104,7 +104,12 @@ struct hlbl_qed {
   hlbl::DevBuf<float> mlp_scale;  // [2][kMlpOut]: offset, scale (normalised -> physical)
   bool mlp_ready = false;
   double mlp_radius = hlbl::qed::kRadius;  // trained domain: |x|, |y| <= mlp_radius (muon units)
+  double* noalloc_scratch = nullptr;  // this device's fixed scratch array (qed_noalloc.hip)
 };
+
+namespace hlbl {
+double* noalloc_scratch_setup(int device);
+}  // namespace hlbl
 
 namespace hlbl {
 struct WindowCache;

@@ -86,6 +86,14 @@ int hlbl_qed_table_copy(const hlbl_qed* q, double* host_out, size_t n_doubles);
  * plain outside the trained domain). */
 int hlbl_qed_kernel(hlbl_qed* q, int mode, const double* d_x, const double* d_y, int n,
                     double* d_kerv, hlbl_stream stream);
+/* hlbl_qed_kernel's plain form (mode 0) without allocation: the per-point scratch is a
+ * fixed array of the library on each GPU (18.9 MB, held until the process ends), so at
+ * most 2048 argument pairs per call (more is an error). No counterpart in the reference:
+ * for callers that cannot allocate (code under graph capture, tight serving loops,
+ * harnesses that forbid allocation). Calls that overlap on one GPU share the array and
+ * race; order them. Results equal hlbl_qed_kernel's bit for bit. */
+int hlbl_qed_kernel_noalloc(hlbl_qed* q, const double* d_x, const double* d_y, int n,
+                            double* d_kerv, hlbl_stream stream);
 /* The interpolated table channels (the 42 frame components of T^I, T^II, T^III,
  * then zero pad) at n points (a=|x|, beta = the angle between x and y in [0, pi],
  * b=|y|): d_acb is [n][3], d_out [n][n_channels]. mode as above. */
