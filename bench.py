@@ -266,6 +266,11 @@ def main():
     dt1, res1 = time_form(con, descs, 1, args.warmup, args.steps)
     dt2, res2 = time_form(con, descs, 2, args.warmup, args.steps)
     judge, nsamp = con.judge()
+    fell_back = con.fell_back()
+    # the surrogate's own results, with the judge's gate lifted for one call
+    con.set_judge_bound(float("inf"))
+    res2_raw = con.window_kernel_sum(descs, XUNIT, form=2)
+    con.set_judge_bound(1e-3)
     _, ms1 = con.window_profile(descs, XUNIT, form=1)
     _, ms2 = con.window_profile(descs, XUNIT, form=2)
     log(f"form 1: {dt1 * 1e3:.1f} ms per window ({65 / dt1:.2f} pairs/s), stages {ms1} ms")
@@ -282,10 +287,13 @@ def main():
     ref = np.array(ref)
     err1 = np.abs(np.array(res1) - ref) / np.abs(ref)
     err2 = np.abs(np.array(res2) - ref) / np.abs(ref)
+    err2_raw = np.abs(np.array(res2_raw) - ref) / np.abs(ref)
     scale = np.sqrt(np.mean(ref ** 2))
     log(f"plain FP64: {dt64 * 1e3:.1f} ms per pair ({1 / dt64:.2f} pairs/s); max rel err form1 "
         f"{err1.max():.2e} form2 {err2.max():.2e} (median {np.median(err2):.2e}, "
-        f"max |diff| / rms pair sum {np.max(np.abs(np.array(res2) - ref)) / scale:.2e})")
+        f"max |diff| / rms pair sum {np.max(np.abs(np.array(res2) - ref)) / scale:.2e}); "
+        f"surrogate {'fell back to form 1' if fell_back else 'used'}, its own results: max rel err "
+        f"{err2_raw.max():.2e}, median {np.median(err2_raw):.2e}")
 
     # headline: the faster form within the 1e-3 error bound
     forms = {1: (dt1, err1.max(), ms1), 2: (dt2, err2.max(), ms2)}
@@ -323,7 +331,10 @@ def main():
                     "peak": MI355X_FP16_MFMA / 1e12, "unit": "TFLOP/s",
                     "frac": flops_pass2 / (ms[3] * 1e-3) / MI355X_FP16_MFMA, "flops": flops_pass2,
                     "window_tflops": flops_window / dt / 1e12},
-        "surrogate_within_bound": bool(err2.max() <= 1e-3),
+        "surrogate_within_bound": bool(not fell_back and err2_raw.max() <= 1e-3),
+        "form2_fell_back_to_form1": bool(fell_back),
+        "surrogate_raw_max_pair_rel_err": float(err2_raw.max()),
+        "surrogate_raw_median_pair_rel_err": float(np.median(err2_raw)),
         "max_pair_rel_err": float(err),
         "plain_fp64_pairs_per_s": 1.0 / dt64,
         "speedup_vs_plain_fp64": value * dt64,
