@@ -7,6 +7,7 @@
 #include <map>
 #include <memory>
 #include <optional>
+#include <set>
 #include <stdexcept>
 #include <vector>
 
@@ -68,10 +69,21 @@ class PropagatorCache {
     return staging_[slot].p;
   }
 
+  // the pinned host copy of a cached position (host tier)
+  const double* host_ptr(const Vec4& p) const {
+    auto it = entries_.find(p);
+    if (it == entries_.end() || !it->second.host) throw Error("PropagatorCache: position is not cached on the host");
+    return it->second.host->p;
+  }
+  // the device solve target of the host tier (holds the block last solved)
+  const double* scratch() const { return scratch_.p; }
+  bool host_tier() const { return host_; }
+
   void pair_done(const Vec4& a, const Vec4& b) {
     for (const Vec4& p : {a, b})
       if (--pending_[p] == 0) evict(p);
   }
+  bool has(const Vec4& p) const { return entries_.count(p) != 0; }
   int size() const { return (int)entries_.size(); }
   int peak() const { return peak_; }
 
@@ -100,6 +112,108 @@ class PropagatorCache {
   std::optional<Vec4> staged_[2];
 };
 
+// Device blocks that feed the host tier's batched calls (one call per window or per
+// new source): the blocks a call reads are staged here from pinned host memory. A
+// block just solved is copied from the solve target on the device; every other block
+// is copied host to device on a stream of its own, issued while the previous call
+// computes (the calls are synchronous, so a slot that no pending call reads can be
+// refilled at once). Slots hold positions until they are needed for others or their
+// position leaves the cache.
+class StagingPool {
+ public:
+  StagingPool(size_t block_len, int n_slots) : len_(block_len), slots_(n_slots) {
+    for (Slot& s : slots_) s.buf.ensure(len_);
+    check_hip(hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking), "copy stream");
+    check_hip(hipEventCreateWithFlags(&done_, hipEventDisableTiming), "copy event");
+  }
+  ~StagingPool() {
+    if (done_) (void)hipEventDestroy(done_);
+    if (copy_) {
+      (void)hipStreamSynchronize(copy_);
+      (void)hipStreamDestroy(copy_);
+    }
+  }
+  StagingPool(const StagingPool&) = delete;
+  StagingPool& operator=(const StagingPool&) = delete;
+
+  // the block at p, solved into `src` on the device, copied into a slot on st
+  void put_solved(const Vec4& p, const double* src, const std::set<Vec4>& keep, hipStream_t st) {
+    Slot& s = take(p, keep);
+    check_hip(hipMemcpyAsync(s.buf.p, src, len_ * sizeof(double), hipMemcpyDeviceToDevice, st), "D2D block");
+  }
+  // issue host-to-device copies of the positions in `want` that no slot holds; slots
+  // holding `keep` or `want` are not overwritten
+  void prefetch(const std::vector<Vec4>& want, const std::set<Vec4>& keep, const PropagatorCache& cache) {
+    std::set<Vec4> hold(keep);
+    hold.insert(want.begin(), want.end());
+    bool issued = false;
+    for (const Vec4& p : want) {
+      if (find(p)) continue;
+      Slot& s = take(p, hold);
+      check_hip(hipMemcpyAsync(s.buf.p, cache.host_ptr(p), len_ * sizeof(double), hipMemcpyHostToDevice, copy_),
+                "H2D block");
+      issued = true;
+      ++copies_;
+    }
+    if (issued) {
+      check_hip(hipEventRecord(done_, copy_), "copy event");
+      pending_ = true;
+    }
+  }
+  // st waits for the copies issued so far
+  void wait(hipStream_t st) {
+    if (!pending_) return;
+    check_hip(hipStreamWaitEvent(st, done_, 0), "wait for copies");
+    pending_ = false;
+  }
+  const double* ptr(const Vec4& p) {
+    Slot* s = find(p);
+    if (!s) throw Error("StagingPool: position is not staged");
+    s->used = ++clock_;
+    return s->buf.p;
+  }
+  void forget(const Vec4& p) {
+    if (Slot* s = find(p)) s->pos.reset();
+  }
+  long copies() const { return copies_; }
+
+ private:
+  struct Slot {
+    DevBuf<double> buf;
+    std::optional<Vec4> pos;
+    long used = 0;
+  };
+  Slot* find(const Vec4& p) {
+    for (Slot& s : slots_)
+      if (s.pos && *s.pos == p) return &s;
+    return nullptr;
+  }
+  // a slot for p: the one holding p, else a free one, else the least recently used
+  // one whose position is not held
+  Slot& take(const Vec4& p, const std::set<Vec4>& hold) {
+    if (Slot* s = find(p)) return *s;
+    Slot* best = nullptr;
+    for (Slot& s : slots_) {
+      if (!s.pos) {
+        best = &s;
+        break;
+      }
+      if (hold.count(*s.pos)) continue;
+      if (!best || s.used < best->used) best = &s;
+    }
+    if (!best) throw Error("StagingPool: no free slot (too few staging blocks)");
+    best->pos = p;
+    best->used = ++clock_;
+    return *best;
+  }
+  size_t len_;
+  std::vector<Slot> slots_;
+  hipStream_t copy_ = nullptr;
+  hipEvent_t done_ = nullptr;
+  bool pending_ = false;
+  long clock_ = 0, copies_ = 0;
+};
+
 void window_entry(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit, int form,
                   double* out, hipStream_t st);
 
@@ -114,10 +228,45 @@ Vec4 min_image(const Vec4& a, const Vec4& b, const hlbl_geom& g) {
           coord_map_diff(a[2], b[2], g.LY), coord_map_diff(a[3], b[3], g.LZ)};
 }
 
+// the source after source i of step `step` in solve order, or null
+const Source* next_source(const Schedule& s, int step, size_t i) {
+  for (int k = step; k < (int)s.size(); ++k) {
+    const auto& src = s[k].sources;
+    const size_t j = k == step ? i + 1 : 0;
+    if (j < src.size()) return &src[j];
+  }
+  return nullptr;
+}
+
+// staging blocks the host tier's batched calls need: the most distinct positions of
+// one window (batch 1), or of one source's call together with the next source's
+// partners (batch 2)
+int staging_slots(const Schedule& s, int batch) {
+  size_t most = 0;
+  for (int k = 0; k < (int)s.size(); ++k) {
+    if (batch == 1) {
+      std::set<Vec4> w;
+      for (const Source& src : s[k].sources)
+        for (const Vec4& q : src.partners) {
+          w.insert(q);
+          w.insert(src.position);
+        }
+      most = std::max(most, w.size());
+      continue;
+    }
+    for (size_t i = 0; i < s[k].sources.size(); ++i) {
+      const Source& src = s[k].sources[i];
+      if (src.partners.empty()) continue;
+      std::set<Vec4> w(src.partners.begin(), src.partners.end());
+      w.insert(src.position);
+      if (const Source* nx = next_source(s, k, i)) w.insert(nx->partners.begin(), nx->partners.end());
+      most = std::max(most, w.size());
+    }
+  }
+  return (int)most;
+}
+
 }  // namespace
-
-
-
 
 void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xunit, bool host_tier,
                int form, int batch, std::vector<PairRow>& rows, double stats[5],
@@ -135,8 +284,34 @@ void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xuni
         ++pending[src.position];
       }
   const size_t len = (size_t)2 * 12 * 24 * c->volume;
+  // the run's own stream: the window calls wait for this stream only, not for the
+  // staging copies of the next call on the copy stream
+  struct StreamGuard {
+    hipStream_t s = nullptr;
+    ~StreamGuard() {
+      if (s) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+      }
+    }
+  } run_stream;
+  check_hip(hipStreamCreateWithFlags(&run_stream.s, hipStreamNonBlocking), "run stream");
   PropagatorCache cache(host_tier, len, pending);
-  hipStream_t stm = nullptr;
+  // the host tier's batched calls read their blocks from a device staging pool sized for
+  // the largest call (windows) or the largest two consecutive calls (sources, whose next
+  // call's blocks are copied while one computes)
+  std::unique_ptr<StagingPool> pool;
+  if (host_tier && batch != 0) {
+    const int n_slots = staging_slots(s, batch);
+    if (n_slots > 0) pool = std::make_unique<StagingPool>(len, n_slots);
+  }
+  auto pair_done = [&](const Vec4& a, const Vec4& b) {
+    cache.pair_done(a, b);
+    if (pool)
+      for (const Vec4& p : {a, b})
+        if (!cache.has(p)) pool->forget(p);
+  };
+  hipStream_t stm = run_stream.s;
   double t_contract = 0, t_solve = 0;
   int solved = 0;
   using clk = std::chrono::steady_clock;
@@ -181,28 +356,41 @@ void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xuni
   for (const Step& st : s) {
     const size_t window_first = rows.size();
     if (batch == 2) {
-      if (host_tier) throw Error("hlbl_run_chain: source batches read every block in place (device tier)");
-      for (const Source& src : st.sources) {
+      for (size_t i = 0; i < st.sources.size(); ++i) {
+        const Source& src = st.sources[i];
         solve(st.index, src.position);
         if (src.partners.empty()) continue;
-        const double* d_y = cache.device_ptr(src.position, 0, stm);
+        std::set<Vec4> keep(src.partners.begin(), src.partners.end());
+        keep.insert(src.position);
+        if (pool) {
+          // this call's blocks: the one just solved from the solve target, the partners
+          // (prefetched during the previous call, else now); then the next call's
+          // partners, copied while this one computes
+          if (cache.has(src.position)) pool->put_solved(src.position, cache.scratch(), keep, stm);
+          pool->prefetch(src.partners, keep, cache);
+          pool->wait(stm);
+          const Source* nx = next_source(s, st.index, i);
+          if (nx && !nx->partners.empty()) pool->prefetch(nx->partners, keep, cache);
+        }
+        auto dptr = [&](const Vec4& p, int slot) { return pool ? pool->ptr(p) : cache.device_ptr(p, slot, stm); };
+        const double* d_y = dptr(src.position, 0);
         const size_t first = rows.size();
         std::vector<hlbl_pair_desc> descs;
         for (const Vec4& q : src.partners) {
           PairRow r{wrap4(q, c->global), wrap4(src.position, c->global),
                     min_image(src.position, q, c->global), st.index, {0, 0}};
-          descs.push_back({cache.device_ptr(q, 1, stm), d_y, to_c(r.x), to_c(r.yv)});
+          descs.push_back({dptr(q, 1), d_y, to_c(r.x), to_c(r.yv)});
           rows.push_back(r);
         }
         std::vector<double> out(2 * descs.size());
         const auto t0 = clk::now();
         window_entry(c, descs.data(), (int)descs.size(), xunit, form, out.data(), stm);
         t_contract += std::chrono::duration<double>(clk::now() - t0).count();
-        for (size_t i = 0; i < descs.size(); ++i) {
-          rows[first + i].ks[0] = out[2 * i];
-          rows[first + i].ks[1] = out[2 * i + 1];
+        for (size_t i2 = 0; i2 < descs.size(); ++i2) {
+          rows[first + i2].ks[0] = out[2 * i2];
+          rows[first + i2].ks[1] = out[2 * i2 + 1];
         }
-        for (const Vec4& q : src.partners) cache.pair_done(q, src.position);
+        for (const Vec4& q : src.partners) pair_done(q, src.position);
       }
     } else if (batch == 0) {
       for (const Source& src : st.sources) {
@@ -223,13 +411,26 @@ void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xuni
           }
           t_contract += std::chrono::duration<double>(clk::now() - t0).count();
           rows.push_back(r);
-          cache.pair_done(q, src.position);
+          pair_done(q, src.position);
         }
       }
     } else {
       // the whole window: solve its sources, contract its pairs in one call
-      if (host_tier) throw Error("hlbl_run_chain: window batches read every block in place (device tier)");
-      for (const Source& src : st.sources) solve(st.index, src.position);
+      std::set<Vec4> keep;
+      for (const Source& src : st.sources)
+        for (const Vec4& q : src.partners) {
+          keep.insert(q);
+          keep.insert(src.position);
+        }
+      for (const Source& src : st.sources) {
+        solve(st.index, src.position);
+        if (pool && cache.has(src.position)) pool->put_solved(src.position, cache.scratch(), keep, stm);
+      }
+      if (pool) {
+        pool->prefetch(std::vector<Vec4>(keep.begin(), keep.end()), keep, cache);
+        pool->wait(stm);
+      }
+      auto dptr = [&](const Vec4& p, int slot) { return pool ? pool->ptr(p) : cache.device_ptr(p, slot, stm); };
       std::vector<hlbl_pair_desc> descs;
       std::vector<std::pair<Vec4, Vec4>> done;
       const size_t first = rows.size();
@@ -237,8 +438,7 @@ void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xuni
         for (const Vec4& q : src.partners) {
           PairRow r{wrap4(q, c->global), wrap4(src.position, c->global),
                     min_image(src.position, q, c->global), st.index, {0, 0}};
-          descs.push_back({cache.device_ptr(q, 1, stm), cache.device_ptr(src.position, 0, stm),
-                           to_c(r.x), to_c(r.yv)});
+          descs.push_back({dptr(q, 1), dptr(src.position, 0), to_c(r.x), to_c(r.yv)});
           rows.push_back(r);
           done.emplace_back(q, src.position);
         }
@@ -252,7 +452,7 @@ void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xuni
           rows[first + i].ks[1] = out[2 * i + 1];
         }
       }
-      for (const auto& d : done) cache.pair_done(d.first, d.second);
+      for (const auto& d : done) pair_done(d.first, d.second);
     }
     if (on_window) on_window(window_first, rows.size() - window_first);
     if (!window_2p2.empty()) {

@@ -30,7 +30,9 @@ struct Row2p2 {
 // reference writes each window's rows when it is done (src/driver.cpp:134-137).
 // batch 0: one pair per call; 1: one call per window (its sources solved first);
 // 2: one call per new source with all its partners (the solve order and cache peak of
-// batch 0, for lattices where a whole window's blocks do not fit).
+// batch 0, for lattices where a whole window's blocks do not fit). host_tier: pinned
+// host blocks; batched calls read them from device staging blocks filled on a second
+// stream (batch 2: the next call's blocks while one computes).
 // plan_2p2: the chain's planned 2+2 sources by site (in [0, L)); a planned site solved
 // in its planned window gets its pieces (flavour iflavor_2p2) from the solved block
 // before it is stored (src/driver.cpp:58-95), and on_2p2(rows) receives each window's

@@ -239,8 +239,11 @@ int hlbl_max_steps_without_wrap(hlbl_coord h, int stride, hlbl_coord lattice,
  * 1 = pinned host with two staging slots), contract per pair (form as above;
  * batch 0 = one pair per call in solve order, 1 = one call per window, its sources
  * solved first, 2 = one call per new source with all its partners, in the solve order
- * and with the cache peak of batch 0; 1 and 2 need the device tier), free blocks
- * after their last pair. rows gets n_pairs * 16 doubles:
+ * and with the cache peak of batch 0), free blocks after their last pair. With the
+ * host tier, batches 1 and 2 read their blocks from device staging blocks (as many as
+ * the largest window, or the largest two consecutive source calls, need): a block just
+ * solved is copied there from the solve target, the others from pinned host memory on
+ * a second stream, batch 2's while the previous call computes. rows gets n_pairs * 16 doubles:
  * x[4], y[4], yv[4], window, chain(=0), ks0, ks1 (ints stored as doubles).
  * stats = {pairs, positions_solved, cache_peak, seconds_contract, seconds_solve}. */
 int hlbl_run_chain(hlbl_contractor* c, const hlbl_schedule* s, uint64_t seed, double xunit,

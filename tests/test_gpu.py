@@ -408,6 +408,30 @@ def test_run_chain_tiers_and_direct(qed):
     con.close()
 
 
+@pytest.mark.parametrize("dims,stride,form", [((8, 8, 8, 8), 1, 0), ((8, 8, 8, 8), 1, 1), ((32, 16, 16, 16), 2, 1)])
+@pytest.mark.parametrize("batch", [1, 2])
+def test_host_tier_batches(qed, dims, stride, form, batch):
+    """The host tier (pinned blocks; src/propagator_cache.cpp:52-68) feeding batched
+    calls: one call per window (batch 1) or per new source (batch 2), whose blocks are
+    staged on the device from pinned memory on a second stream. Rows, sums and the
+    cache peak equal the device tier's bit for bit."""
+    xunit, seed = 105.658 / 197.3269804 * 0.1, 5
+    lat = H.Lattice(dims)
+    con = H.Contractor(lat, qed)
+    sch = H.Schedule((1, -1, 1, -1), stride, 2)
+    print(f"host tier batches: dims {dims} form {form} batch {batch}", flush=True)
+    dev_rows, dst = H.run_chain(con, sch, seed, xunit, residency=0, form=form, batch=batch)
+    host_rows, hst = H.run_chain(con, sch, seed, xunit, residency=1, form=form, batch=batch)
+    assert len(dev_rows) == 15 + 2 * 65
+    assert host_rows == dev_rows
+    assert hst["cache_peak"] == dst["cache_peak"]
+    if batch == 2:
+        assert hst["cache_peak"] == sch.simulate(True)["peak_cache"]
+    log(f"host tier {dims} form {form} batch {batch}: contract {hst['seconds_contract']:.2f} s "
+        f"(device tier {dst['seconds_contract']:.2f} s)")
+    con.close()
+
+
 @pytest.mark.skipif(not os.path.exists(H.H5_LIB_PATH), reason="libhlbl_h5.so absent (no HDF5 at build time)")
 @pytest.mark.parametrize("batch", [0, 1])
 def test_run_chain_writes_result_file(qed, tmp_path, batch):
