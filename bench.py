@@ -1,14 +1,17 @@
 """Benchmark: pairs/s of the connected kernel sum at 24^3x48 on one MI355X.
 
-Workload (one step = one window): the sliding-hypercube chain K=8, stride n=3,
-h=(1,1,1,1); window 1 = 65 pairs x 2 flavours over 30 distinct propagator blocks
-(3.06 GB each, FP64, resident in HBM), random fields from seed 2 (RandomProvider,
-generated on the device). Two window-batched forms run as one hipGraph per
-window: FP32 products with the plain QED kernel (form 1), and the same with the
-MLP surrogate's channels (form 2, network trained first over the lattice's
-reach, seed 4). The plain FP64 per-pair path runs on the same fields as the
-reference; every pair sum of both forms is compared to it. The headline value is
-the faster form whose largest per-pair relative error is within 1e-3.
+Workload (one step = one chain): the sliding-hypercube chain K=8, stride n=3,
+h=(1,1,1,1) at 24^3x48, seed 2 (RandomProvider, generated on the device): 136
+positions, 535 pairs, contracted by the library's schedule loop (run_chain) with one
+window-batched call per window (the FP32 form, one hipGraph per window; the
+coefficient fields of the window's distinct yv built once per chain, the kept ones
+dropped before each timed chain). The timed part is the contraction; the solves
+(device fills) are reported beside it. Window 1 (65 pairs x 2 flavours over 30
+blocks) is also run on its own: its pair sums in the FP32 form (form 1) and with the
+MLP surrogate's channels (form 2, network trained first over the lattice's reach,
+seed 4) are compared with the plain FP64 per-pair path on the same fields, which
+also gives the plain path's pairs/s; its stages are timed with HIP events. The
+headline form is the fastest within 1e-3 per pair.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--epochs E]
 """
@@ -132,21 +135,6 @@ def cpu_full_pair(H):
                       "cores": threads, "kernel_sum": list(out)}), flush=True)
 
 
-def time_form(con, descs, form, warmup, steps):
-    """seconds per window (host clock around synchronised graph launches) and the
-    last result"""
-    import torch
-    res = None
-    for _ in range(warmup):
-        res = con.window_kernel_sum(descs, XUNIT, form=form)
-    torch.cuda.synchronize()
-    t0 = time.time()
-    for _ in range(steps):
-        res = con.window_kernel_sum(descs, XUNIT, form=form)
-    torch.cuda.synchronize()
-    return (time.time() - t0) / steps, res
-
-
 def bench_2p2(H, torch, lat, con, args):
     """The 2+2 pieces of one source (Contractor::pieces_2p2, src/contract.cpp:116-150):
     P1 alone (pi_2p2 + p1_2p2), then P1 and four targets (p234_sites + p234_reduce per
@@ -193,7 +181,7 @@ def bench_2p2(H, torch, lat, con, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-full-pair", action="store_true",
@@ -262,20 +250,23 @@ def main():
     descs = [(blocks[q].data_ptr(), blocks[p].data_ptr(), wrap(q), lat.site_map_diff(p, q))
              for q, p in pairs_w]
 
-    # the two window forms, then their stage times (HIP events on the window stream)
-    dt1, res1 = time_form(con, descs, 1, args.warmup, args.steps)
-    dt2, res2 = time_form(con, descs, 2, args.warmup, args.steps)
+    # window 1 in both forms: stages with every field built (cold) and with the kept
+    # fields (as in a chain's later windows), results
+    con.drop_fields()
+    res1, ms1_cold = con.window_profile(descs, XUNIT, form=1)
+    _, ms1 = con.window_profile(descs, XUNIT, form=1)
+    dt1_w = sum(ms1) * 1e-3
+    con.drop_fields()
+    res2 = con.window_kernel_sum(descs, XUNIT, form=2)
     judge, nsamp = con.judge()
     fell_back = con.fell_back()
     # the surrogate's own results, with the judge's gate lifted for one call
     con.set_judge_bound(float("inf"))
-    res2_raw = con.window_kernel_sum(descs, XUNIT, form=2)
+    con.drop_fields()
+    res2_raw, ms2_cold = con.window_profile(descs, XUNIT, form=2)
     con.set_judge_bound(1e-3)
-    _, ms1 = con.window_profile(descs, XUNIT, form=1)
-    _, ms2 = con.window_profile(descs, XUNIT, form=2)
-    log(f"form 1: {dt1 * 1e3:.1f} ms per window ({65 / dt1:.2f} pairs/s), stages {ms1} ms")
-    log(f"form 2: {dt2 * 1e3:.1f} ms per window ({65 / dt2:.2f} pairs/s), stages {ms2} ms, "
-        f"judge {judge:.2e} ({nsamp} sites)")
+    log(f"window 1 form 1: stages cold {ms1_cold} ms, kept fields {ms1} ms")
+    log(f"window 1 form 2: stages cold {ms2_cold} ms, judge {judge:.2e} ({nsamp} sites)")
 
     # plain FP64, one pair per call, over the whole window (the reference sums)
     ref = []
@@ -290,28 +281,54 @@ def main():
     err2_raw = np.abs(np.array(res2_raw) - ref) / np.abs(ref)
     scale = np.sqrt(np.mean(ref ** 2))
     log(f"plain FP64: {dt64 * 1e3:.1f} ms per pair ({1 / dt64:.2f} pairs/s); max rel err form1 "
-        f"{err1.max():.2e} form2 {err2.max():.2e} (median {np.median(err2):.2e}, "
-        f"max |diff| / rms pair sum {np.max(np.abs(np.array(res2) - ref)) / scale:.2e}); "
-        f"surrogate {'fell back to form 1' if fell_back else 'used'}, its own results: max rel err "
+        f"{err1.max():.2e} (median {np.median(err1):.2e}, max |diff| / rms pair sum "
+        f"{np.max(np.abs(np.array(res1) - ref)) / scale:.2e}); surrogate "
+        f"{'fell back to form 1' if fell_back else 'used'}, its own results: max rel err "
         f"{err2_raw.max():.2e}, median {np.median(err2_raw):.2e}")
+    del blocks, descs
+    torch.cuda.empty_cache()
 
-    # headline: the faster form within the 1e-3 error bound
-    forms = {1: (dt1, err1.max(), ms1), 2: (dt2, err2.max(), ms2)}
-    ok = [f for f in forms if forms[f][1] <= 1e-3]
-    head = min(ok, key=lambda f: forms[f][0]) if ok else 1
-    dt, err, ms = forms[head]
-    value = 65.0 / dt
-    # roofline of the dominant kernel, pass 2. HBM: the window's compulsory bytes, each of
-    # its 31 FP64 blocks read once (31 x 4608 V bytes). Compute: pass 2's nominal flops,
-    # per pair 2 V (8 x 6912 + 4 x 144 x 640) = 847,872 V (the dxu products and the 640
-    # real 12x12 traces of both flavours), against the dense FP16 MFMA peak the traces
-    # run on (each product is issued as three FP16 products; counted once). The whole
-    # window: 958,464 V per pair (pass 1 included).
+    # headline: the faster form within the 1e-3 error bound, timed over whole chains
+    head = 2 if (not fell_back and err2_raw.max() <= 1e-3) else 1
+    n_pairs = len(sch.pairs())
+    assert n_pairs == 535
+
+    def chain():
+        con.drop_fields()
+        torch.cuda.synchronize()
+        t0 = time.time()
+        rows, st = H.run_chain(con, sch, 2, XUNIT, residency=0, form=head, batch=1)
+        torch.cuda.synchronize()
+        return time.time() - t0, rows, st
+
+    for _ in range(args.warmup):
+        chain()
+    t_contract, t_solve, t_wall = [], [], []
+    for _ in range(args.steps):
+        tw, rows, st = chain()
+        t_contract.append(st["seconds_contract"])
+        t_solve.append(st["seconds_solve"])
+        t_wall.append(tw)
+    dt = float(np.mean(t_contract))
+    value = n_pairs / dt
+    # chain rows of window 1 are the window computed above (same fields, same form)
+    w1 = [r["kernel_sum"] for r in rows if r["window"] == 1]
+    chain_vs_window = float(np.max(np.abs(np.array(w1) - np.array(res1 if head == 1 else res2_raw))))
+    log(f"chain K=8 ({n_pairs} pairs), form {head}: contraction {dt:.3f} s ({value:.1f} pairs/s), "
+        f"solves {np.mean(t_solve):.2f} s, wall {np.mean(t_wall):.2f} s, cache peak {st['cache_peak']}; "
+        f"window-1 rows vs the window call: max |diff| {chain_vs_window:.1e}")
+
+    # roofline of the dominant kernel, pass 2 (kept-field window stages). HBM: the
+    # window's compulsory bytes, each of its 31 FP64 blocks read once (31 x 4608 V
+    # bytes). Compute: pass 2's nominal flops, per pair 2 V (8 x 6912 + 4 x 144 x 640)
+    # = 847,872 V (the dxu products and the 640 real 12x12 traces of both flavours),
+    # against the dense FP16 MFMA peak the traces run on (each product is issued as
+    # three FP16 products; counted once). The whole chain: 958,464 V per pair.
     V = lat.volume()
     bytes_window = 31 * 4608 * V
-    achieved = bytes_window / (ms[3] * 1e-3)
+    achieved = bytes_window / (ms1[3] * 1e-3)
     flops_pass2 = 65 * 847872.0 * V
-    flops_window = 65 * 958464.0 * V
+    flops_chain = n_pairs * 958464.0 * V
     traffic = measured_traffic("pass2_window")
     cpu = None if args.no_cpu else cpu_baseline(H)
     names = {1: "FP32 window-batched (pass-2 products as scaled FP16 hi/lo x3), plain QED kernel",
@@ -321,28 +338,29 @@ def main():
         "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp16x3/fp32", "data": "synthetic",
-        "config": {"workload": "24^3x48 window 1 of chain K=8 n=3 h=(1,1,1,1): 65 pairs x 2 flavours, "
-                               f"30 resident FP64 blocks, seed 2; {names[head]}",
-                   "lattice": list(DIMS), "pairs_per_step": 65, "xunit": XUNIT, "form": head},
+        "config": {"workload": "24^3x48 chain K=8 n=3 h=(1,1,1,1), seed 2: 535 pairs x 2 flavours, one "
+                               f"window call per window (run_chain, device tier); {names[head]}",
+                   "lattice": list(DIMS), "pairs_per_step": n_pairs, "xunit": XUNIT, "form": head},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": MI355X_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / MI355X_HBM, "traffic": traffic,
-                     "kernel": "pass2_window", "ms": ms[3], "bytes": bytes_window},
-        "compute": {"bound": "mfma", "kernel": "pass2_window", "achieved": flops_pass2 / (ms[3] * 1e-3) / 1e12,
+                     "kernel": "pass2_window", "ms": ms1[3], "bytes": bytes_window},
+        "compute": {"bound": "mfma", "kernel": "pass2_window", "achieved": flops_pass2 / (ms1[3] * 1e-3) / 1e12,
                     "peak": MI355X_FP16_MFMA / 1e12, "unit": "TFLOP/s",
-                    "frac": flops_pass2 / (ms[3] * 1e-3) / MI355X_FP16_MFMA, "flops": flops_pass2,
-                    "window_tflops": flops_window / dt / 1e12},
-        "surrogate_within_bound": bool(not fell_back and err2_raw.max() <= 1e-3),
+                    "frac": flops_pass2 / (ms1[3] * 1e-3) / MI355X_FP16_MFMA, "flops": flops_pass2,
+                    "chain_tflops": flops_chain / dt / 1e12},
+        "chain_seconds_solve": float(np.mean(t_solve)), "chain_cache_peak": st["cache_peak"],
+        "surrogate_within_bound": bool(head == 2),
         "form2_fell_back_to_form1": bool(fell_back),
         "surrogate_raw_max_pair_rel_err": float(err2_raw.max()),
         "surrogate_raw_median_pair_rel_err": float(np.median(err2_raw)),
-        "max_pair_rel_err": float(err),
+        "max_pair_rel_err": float(err1.max() if head == 1 else err2_raw.max()),
+        "median_pair_rel_err": float(np.median(err1 if head == 1 else err2_raw)),
         "plain_fp64_pairs_per_s": 1.0 / dt64,
         "speedup_vs_plain_fp64": value * dt64,
-        "form1_pairs_per_s": 65.0 / dt1, "form1_max_pair_rel_err": float(err1.max()),
-        "form2_pairs_per_s": 65.0 / dt2, "form2_max_pair_rel_err": float(err2.max()),
-        "form2_median_pair_rel_err": float(np.median(err2)),
+        "window1_form1_pairs_per_s": 65.0 / dt1_w, "window1_form1_max_pair_rel_err": float(err1.max()),
+        "window1_form2_max_pair_rel_err": float(err2.max()),
         "surrogate_judge_max_rel": judge, "surrogate_train_s": t_train,
-        "stage_ms_form1": ms1, "stage_ms_form2": ms2,
+        "stage_ms_form1_kept_fields": ms1, "stage_ms_form1_cold": ms1_cold, "stage_ms_form2_cold": ms2_cold,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

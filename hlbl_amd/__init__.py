@@ -64,6 +64,8 @@ _SIGS = {
     "hlbl_qed_init": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
     "hlbl_window_set_judge_bound": (C.c_int, [C.c_void_p, C.c_double]),
     "hlbl_window_fallback": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "hlbl_window_drop_fields": (C.c_int, [C.c_void_p]),
+    "hlbl_window_fields_built": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "hlbl_qed_init_file": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_char_p]),
     "hlbl_qed_exact": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
     "hlbl_qed_free": (C.c_int, [C.c_void_p]),
@@ -370,6 +372,16 @@ class Contractor:
     def set_judge_bound(self, bound):
         """Form 2 returns the plain kernel's results (form 1) when the judge exceeds bound."""
         check(lib().hlbl_window_set_judge_bound(self.handle, float(bound)))
+
+    def drop_fields(self):
+        """Forget the kept coefficient fields (the next window call builds its own)."""
+        check(lib().hlbl_window_drop_fields(self.handle))
+
+    def fields_built(self):
+        """Coefficient fields the last window call built (not served by kept ones)."""
+        n = C.c_int()
+        check(lib().hlbl_window_fields_built(self.handle, C.byref(n)))
+        return n.value
 
     def fell_back(self):
         """Whether the last form-2 window returned the plain kernel's results."""

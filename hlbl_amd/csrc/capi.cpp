@@ -517,6 +517,8 @@ void window_run(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double x
 void window_judge_result(const hlbl_contractor* c, double* err, int* n);
 void window_set_judge_bound(hlbl_contractor* c, double bound);
 void window_fallback_result(const hlbl_contractor* c, int* fell_back);
+void window_drop_fields(hlbl_contractor* c);
+int window_fields_built(const hlbl_contractor* c);
 
 
 // a row as the C ABI hands it out: x[4], y[4], yv[4], window, chain, ks0, ks1
@@ -591,6 +593,21 @@ int hlbl_window_fallback(const hlbl_contractor* c, int* fell_back) {
   });
 }
 
+int hlbl_window_drop_fields(hlbl_contractor* c) {
+  return guarded([&] {
+    check_ptr(c, "contractor");
+    window_drop_fields(c);
+  });
+}
+
+int hlbl_window_fields_built(const hlbl_contractor* c, int* n) {
+  return guarded([&] {
+    check_ptr(c, "contractor");
+    check_ptr(n, "n");
+    *n = window_fields_built(c);
+  });
+}
+
 int hlbl_mlp_param_count(const hlbl_qed* q) { return q ? mlp_layout().total : -1; }
 
 int hlbl_mlp_init(hlbl_qed* q, uint64_t seed) {
@@ -599,6 +616,7 @@ int hlbl_mlp_init(hlbl_qed* q, uint64_t seed) {
     check_hip(hipSetDevice(q->device), "hipSetDevice");
     mlp_init_params(q, seed);
     q->mlp_ready = false;
+    ++q->mlp_gen;
   });
 }
 
@@ -620,6 +638,7 @@ int hlbl_mlp_set_domain(hlbl_qed* q, double radius) {
       throw Error("hlbl_mlp_set_domain: need 0 < radius <= the kernel radius");
     q->mlp_radius = radius;
     q->mlp_ready = false;
+    ++q->mlp_gen;
   });
 }
 
@@ -644,6 +663,7 @@ int hlbl_mlp_set_params(hlbl_qed* q, const float* host, int n) {
     check_hip(hipMemcpy(q->mlp_w.p, host, total * sizeof(float), hipMemcpyHostToDevice), "copy params");
     mlp_requantise(q, nullptr);
     check_hip(hipDeviceSynchronize(), "requantise");
+    ++q->mlp_gen;
   });
 }
 

@@ -103,6 +103,7 @@ struct hlbl_qed {
   hlbl::DevBuf<unsigned short> mlp_w_bf16;
   hlbl::DevBuf<float> mlp_scale;  // [2][kMlpOut]: offset, scale (normalised -> physical)
   bool mlp_ready = false;
+  uint64_t mlp_gen = 0;  // bumped whenever the network or its domain changes
   double mlp_radius = hlbl::qed::kRadius;  // trained domain: |x|, |y| <= mlp_radius (muon units)
   double* noalloc_scratch = nullptr;  // this device's fixed scratch array (qed_noalloc.hip)
 };

@@ -653,6 +653,7 @@ void mlp_train(hlbl_qed* q, int n_samples, int epochs, int batch, float lr, uint
   mlp_requantise(q, st);
   check_hip(hipStreamSynchronize(st), "mlp train");
   q->mlp_ready = true;
+  ++q->mlp_gen;
 }
 
 }  // namespace hlbl

@@ -22,17 +22,24 @@ struct WinPair {
   const double* Y;   // fwd_y: block of y
   hlbl_coord gsx;
   hlbl_coord yv;
-  int yv_slot;       // index of yv among the window's distinct yv
+  int yv_slot;       // the slot of yv's coefficient field in the field store
   int pad;
 };
 
+// A window call reads one coefficient field per distinct yv from the contractor's field
+// store and builds the ones the store lacks: n_unique of them, listed in d_yvs with
+// their store slots in d_slots.
 struct WindowPlan {
   int n_pairs = 0, n_unique = 0, form = 0, n_samples = 0;
   double xunit = 0.0;
   DevBuf<WinPair> d_pairs;
-  DevBuf<hlbl_coord> d_yvs;
+  DevBuf<hlbl_coord> d_yvs;  // the yv whose fields this call builds
+  DevBuf<int> d_slots;       // their slots in the store
   PinBuf<WinPair> h_pairs;
   PinBuf<hlbl_coord> h_yvs;
+  PinBuf<int> h_slots;
+  std::vector<int> call_slots;  // the store slots of the call's distinct yv
+  float* coef = nullptr;        // the field store [slot][V][640]
   int unit_slots = 0;      // pass-1 unit lists: kPass1Groups x unit_slots (pair * 2 + f, or -1)
   DevBuf<int> d_units;
   PinBuf<int> h_units;
@@ -40,7 +47,6 @@ struct WindowPlan {
   DevBuf<double> red1, xi, dz, gdz;
   DevBuf<unsigned short> bsplit;  // pass-2 B operands, FP16 hi/lo in MFMA fragment order
   DevBuf<int> bexp;               // their column exponents [pair][f][160]
-  DevBuf<float> coef;      // [n_unique][V][640]
   DevBuf<float> ax;        // L0(x, 0) and L0(0, x) of every site [2][V][384]
   DevBuf<double> cst64;    // pair constants L0(0,y), L0(y,0) per distinct yv [n_unique][768]
   DevBuf<float> cst32;
@@ -48,7 +54,7 @@ struct WindowPlan {
   DevBuf<double> part2, result;
   PinBuf<double> h_result;
   DevBuf<size_t> d_samples;
-  DevBuf<double> judge;    // [n_unique][n_samples]
+  DevBuf<double> judge;    // [n_unique][n_samples] of the fields built
   PinBuf<double> h_judge;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -64,9 +70,9 @@ struct WindowPlan {
   }
 };
 
-// kernels of a window on `st` (recorded into the graph); stage 0 = pass 1 and its
-// reductions, 1 = coefficient fields (and the judge), 2 = pass 2 and its reduction
-void window_record(hlbl_contractor* c, WindowPlan& w, hipStream_t st);
+// kernels of a window on `st`; stage 0 = pass 1 and its reductions (recorded into the
+// window's graph), 1 = the coefficient fields the store lacks (and their judge), 2 =
+// pass 2 and its reduction (recorded)
 void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st);
 size_t window_part2_len(const hlbl_contractor* c, int n_pairs);
 size_t window_bsplit_len(int n_pairs);  // FP16 elements of the pass-2 B operands

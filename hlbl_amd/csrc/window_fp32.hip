@@ -102,8 +102,8 @@ __global__ __launch_bounds__(kKmThreads) void kfield_axes(const float* __restric
 // (x, y), (y, x) and (x, d) are evaluated; (x, 0) and (0, x) are this site's rows
 // and (0, d) is the (0, x) row of the site d = x - y.
 __global__ __launch_bounds__(kKxThreads, 4) void kfield_mixed(
-    const float* __restrict__ table32, const hlbl_coord* __restrict__ yvs, int n_unique,
-    hlbl_geom global, double xunit, const float* __restrict__ cst32,
+    const float* __restrict__ table32, const hlbl_coord* __restrict__ yvs, const int* __restrict__ slots,
+    int n_unique, hlbl_geom global, double xunit, const float* __restrict__ cst32,
     const float* __restrict__ chbuf, const float* __restrict__ ax, float* __restrict__ coef) {
   static_assert(sizeof(TermLds<float>) + kKxThreads / 64 * kKwLen3 * sizeof(float) <= 163840 / 2,
                 "two workgroups per CU");
@@ -138,7 +138,8 @@ __global__ __launch_bounds__(kKxThreads, 4) void kfield_mixed(
     }
     kfield_wave_axes<float, float>(table32, tt, g, rows, cst32 + (size_t)u * 2 * qed::kNL,
                                           ax + site * qed::kNL, ax + (V + site) * qed::kNL,
-                                          ax + (V + sd) * qed::kNL, w[wave], coef + i * kCoefField);
+                                          ax + (V + sd) * qed::kNL, w[wave],
+                                          coef + ((size_t)slots[u] * V + site) * kCoefField);
   }
 }
 
@@ -147,8 +148,8 @@ __global__ __launch_bounds__(kKxThreads, 4) void kfield_mixed(
 // 1) in every direction; a candidate with |x| or |x - y| below kNearRadius is
 // recomputed (a site in both boxes twice, with the same result).
 __global__ __launch_bounds__(kKmThreads) void kfield_near(
-    const double* __restrict__ table64, const hlbl_coord* __restrict__ yvs, int n_unique,
-    hlbl_geom global, double xunit, const double* __restrict__ cst64, hlbl_coord box,
+    const double* __restrict__ table64, const hlbl_coord* __restrict__ yvs, const int* __restrict__ slots,
+    int n_unique, hlbl_geom global, double xunit, const double* __restrict__ cst64, hlbl_coord box,
     float* __restrict__ coef) {
   __shared__ TermLds<double> tt;
   __shared__ double w[kKmThreads / 64][kKwLen];
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(kKmThreads) void kfield_near(
     if (!(norm2(g.xm) < near2 || norm2(g.dm) < near2)) continue;  // wave-uniform
     kfield_wave<double, double, float>(table64, tt, g, ChannelRows::none(),
                                        cst64 + (size_t)u * 2 * qed::kNL, w[wave],
-                                       coef + ((size_t)u * V + site) * kCoefField);
+                                       coef + ((size_t)slots[u] * V + site) * kCoefField);
   }
 }
 
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(kKmThreads) void kfield_near(
 __global__ __launch_bounds__(64) void judge_coef(
     const double* __restrict__ table64, const hlbl_coord* __restrict__ yvs, hlbl_geom global,
     double xunit, const double* __restrict__ cst64, const size_t* __restrict__ samples,
-    int n_samples, const float* __restrict__ coef, double* __restrict__ err) {
+    int n_samples, const int* __restrict__ slots, const float* __restrict__ coef, double* __restrict__ err) {
   __shared__ TermLds<double> tt;
   __shared__ double w[kKwLen];
   __shared__ double ref[kCoefField];
@@ -203,7 +204,7 @@ __global__ __launch_bounds__(64) void judge_coef(
   site_geom(site, global, global, zero, zero, yvs[u], xunit, g);
   kfield_wave<double, double, double>(table64, tt, g, ChannelRows::none(),
                                       cst64 + (size_t)u * 2 * qed::kNL, w, ref);
-  const float* c = coef + ((size_t)u * V + site) * kCoefField;
+  const float* c = coef + ((size_t)slots[u] * V + site) * kCoefField;
   double dmax = 0.0, rmax = 0.0;
   for (int t = lane; t < kCoefField; t += 64) {  // the entries this lane wrote
     dmax = fmax(dmax, fabs((double)c[t] - ref[t]));
@@ -655,12 +656,6 @@ __global__ void reduce_window2(const double* __restrict__ part, int nwg, int n_p
 
 // ------------------------------------------------------------- launchers --
 
-void window_record(hlbl_contractor* c, WindowPlan& w, hipStream_t st) {
-  window_stage(c, w, 0, st);
-  window_stage(c, w, 1, st);
-  window_stage(c, w, 2, st);
-}
-
 void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) {
   const hlbl_geom g = c->global;
   const size_t V = c->volume;
@@ -675,8 +670,8 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
                        w.bexp.p, w.bsplit.p);
     check_hip(hipGetLastError(), "form_bsplit launch");
   }
-  // coefficient fields, one per distinct yv
-  if (stage == 1) {
+  // the coefficient fields to build, one per listed yv
+  if (stage == 1 && nu > 0) {
     hipLaunchKernelGGL(kfield_consts_win, dim3(nu), dim3(64), 0, st, c->qed->table64.p, w.d_yvs.p, g,
                        w.xunit, w.cst64.p, w.cst32.p);
     check_hip(hipGetLastError(), "kfield_consts_win launch");
@@ -686,8 +681,8 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
     check_hip(hipGetLastError(), "kfield_axes launch");
     const int nwg = (int)std::min<size_t>(div_up((size_t)nu * V, kKxThreads / 64), (size_t)2 * c->n_cu);
     hipLaunchKernelGGL(kfield_mixed, dim3(nwg), dim3(kKxThreads), 0, st, c->qed->table32.p,
-                       w.d_yvs.p, nu, g, w.xunit, w.cst32.p, w.form == 2 ? w.chbuf.p : nullptr,
-                       w.ax.p, w.coef.p);
+                       w.d_yvs.p, w.d_slots.p, nu, g, w.xunit, w.cst32.p, w.form == 2 ? w.chbuf.p : nullptr,
+                       w.ax.p, w.coef);
     check_hip(hipGetLastError(), "kfield_mixed launch");
     // candidate boxes: ceil(kNearRadius / xunit) lattice units, at most the whole extent
     hlbl_coord box;
@@ -697,12 +692,12 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
     box.y = std::min(hb, (g.LY - 1) / 2);
     box.z = std::min(hb, (g.LZ - 1) / 2);
     hipLaunchKernelGGL(kfield_near, dim3((unsigned)c->n_cu), dim3(kKmThreads), 0, st, c->qed->table64.p,
-                       w.d_yvs.p, nu, g, w.xunit, w.cst64.p, box, w.coef.p);
+                       w.d_yvs.p, w.d_slots.p, nu, g, w.xunit, w.cst64.p, box, w.coef);
     check_hip(hipGetLastError(), "kfield_near launch");
     if (w.form == 2) {
       hipLaunchKernelGGL(judge_coef, dim3(w.n_samples, nu), dim3(64), 0, st, c->qed->table64.p,
-                         w.d_yvs.p, g, w.xunit, w.cst64.p, w.d_samples.p, w.n_samples, w.coef.p,
-                         w.judge.p);
+                         w.d_yvs.p, g, w.xunit, w.cst64.p, w.d_samples.p, w.n_samples, w.d_slots.p,
+                         w.coef, w.judge.p);
       check_hip(hipGetLastError(), "judge_coef launch");
     }
   }
@@ -712,7 +707,7 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
     const size_t spw = div_up(div_up(V, (size_t)nwg), kS2Sites) * kS2Sites;
     const int n = (int)div_up(V, spw);
     hipLaunchKernelGGL(pass2_window, dim3(n), dim3(kS2Threads), 0, st, w.d_pairs.p, np, w.bsplit.p, w.bexp.p,
-                       w.coef.p, g, spw, n, w.part2.p);
+                       w.coef, g, spw, n, w.part2.p);
     check_hip(hipGetLastError(), "pass2_window launch");
     hipLaunchKernelGGL(reduce_window2, dim3((unsigned)div_up(2 * np, 64)), dim3(64), 0, st,
                        w.part2.p, n, np, w.result.p);

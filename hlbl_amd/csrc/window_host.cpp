@@ -113,13 +113,40 @@ void pass1_group_units(const WinPair* pairs, int n_pairs, int* units, int unit_s
 // the plain kernel (form 1).
 constexpr double kDefaultJudgeBound = 1e-3;
 
+// Coefficient fields kept across calls. A field depends on (yv, xunit, form) and, in the
+// surrogate form, on the network (its generation counter) only, so the windows of a
+// chain, which see the same distinct yv (the schedule is the same hypercube shifted),
+// build them once. Slots are reused least recently used first.
+struct FieldStore {
+  struct Key {
+    hlbl_coord yv;
+    double xunit;
+    int form;
+    uint64_t gen;
+  };
+  DevBuf<float> coef;  // [cap][V][640]
+  int cap = 0;
+  std::vector<Key> key;
+  std::vector<char> valid;
+  std::vector<long> used;
+  std::vector<double> judge;  // the surrogate judge of each form-2 field
+  long clock = 0;
+  bool matches(int s, const Key& k) const {
+    const Key& a = key[s];
+    return valid[s] && same(a.yv, k.yv) && a.xunit == k.xunit && a.form == k.form && a.gen == k.gen;
+  }
+  void drop() { std::fill(valid.begin(), valid.end(), 0); }
+};
+
 struct WindowCache {
   std::vector<std::unique_ptr<WindowPlan>> plans;
+  FieldStore fields;
   hipStream_t stream = nullptr;
   double last_judge = -1.0;
   int last_judge_n = 0;
   double judge_bound = kDefaultJudgeBound;
   int last_fallback = 0;  // the last form-2 window returned form 1's results
+  int last_built = 0;     // fields the last call built
   ~WindowCache() {
     plans.clear();
     if (stream) (void)hipStreamDestroy(stream);
@@ -143,6 +170,12 @@ void window_set_judge_bound(hlbl_contractor* c, double bound) {
   if (!(bound > 0)) throw Error("hlbl_window_set_judge_bound: the bound must be positive");
   window_cache(c)->judge_bound = bound;
 }
+
+void window_drop_fields(hlbl_contractor* c) {
+  if (c->window) c->window->fields.drop();
+}
+
+int window_fields_built(const hlbl_contractor* c) { return c->window ? c->window->last_built : 0; }
 
 void window_fallback_result(const hlbl_contractor* c, int* fell_back) {
   if (!c->window || c->window->last_judge < 0) throw Error("hlbl_window_fallback: no surrogate window has run");
@@ -174,27 +207,72 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
     slot[i] = s;
   }
   const int nu = (int)yvs.size();
+  const size_t V = c->volume;
+
+  // the field store: at least one slot per distinct yv of the call (growing it drops
+  // the cached fields and every captured graph, which hold its address)
+  FieldStore& fs = cache->fields;
+  if (fs.cap < nu) {
+    fs.coef.release();
+    fs.coef.ensure((size_t)nu * V * 640);
+    fs.cap = nu;
+    fs.key.assign(nu, FieldStore::Key{});
+    fs.valid.assign(nu, 0);
+    fs.used.assign(nu, 0);
+    fs.judge.assign(nu, 0.0);
+    for (auto& pl : cache->plans) pl->drop_graph();
+  }
+  const uint64_t gen = form == 2 ? c->qed->mlp_gen : 0;
+  std::vector<int> call_slot(nu, -1);
+  std::vector<char> taken(fs.cap, 0);
+  for (int u = 0; u < nu; ++u) {
+    const FieldStore::Key k{yvs[u], xunit, form, gen};
+    for (int s = 0; s < fs.cap; ++s)
+      if (!taken[s] && fs.matches(s, k)) {
+        call_slot[u] = s;
+        taken[s] = 1;
+        break;
+      }
+  }
+  std::vector<int> build;  // distinct yv (indices u) whose fields this call builds
+  for (int u = 0; u < nu; ++u) {
+    if (call_slot[u] >= 0) continue;
+    int best = -1;
+    for (int s = 0; s < fs.cap; ++s) {
+      if (taken[s]) continue;
+      if (!fs.valid[s]) {
+        best = s;
+        break;
+      }
+      if (best < 0 || fs.used[s] < fs.used[best]) best = s;
+    }
+    call_slot[u] = best;
+    taken[best] = 1;
+    fs.valid[best] = 0;  // valid again once its build has run
+    fs.key[best] = FieldStore::Key{yvs[u], xunit, form, gen};
+    build.push_back(u);
+  }
+  for (int u = 0; u < nu; ++u) fs.used[call_slot[u]] = ++fs.clock;
+  const int nb = (int)build.size();
+  cache->last_built = nb;
 
   // One plan whose buffers only grow: a window of another size or form reuses them and
   // its graph is captured again, so the footprint is that of the largest window so far
   // and a run of differently sized batches does not reallocate.
   if (cache->plans.empty()) cache->plans.push_back(std::make_unique<WindowPlan>());
   WindowPlan* w = cache->plans.front().get();
-  if (!(w->n_pairs == n && w->n_unique == nu && w->form == form && w->xunit == xunit)) {
+  if (!(w->n_pairs == n && w->form == form && w->xunit == xunit && w->coef == fs.coef.p)) {
     w->drop_graph();
     w->n_pairs = n;
-    w->n_unique = nu;
     w->form = form;
     w->xunit = xunit;
     w->n_samples = kJudgeSamples;
-    const size_t V = c->volume;
+    w->coef = fs.coef.p;
     w->d_pairs.ensure(n);
     w->h_pairs.ensure(n);
     w->unit_slots = (2 * n + kPass1Groups - 1) / kPass1Groups;
     w->d_units.ensure((size_t)kPass1Groups * w->unit_slots);
     w->h_units.ensure((size_t)kPass1Groups * w->unit_slots);
-    w->d_yvs.ensure(nu);
-    w->h_yvs.ensure(nu);
     w->part1.ensure(pass1_part_len(c->global, n));
     w->red1.ensure(pass1_red_len(c->global, n));
     w->xi.ensure((size_t)n * 2 * 8 * 5 * 144 * 2);
@@ -202,11 +280,6 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
     w->gdz.ensure((size_t)n * 2 * 40 * 144 * 2);
     w->bsplit.ensure(window_bsplit_len(n));
     w->bexp.ensure(window_bexp_len(n));
-    w->coef.ensure((size_t)nu * V * 640);
-    w->ax.ensure((size_t)2 * V * qed::kNL);
-    w->cst64.ensure((size_t)nu * 2 * qed::kNL);
-    w->cst32.ensure((size_t)nu * 2 * qed::kNL);
-    if (form == 2) w->chbuf.ensure((size_t)(2 + 3 * nu) * V * qed::kNCh);
     w->part2.ensure(window_part2_len(c, n));
     w->result.ensure((size_t)2 * n);
     w->h_result.ensure((size_t)2 * n);
@@ -218,56 +291,109 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
       check_hip(hipMemcpy(w->d_samples.p, smp.data(), smp.size() * sizeof(size_t), hipMemcpyHostToDevice),
                 "hipMemcpy samples");
     }
-    w->judge.ensure((size_t)nu * kJudgeSamples);
-    w->h_judge.ensure((size_t)nu * kJudgeSamples);
   }
+  // the builds: their yv, slots and work buffers (grown as needed, outside the graph)
+  w->n_unique = nb;
+  if (nb > 0) {
+    w->d_yvs.ensure(nb);
+    w->h_yvs.ensure(nb);
+    w->d_slots.ensure(nb);
+    w->h_slots.ensure(nb);
+    w->ax.ensure((size_t)2 * V * qed::kNL);
+    w->cst64.ensure((size_t)nb * 2 * qed::kNL);
+    w->cst32.ensure((size_t)nb * 2 * qed::kNL);
+    if (form == 2) w->chbuf.ensure((size_t)(2 + 3 * nb) * V * qed::kNCh);
+    w->judge.ensure((size_t)nb * kJudgeSamples);
+    w->h_judge.ensure((size_t)nb * kJudgeSamples);
+    for (int j = 0; j < nb; ++j) {
+      w->h_yvs.p[j] = yvs[build[j]];
+      w->h_slots.p[j] = call_slot[build[j]];
+    }
+  }
+  w->call_slots = call_slot;
   for (int i = 0; i < n; ++i) {
     // gsx inside the lattice: the kernels step site coordinates relative to it
     const hlbl_coord gs = {mod_pos(pairs[i].gsx.t, g.T), mod_pos(pairs[i].gsx.x, g.LX),
                            mod_pos(pairs[i].gsx.y, g.LY), mod_pos(pairs[i].gsx.z, g.LZ)};
-    w->h_pairs.p[i] = WinPair{pairs[i].d_x, pairs[i].d_y, gs, pairs[i].yv, slot[i], 0};
+    w->h_pairs.p[i] = WinPair{pairs[i].d_x, pairs[i].d_y, gs, pairs[i].yv, call_slot[slot[i]], 0};
   }
-  for (int u = 0; u < nu; ++u) w->h_yvs.p[u] = yvs[u];
   pass1_group_units(w->h_pairs.p, n, w->h_units.p, w->unit_slots);
   if (!cache->stream) check_hip(hipStreamCreateWithFlags(&cache->stream, hipStreamNonBlocking), "hipStreamCreate");
   return w;
 }
 
 namespace {
+// results; with the surrogate, the judge of the call's fields (built now or before)
 void window_finish(hlbl_contractor* c, WindowPlan* w, double* out) {
   for (int i = 0; i < 2 * w->n_pairs; ++i) out[i] = w->h_result.p[i];
+  FieldStore& fs = c->window->fields;
+  for (int j = 0; j < w->n_unique; ++j) {
+    const int s = w->h_slots.p[j];
+    double e = 0.0;
+    if (w->form == 2)
+      for (int k = 0; k < w->n_samples; ++k) e = std::max(e, w->h_judge.p[(size_t)j * w->n_samples + k]);
+    fs.judge[s] = e;
+    fs.valid[s] = 1;
+  }
   if (w->form == 2) {
     double e = 0.0;
-    for (int k = 0; k < w->n_unique * w->n_samples; ++k) e = std::max(e, w->h_judge.p[k]);
+    for (int s : w->call_slots) e = std::max(e, fs.judge[s]);
     c->window->last_judge = e;
-    c->window->last_judge_n = w->n_unique * w->n_samples;
+    c->window->last_judge_n = (int)w->call_slots.size() * w->n_samples;
   }
 }
 void window_tail_copies(WindowPlan* w, hipStream_t st) {
   check_hip(hipMemcpyAsync(w->h_result.p, w->result.p, 2 * w->n_pairs * sizeof(double), hipMemcpyDeviceToHost, st), "copy result");
-  if (w->form == 2)
-    check_hip(hipMemcpyAsync(w->h_judge.p, w->judge.p, (size_t)w->n_unique * w->n_samples * sizeof(double),
-                             hipMemcpyDeviceToHost, st), "copy judge");
 }
 void window_head_copies(WindowPlan* w, hipStream_t st) {
   check_hip(hipMemcpyAsync(w->d_pairs.p, w->h_pairs.p, w->n_pairs * sizeof(WinPair), hipMemcpyHostToDevice, st), "copy pairs");
-  check_hip(hipMemcpyAsync(w->d_yvs.p, w->h_yvs.p, w->n_unique * sizeof(hlbl_coord), hipMemcpyHostToDevice, st), "copy yv");
   check_hip(hipMemcpyAsync(w->d_units.p, w->h_units.p, (size_t)kPass1Groups * w->unit_slots * sizeof(int),
                            hipMemcpyHostToDevice, st), "copy units");
 }
+// the fields the store lacks: their list, the surrogate's channels (form 2), the fields
+// and the judge, launched directly on st (not part of the window's graph)
+void window_builds(hlbl_contractor* c, WindowPlan* w, hipStream_t st, hipEvent_t mid) {
+  const int nb = w->n_unique;
+  if (nb == 0) {
+    if (mid) check_hip(hipEventRecord(mid, st), "event");
+    return;
+  }
+  check_hip(hipMemcpyAsync(w->d_yvs.p, w->h_yvs.p, nb * sizeof(hlbl_coord), hipMemcpyHostToDevice, st), "copy yv");
+  check_hip(hipMemcpyAsync(w->d_slots.p, w->h_slots.p, nb * sizeof(int), hipMemcpyHostToDevice, st), "copy slots");
+  if (w->form == 2) launch_surrogate_channels(c, *w, st);
+  if (mid) check_hip(hipEventRecord(mid, st), "event");
+  window_stage(c, *w, 1, st);
+  if (w->form == 2)
+    check_hip(hipMemcpyAsync(w->h_judge.p, w->judge.p, (size_t)nb * w->n_samples * sizeof(double),
+                             hipMemcpyDeviceToHost, st), "copy judge");
+}
+// a new window's first call leaves the store without the fields it was building when
+// anything fails before they are complete
+struct BuildGuard {
+  hlbl_contractor* c;
+  WindowPlan* w;
+  bool done = false;
+  ~BuildGuard() {
+    if (done) return;
+    FieldStore& fs = c->window->fields;
+    for (int j = 0; j < w->n_unique; ++j) fs.valid[w->h_slots.p[j]] = 0;
+  }
+};
 }  // namespace
 
 void window_run(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit, int form,
                 double* out, hipStream_t user_stream) {
   WindowPlan* w = window_prepare(c, pairs, n, xunit, form);
+  BuildGuard guard{c, w};
   hipStream_t st = c->window->stream;
-  // capture once per plan; graphs need a stream that is not the null stream
+  // capture once per plan (pass 1 and pass 2); graphs need a stream that is not the
+  // null stream
   if (!w->exec) {
     check_hip(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
       window_head_copies(w, st);
-      if (form == 2) launch_surrogate_channels(c, *w, st);
-      window_record(c, *w, st);
+      window_stage(c, *w, 0, st);
+      window_stage(c, *w, 2, st);
       window_tail_copies(w, st);
     } catch (...) {
       hipGraph_t gr = nullptr;
@@ -281,8 +407,10 @@ void window_run(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double x
   // order after the caller's earlier work on its stream
   if (user_stream) check_hip(hipStreamSynchronize(user_stream), "user stream");
   else check_hip(hipDeviceSynchronize(), "device");
+  window_builds(c, w, st, nullptr);
   check_hip(hipGraphLaunch(w->exec, st), "hipGraphLaunch");
   check_hip(hipStreamSynchronize(st), "window");
+  guard.done = true;
   window_finish(c, w, out);
   if (form == 2) {
     // the judge as a gate: past the bound, the plain kernel's results instead
@@ -303,6 +431,7 @@ void window_profile(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, doub
                     double* out, double ms[4]) {
   if (form != 1 && form != 2) throw Error("hlbl_window_profile: form must be 1 or 2");
   WindowPlan* w = window_prepare(c, pairs, n, xunit, form);
+  BuildGuard guard{c, w};
   hipStream_t st = c->window->stream;
   check_hip(hipDeviceSynchronize(), "device");
   hipEvent_t ev[5];
@@ -312,9 +441,7 @@ void window_profile(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, doub
     check_hip(hipEventRecord(ev[0], st), "event");
     window_stage(c, *w, 0, st);
     check_hip(hipEventRecord(ev[1], st), "event");
-    if (form == 2) launch_surrogate_channels(c, *w, st);
-    check_hip(hipEventRecord(ev[2], st), "event");
-    window_stage(c, *w, 1, st);
+    window_builds(c, w, st, ev[2]);
     check_hip(hipEventRecord(ev[3], st), "event");
     window_stage(c, *w, 2, st);
     check_hip(hipEventRecord(ev[4], st), "event");
@@ -330,6 +457,7 @@ void window_profile(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, doub
     throw;
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
+  guard.done = true;
   window_finish(c, w, out);
 }
 

@@ -165,6 +165,11 @@ int hlbl_4pt_contraction(hlbl_contractor* c, double* d_kernel_sum, const double*
  * domain). Pass 1 and pass 2 of all pairs each run as one persistent launch; the
  * sequence is captured as a hipGraph and replayed for later windows of the same size
  * and form (another size reuses the buffers, which only grow, and is captured again).
+ * The QED kernel enters through one coefficient field per distinct yv, which depends
+ * only on (yv, xunit, form, the surrogate's parameters): the contractor keeps them
+ * across calls (at least as many as the largest call's distinct yv, least recently
+ * used reused first), so later windows of a chain, which see the same yv, build none;
+ * a call builds the fields it lacks before its graph runs.
  * out is host [n][2]. No counterpart in the reference (window batching is this
  * library's addition); it replaces the pair loop of src/driver.cpp:110-132. */
 typedef struct {
@@ -184,10 +189,18 @@ int hlbl_window_judge(const hlbl_contractor* c, double* max_rel_err, int* n_samp
  * in the reference (the surrogate is this library's addition). */
 int hlbl_window_set_judge_bound(hlbl_contractor* c, double bound);
 int hlbl_window_fallback(const hlbl_contractor* c, int* fell_back);
+/* Forget the kept coefficient fields: the next window call builds all of its own. No
+ * counterpart in the reference. */
+int hlbl_window_drop_fields(hlbl_contractor* c);
+/* n = the number of coefficient fields the last window call built (its distinct yv
+ * that no kept field served). No counterpart in the reference. */
+int hlbl_window_fields_built(const hlbl_contractor* c, int* n);
 /* Measurement hook (no counterpart in the reference): runs a window of form 1 or 2
  * once without the graph, with HIP events around each stage on the launch stream;
  * ms = {pass 1 incl. its reductions, surrogate channels (0 for form 1),
- * coefficient fields, pass 2 incl. its reduction}. out as for the window entry. */
+ * coefficient fields, pass 2 incl. its reduction}; the surrogate channels and the
+ * fields cost nothing when the kept fields serve the call. out as for the window
+ * entry. */
 int hlbl_window_profile(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n_pairs,
                         double xunit, int form, double* out, double ms[4]);
 

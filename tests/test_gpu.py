@@ -359,6 +359,55 @@ def test_window_forms(qed):
     con.close()
 
 
+def test_window_field_store(qed):
+    """The kept coefficient fields: windows 1 and 2 of a chain see the same 15 distinct
+    yv, so window 2 after window 1 builds none and its sums equal those of window 2
+    with every field rebuilt, bit for bit; a call with one new yv builds one; a changed
+    xunit builds all again."""
+    dims = (8, 8, 8, 8)
+    lat = H.Lattice(dims)
+    con = H.Contractor(lat, qed)
+    xunit, seed = 0.3, 9
+    sch = H.Schedule((1, 1, 1, 1), 1, 2)
+    wrap = lambda v: tuple(a % L for a, L in zip(v, dims))
+    blocks = {}
+
+    def window(k):
+        pairs = [(q, p) for w, p, parts in sch.sources() if w == k for q in parts]
+        for q, p in pairs:
+            for pos in (q, p):
+                if wrap(pos) not in blocks:
+                    blocks[wrap(pos)] = dev_block(lat, H.source_seed(seed, wrap(pos)))
+        return [(blocks[wrap(q)].data_ptr(), blocks[wrap(p)].data_ptr(), wrap(q), lat.site_map_diff(p, q))
+                for q, p in pairs]
+
+    w1, w2 = window(1), window(2)
+    assert len(w1) == len(w2) == 65
+    yv1 = {lat.site_map_diff(d[3], (0, 0, 0, 0)) for d in w1}
+    assert yv1 == {lat.site_map_diff(d[3], (0, 0, 0, 0)) for d in w2}
+    con.window_kernel_sum(w1, xunit, form=1)
+    assert con.fields_built() == len(yv1)
+    kept = con.window_kernel_sum(w2, xunit, form=1)
+    assert con.fields_built() == 0
+    con.drop_fields()
+    fresh = con.window_kernel_sum(w2, xunit, form=1)
+    assert con.fields_built() == len(yv1)
+    assert kept == fresh
+    # one pair with a yv outside the set: one build
+    extra = w2[:3] + [(w2[0][0], w2[0][1], w2[0][2], (3, 0, 0, 0))]
+    assert (3, 0, 0, 0) not in yv1
+    part = con.window_kernel_sum(extra, xunit, form=1)
+    assert con.fields_built() == 1
+    assert part[:3] == fresh[:3]
+    con.window_kernel_sum(w2, xunit * 1.5, form=1)
+    assert con.fields_built() == len(yv1)
+    # the plain FP64 form of the same pairs is unaffected
+    ref = [con.kernel_sum(*d, xunit) for d in w2[:2]]
+    err = np.abs(np.array(fresh[:2]) - np.array(ref)) / np.abs(np.array(ref))
+    assert np.all(err < 1e-4)
+    con.close()
+
+
 @pytest.mark.parametrize("dims", [(6, 6, 6, 6), (6, 6, 6, 12)])
 def test_window_form1_short_lines(qed, dims):
     """Form 1 where a (t, x, y) line is not a whole number of the plane sums' 8-site
