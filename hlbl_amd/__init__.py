@@ -15,6 +15,7 @@ The library must be built (hlbl_amd.build); there is no CPU fallback.
 """
 import ctypes as C
 import math
+import json
 import os
 
 import numpy as np
@@ -37,6 +38,30 @@ class Coord(C.Structure):
 
 class Pair(C.Structure):
     _fields_ = [("a", C.c_double), ("b", C.c_double)]
+
+
+class Tuning(C.Structure):
+    """Launch shapes of a contractor's kernels (hlbl_tuning of include/hlbl_amd.h)."""
+    _fields_ = [(n, C.c_int) for n in ("pass1_groups", "field_waves", "field_wg_per_cu", "axes_wg_per_cu",
+                                        "near_wg_per_cu", "pass2_wg_per_cu", "fp64_field_wg_per_cu",
+                                        "fp64_pass2_wg_per_cu")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+TUNING_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "tuning_gfx950.json")
+
+
+def tuned_shapes(global_dims, path=TUNING_PATH):
+    """The recorded launch shapes for a lattice (from hlbl_amd.tune), or None."""
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    entry = table.get("lattices", {}).get("x".join(str(int(d)) for d in global_dims))
+    return None if entry is None else entry.get("picks")
 
 
 class PairDesc(C.Structure):
@@ -65,6 +90,9 @@ _SIGS = {
     "hlbl_window_set_judge_bound": (C.c_int, [C.c_void_p, C.c_double]),
     "hlbl_window_fallback": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "hlbl_window_drop_fields": (C.c_int, [C.c_void_p]),
+    "hlbl_tuning_defaults": (C.c_int, [C.c_void_p]),
+    "hlbl_contractor_set_tuning": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hlbl_contractor_get_tuning": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hlbl_window_fields_built": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "hlbl_qed_init_file": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_char_p]),
     "hlbl_qed_exact": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
@@ -320,7 +348,9 @@ class QedKernel:
 class Contractor:
     """The connected kernel sum of a pair (include/hlbl/contract.hpp:42-94)."""
 
-    def __init__(self, lattice, qed):
+    def __init__(self, lattice, qed, tuning="recorded"):
+        """tuning: "recorded" (the shapes hlbl_amd.tune recorded for this lattice, else the
+        defaults), "default", or a dict of hlbl_tuning fields (the rest default)."""
         h = C.c_void_p()
         check(lib().hlbl_contractor_create(C.byref(h), _geom(lattice.global_),
                                            _geom(lattice.local), _coord(lattice.proc_coords),
@@ -328,6 +358,25 @@ class Contractor:
         self.handle = h
         self.lattice = lattice
         self.qed = qed
+        if tuning == "recorded":
+            tuning = tuned_shapes(lattice.global_)
+        if isinstance(tuning, dict):
+            self.set_tuning(**tuning)
+
+    def tuning(self):
+        t = Tuning()
+        check(lib().hlbl_contractor_get_tuning(self.handle, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **fields):
+        """Set launch shapes (hlbl_tuning fields); unnamed fields keep the defaults."""
+        t = Tuning()
+        check(lib().hlbl_tuning_defaults(C.byref(t)))
+        for k, v in fields.items():
+            if k not in dict(Tuning._fields_):
+                raise HlblError(f"unknown tuning field {k}")
+            setattr(t, k, int(v))
+        check(lib().hlbl_contractor_set_tuning(self.handle, C.byref(t)))
 
     def close(self):
         if self.handle:

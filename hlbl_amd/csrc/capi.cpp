@@ -83,6 +83,8 @@ std::string default_table_path() {
 }
 
 }  // namespace
+
+void window_drop_graphs(hlbl_contractor* c);
 }  // namespace hlbl
 
 extern "C" {
@@ -276,7 +278,7 @@ int hlbl_contractor_create(hlbl_contractor** out, hlbl_geom global, hlbl_geom lo
       c->volume = geom_volume(local);
       check_hip(hipSetDevice(qed->device), "hipSetDevice");
       c->n_cu = device_cu_count(qed->device);
-      c->nwg2 = 2 * c->n_cu;
+      c->nwg2 = c->tune.fp64_pass2_wg_per_cu * c->n_cu;
       c->result.ensure(2);
       c->host_result.ensure(2);
     } catch (...) {
@@ -289,6 +291,42 @@ int hlbl_contractor_create(hlbl_contractor** out, hlbl_geom global, hlbl_geom lo
 
 int hlbl_contractor_free(hlbl_contractor* c) {
   return guarded([&] { delete c; });
+}
+
+int hlbl_tuning_defaults(hlbl_tuning* out) {
+  return guarded([&] {
+    check_ptr(out, "out");
+    *out = default_tuning();
+  });
+}
+
+int hlbl_contractor_set_tuning(hlbl_contractor* c, const hlbl_tuning* t) {
+  return guarded([&] {
+    check_ptr(c, "contractor");
+    check_ptr(t, "tuning");
+    auto in = [](int v, int lo, int hi, const char* what) {
+      if (v < lo || v > hi) throw Error(std::string("hlbl_contractor_set_tuning: ") + what + " out of range");
+    };
+    in(t->pass1_groups, 1, 64, "pass1_groups");
+    if (t->field_waves != 4 && t->field_waves != 8) throw Error("hlbl_contractor_set_tuning: field_waves must be 4 or 8");
+    in(t->field_wg_per_cu, 1, t->field_waves == 8 ? 2 : 3, "field_wg_per_cu");
+    in(t->axes_wg_per_cu, 1, 16, "axes_wg_per_cu");
+    in(t->near_wg_per_cu, 1, 8, "near_wg_per_cu");
+    in(t->pass2_wg_per_cu, 1, 4, "pass2_wg_per_cu");
+    in(t->fp64_field_wg_per_cu, 1, 16, "fp64_field_wg_per_cu");
+    in(t->fp64_pass2_wg_per_cu, 1, 8, "fp64_pass2_wg_per_cu");
+    c->tune = *t;
+    c->nwg2 = t->fp64_pass2_wg_per_cu * c->n_cu;
+    window_drop_graphs(c);
+  });
+}
+
+int hlbl_contractor_get_tuning(const hlbl_contractor* c, hlbl_tuning* out) {
+  return guarded([&] {
+    check_ptr(c, "contractor");
+    check_ptr(out, "out");
+    *out = c->tune;
+  });
 }
 
 int hlbl_kernel_sum(hlbl_contractor* c, const double* d_fwd_src, const double* d_fwd_y,

@@ -117,8 +117,13 @@ struct WindowCache;
 void window_cache_free(WindowCache* w);
 }  // namespace hlbl
 
+namespace hlbl {
+inline hlbl_tuning default_tuning() { return hlbl_tuning{8, 8, 2, 4, 1, 1, 2, 2}; }
+}  // namespace hlbl
+
 struct hlbl_contractor {
   hlbl_geom global{}, local{};
+  hlbl_tuning tune = hlbl::default_tuning();
   hlbl_coord proc{};
   hlbl_qed* qed = nullptr;
   size_t volume = 0;

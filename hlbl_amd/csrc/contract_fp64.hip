@@ -340,7 +340,7 @@ void launch_kfield(hlbl_contractor* c, hlbl_coord proc, hlbl_coord gsx, hlbl_coo
   hipLaunchKernelGGL(kfield_consts64, dim3(1), dim3(64), 0, st, c->qed->table64.p, global, yv, xunit,
                      c->kcst.p);
   check_hip(hipGetLastError(), "kfield_consts64 launch");
-  const int nwg = (int)std::min<size_t>(div_up(V, kKfThreads / 64), (size_t)2 * c->n_cu);
+  const int nwg = (int)std::min<size_t>(div_up(V, kKfThreads / 64), (size_t)c->tune.fp64_field_wg_per_cu * c->n_cu);
   hipLaunchKernelGGL(kfield_fp64, dim3(nwg), dim3(kKfThreads), 0, st, c->qed->table64.p, global,
                      local, proc, gsx, yv, xunit, c->kcst.p, c->coef.p);
   check_hip(hipGetLastError(), "kfield_fp64 launch");

@@ -21,7 +21,9 @@ namespace hlbl {
 
 // Blocks the schedule still needs. Device tier: blocks on the GPU, read in
 // place. Host tier: pinned host blocks, a device solve target and two device
-// staging slots (0 for y, 1 for x). A position is freed after its last pair.
+// staging slots (0 for y, 1 for x). A position is freed after its last pair; its
+// memory (device or pinned) is kept for the next position solved, so a run allocates
+// no more blocks than its cache peak.
 class PropagatorCache {
  public:
   PropagatorCache(bool host, size_t block_len, std::map<Vec4, int> pending)
@@ -31,8 +33,13 @@ class PropagatorCache {
     if (entries_.count(p)) throw Error("PropagatorCache: position inserted twice");
     if (!host_) {
       Entry& e = entries_[p];
-      e.dev = std::make_unique<DevBuf<double>>();
-      e.dev->ensure(len_);
+      if (dev_pool_.empty()) {
+        e.dev = std::make_unique<DevBuf<double>>();
+        e.dev->ensure(len_);
+      } else {
+        e.dev = std::move(dev_pool_.back());
+        dev_pool_.pop_back();
+      }
       return e.dev->p;
     }
     scratch_.ensure(len_);
@@ -96,6 +103,7 @@ class PropagatorCache {
     auto it = entries_.find(p);
     if (it != entries_.end()) {
       if (host_ && it->second.host) pool_.push_back(std::move(it->second.host));
+      if (!host_ && it->second.dev) dev_pool_.push_back(std::move(it->second.dev));
       entries_.erase(it);
     }
     for (auto& s : staged_)
@@ -107,6 +115,7 @@ class PropagatorCache {
   std::map<Vec4, Entry> entries_;
   int peak_ = 0;
   std::vector<std::unique_ptr<PinBuf<double>>> pool_;
+  std::vector<std::unique_ptr<DevBuf<double>>> dev_pool_;
   DevBuf<double> scratch_;
   DevBuf<double> staging_[2];
   std::optional<Vec4> staged_[2];

@@ -262,14 +262,14 @@ __device__ inline unsigned h1_imrow(unsigned d) { return __builtin_amdgcn_alignb
 
 __global__ __launch_bounds__(kH1Threads, 1) void pass1_mfma(
     const WinPair* __restrict__ pairs, WinPair single, hlbl_geom global, hlbl_geom local,
-    hlbl_coord proc, int flav_mask, const int* __restrict__ units, int unit_slots,
+    hlbl_coord proc, int flav_mask, const int* __restrict__ units, int unit_slots, int groups,
     float* __restrict__ part) {
   __shared__ __attribute__((aligned(16))) float sF[2][kH1F];
   __shared__ __attribute__((aligned(16))) _Float16 sH[2][kH1H];
   __shared__ unsigned sMax[4][2];  // per chunk c % 4: largest |S|, |Y| (float bits)
   int unit = blockIdx.x;
   if (units) {
-    unit = units[(blockIdx.x % kPass1Groups) * unit_slots + blockIdx.x / kPass1Groups];
+    unit = units[(blockIdx.x % groups) * unit_slots + blockIdx.x / groups];
     if (unit < 0) return;
   }
   const int ip = unit >> 1, f = unit & 1;
@@ -594,13 +594,13 @@ template <class T>
 void launch_pass1_planes(const WinPair* d_pairs, const WinPair& single, int n_pairs,
                          const hlbl_geom& global, const hlbl_geom& local, const hlbl_coord& proc,
                          int flav_mask, T* part, double* red1, double* xi, hipStream_t st,
-                         const int* d_units, int unit_slots) {
+                         const int* d_units, int unit_slots, int groups) {
   if (n_pairs <= 0) return;
   const int np = n_planes(local), nc = n_chunks(local);
   if constexpr (std::is_same<T, float>::value)
-    hipLaunchKernelGGL(pass1_mfma, dim3(d_units ? kPass1Groups * unit_slots : 2 * n_pairs, np),
+    hipLaunchKernelGGL(pass1_mfma, dim3(d_units ? groups * unit_slots : 2 * n_pairs, np),
                        dim3(kH1Threads), 0, st, d_pairs, single, global, local, proc, flav_mask,
-                       d_units, unit_slots, part);
+                       d_units, unit_slots, groups, part);
   else
     hipLaunchKernelGGL(pass1_planes<T>, dim3(2 * n_pairs, np), dim3(kPlThreads), 0, st, d_pairs,
                        single, global, local, proc, flav_mask, part);
@@ -615,9 +615,9 @@ void launch_pass1_planes(const WinPair* d_pairs, const WinPair& single, int n_pa
 
 template void launch_pass1_planes<double>(const WinPair*, const WinPair&, int, const hlbl_geom&,
                                           const hlbl_geom&, const hlbl_coord&, int, double*,
-                                          double*, double*, hipStream_t, const int*, int);
+                                          double*, double*, hipStream_t, const int*, int, int);
 template void launch_pass1_planes<float>(const WinPair*, const WinPair&, int, const hlbl_geom&,
                                          const hlbl_geom&, const hlbl_coord&, int, float*, double*,
-                                         double*, hipStream_t, const int*, int);
+                                         double*, hipStream_t, const int*, int, int);
 
 }  // namespace hlbl

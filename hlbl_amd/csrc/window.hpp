@@ -40,7 +40,7 @@ struct WindowPlan {
   PinBuf<int> h_slots;
   std::vector<int> call_slots;  // the store slots of the call's distinct yv
   float* coef = nullptr;        // the field store [slot][V][640]
-  int unit_slots = 0;      // pass-1 unit lists: kPass1Groups x unit_slots (pair * 2 + f, or -1)
+  int groups = 0, unit_slots = 0;  // pass-1 unit lists: groups x unit_slots (pair * 2 + f, or -1)
   DevBuf<int> d_units;
   PinBuf<int> h_units;
   DevBuf<float> part1;     // pass-1 plane partials

@@ -43,10 +43,9 @@ size_t div_up(size_t a, size_t b) { return (a + b - 1) / b; }
 // kNearRadius, a list built on the host) are then recomputed in FP64 with the
 // FP64 table, where the Lambda subtraction cancels.
 constexpr int kKmThreads = 256;
-// kfield_mixed: 8 waves share the term table; two workgroups per CU (four waves per
-// SIMD, which its launch bound asks for) fill the LDS exactly (term table 13,824 B +
-// 8 wave areas of 8,512 B = 81,920 B)
-constexpr int kKxThreads = 512;
+// kfield_mixed: Waves (4 or 8) waves share the term table. With 8, two workgroups per
+// CU (four waves per SIMD, which its launch bound asks for) fill the LDS exactly (term
+// table 13,824 B + 8 wave areas of 8,512 B = 81,920 B); with 4, three fit (47,872 B).
 
 // pair constants of each distinct yv: cst64[u][768] and an FP32 copy; one wave per u
 __global__ __launch_bounds__(64) void kfield_consts_win(const double* __restrict__ table64,
@@ -101,22 +100,23 @@ __global__ __launch_bounds__(kKmThreads) void kfield_axes(const float* __restric
 // The field of each distinct yv in FP32 from the axis rows: per (yv, site) only
 // (x, y), (y, x) and (x, d) are evaluated; (x, 0) and (0, x) are this site's rows
 // and (0, d) is the (0, x) row of the site d = x - y.
-__global__ __launch_bounds__(kKxThreads, 4) void kfield_mixed(
+template <int Waves>
+__global__ __launch_bounds__(64 * Waves, Waves == 8 ? 4 : 3) void kfield_mixed(
     const float* __restrict__ table32, const hlbl_coord* __restrict__ yvs, const int* __restrict__ slots,
     int n_unique, hlbl_geom global, double xunit, const float* __restrict__ cst32,
     const float* __restrict__ chbuf, const float* __restrict__ ax, float* __restrict__ coef) {
-  static_assert(sizeof(TermLds<float>) + kKxThreads / 64 * kKwLen3 * sizeof(float) <= 163840 / 2,
-                "two workgroups per CU");
+  static_assert(sizeof(TermLds<float>) + Waves * kKwLen3 * sizeof(float) <= 163840 / (Waves == 8 ? 2 : 3),
+                "two (8 waves) or three (4 waves) workgroups per CU");
   __shared__ TermLds<float> tt;
-  __shared__ float w[kKxThreads / 64][kKwLen3];
+  __shared__ float w[Waves][kKwLen3];
   stage_terms(c_terms32, &tt);
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: site math on the scalar unit
   const size_t V = geom_volume(global);
   const size_t total = (size_t)n_unique * V;
-  const size_t nw = (size_t)gridDim.x * (kKxThreads / 64);
+  const size_t nw = (size_t)gridDim.x * Waves;
   const hlbl_coord zero = {0, 0, 0, 0};
-  for (size_t i = (size_t)blockIdx.x * (kKxThreads / 64) + wave; i < total; i += nw) {
+  for (size_t i = (size_t)blockIdx.x * Waves + wave; i < total; i += nw) {
     const int u = (int)(i / V);
     const size_t site = i - (size_t)u * V;
     const hlbl_coord yv = yvs[u];
@@ -663,7 +663,7 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
   // pass 1
   if (stage == 0) {
     launch_pass1_planes<float>(w.d_pairs.p, WinPair{}, np, g, g, c->proc, 3, w.part1.p, w.red1.p,
-                               w.xi.p, st, w.d_units.p, w.unit_slots);
+                               w.xi.p, st, w.d_units.p, w.unit_slots, w.groups);
     launch_forms_batched(w.xi.p, w.dz.p, w.gdz.p, np, st);
     hipLaunchKernelGGL(bsplit_exp, dim3(1, np), dim3(128), 0, st, w.gdz.p, w.bexp.p);
     hipLaunchKernelGGL(form_bsplit, dim3((unsigned)div_up(kBsplitLen, 256), np), dim3(256), 0, st, w.gdz.p,
@@ -675,14 +675,21 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
     hipLaunchKernelGGL(kfield_consts_win, dim3(nu), dim3(64), 0, st, c->qed->table64.p, w.d_yvs.p, g,
                        w.xunit, w.cst64.p, w.cst32.p);
     check_hip(hipGetLastError(), "kfield_consts_win launch");
-    const int nwa = (int)std::min<size_t>(div_up(V, kKmThreads / 64), (size_t)4 * c->n_cu);
+    const hlbl_tuning& tu = c->tune;
+    const int nwa = (int)std::min<size_t>(div_up(V, kKmThreads / 64), (size_t)tu.axes_wg_per_cu * c->n_cu);
     hipLaunchKernelGGL(kfield_axes, dim3(nwa), dim3(kKmThreads), 0, st, c->qed->table32.p, g, w.xunit,
                        w.form == 2 ? w.chbuf.p : nullptr, w.ax.p);
     check_hip(hipGetLastError(), "kfield_axes launch");
-    const int nwg = (int)std::min<size_t>(div_up((size_t)nu * V, kKxThreads / 64), (size_t)2 * c->n_cu);
-    hipLaunchKernelGGL(kfield_mixed, dim3(nwg), dim3(kKxThreads), 0, st, c->qed->table32.p,
-                       w.d_yvs.p, w.d_slots.p, nu, g, w.xunit, w.cst32.p, w.form == 2 ? w.chbuf.p : nullptr,
-                       w.ax.p, w.coef);
+    const int fw = tu.field_waves;
+    const int nwg = (int)std::min<size_t>(div_up((size_t)nu * V, (size_t)fw), (size_t)tu.field_wg_per_cu * c->n_cu);
+    if (fw == 8)
+      hipLaunchKernelGGL(kfield_mixed<8>, dim3(nwg), dim3(64 * 8), 0, st, c->qed->table32.p,
+                         w.d_yvs.p, w.d_slots.p, nu, g, w.xunit, w.cst32.p, w.form == 2 ? w.chbuf.p : nullptr,
+                         w.ax.p, w.coef);
+    else
+      hipLaunchKernelGGL(kfield_mixed<4>, dim3(nwg), dim3(64 * 4), 0, st, c->qed->table32.p,
+                         w.d_yvs.p, w.d_slots.p, nu, g, w.xunit, w.cst32.p, w.form == 2 ? w.chbuf.p : nullptr,
+                         w.ax.p, w.coef);
     check_hip(hipGetLastError(), "kfield_mixed launch");
     // candidate boxes: ceil(kNearRadius / xunit) lattice units, at most the whole extent
     hlbl_coord box;
@@ -691,7 +698,8 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
     box.x = std::min(hb, (g.LX - 1) / 2);
     box.y = std::min(hb, (g.LY - 1) / 2);
     box.z = std::min(hb, (g.LZ - 1) / 2);
-    hipLaunchKernelGGL(kfield_near, dim3((unsigned)c->n_cu), dim3(kKmThreads), 0, st, c->qed->table64.p,
+    hipLaunchKernelGGL(kfield_near, dim3((unsigned)(tu.near_wg_per_cu * c->n_cu)), dim3(kKmThreads), 0, st,
+                       c->qed->table64.p,
                        w.d_yvs.p, w.d_slots.p, nu, g, w.xunit, w.cst64.p, box, w.coef);
     check_hip(hipGetLastError(), "kfield_near launch");
     if (w.form == 2) {
@@ -703,7 +711,7 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
   }
   // pass 2
   if (stage == 2) {
-    const int nwg = c->n_cu;
+    const int nwg = c->tune.pass2_wg_per_cu * c->n_cu;
     const size_t spw = div_up(div_up(V, (size_t)nwg), kS2Sites) * kS2Sites;
     const int n = (int)div_up(V, spw);
     hipLaunchKernelGGL(pass2_window, dim3(n), dim3(kS2Threads), 0, st, w.d_pairs.p, np, w.bsplit.p, w.bexp.p,
@@ -716,7 +724,7 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
 }
 
 size_t window_part2_len(const hlbl_contractor* c, int n_pairs) {
-  return (size_t)n_pairs * 2 * c->n_cu * kS2Mfma;
+  return (size_t)n_pairs * 2 * c->tune.pass2_wg_per_cu * c->n_cu * kS2Mfma;
 }
 
 size_t window_bsplit_len(int n_pairs) { return (size_t)n_pairs * 2 * kBsplitLen; }

@@ -37,8 +37,8 @@ constexpr int kJudgeSamples = 64;
 
 }  // namespace
 
-void pass1_group_units(const WinPair* pairs, int n_pairs, int* units, int unit_slots) {
-  const int nu = 2 * n_pairs, G = kPass1Groups;
+void pass1_group_units(const WinPair* pairs, int n_pairs, int* units, int unit_slots, int groups) {
+  const int nu = 2 * n_pairs, G = groups;
   // the two (block, flavour) halves a unit reads: S[f] of x and Y[1-f] of y
   std::map<std::pair<const double*, int>, int> ids;
   auto hid = [&](const double* b, int f) {
@@ -177,6 +177,12 @@ void window_drop_fields(hlbl_contractor* c) {
 
 int window_fields_built(const hlbl_contractor* c) { return c->window ? c->window->last_built : 0; }
 
+// launch shapes changed: every window graph is captured again
+void window_drop_graphs(hlbl_contractor* c) {
+  if (c->window)
+    for (auto& pl : c->window->plans) pl->drop_graph();
+}
+
 void window_fallback_result(const hlbl_contractor* c, int* fell_back) {
   if (!c->window || c->window->last_judge < 0) throw Error("hlbl_window_fallback: no surrogate window has run");
   *fell_back = c->window->last_fallback;
@@ -261,7 +267,7 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
   // and a run of differently sized batches does not reallocate.
   if (cache->plans.empty()) cache->plans.push_back(std::make_unique<WindowPlan>());
   WindowPlan* w = cache->plans.front().get();
-  if (!(w->n_pairs == n && w->form == form && w->xunit == xunit && w->coef == fs.coef.p)) {
+  if (!(w->n_pairs == n && w->form == form && w->xunit == xunit && w->coef == fs.coef.p && w->exec)) {
     w->drop_graph();
     w->n_pairs = n;
     w->form = form;
@@ -270,9 +276,10 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
     w->coef = fs.coef.p;
     w->d_pairs.ensure(n);
     w->h_pairs.ensure(n);
-    w->unit_slots = (2 * n + kPass1Groups - 1) / kPass1Groups;
-    w->d_units.ensure((size_t)kPass1Groups * w->unit_slots);
-    w->h_units.ensure((size_t)kPass1Groups * w->unit_slots);
+    w->groups = c->tune.pass1_groups;
+    w->unit_slots = (2 * n + w->groups - 1) / w->groups;
+    w->d_units.ensure((size_t)w->groups * w->unit_slots);
+    w->h_units.ensure((size_t)w->groups * w->unit_slots);
     w->part1.ensure(pass1_part_len(c->global, n));
     w->red1.ensure(pass1_red_len(c->global, n));
     w->xi.ensure((size_t)n * 2 * 8 * 5 * 144 * 2);
@@ -317,7 +324,7 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
                            mod_pos(pairs[i].gsx.y, g.LY), mod_pos(pairs[i].gsx.z, g.LZ)};
     w->h_pairs.p[i] = WinPair{pairs[i].d_x, pairs[i].d_y, gs, pairs[i].yv, call_slot[slot[i]], 0};
   }
-  pass1_group_units(w->h_pairs.p, n, w->h_units.p, w->unit_slots);
+  pass1_group_units(w->h_pairs.p, n, w->h_units.p, w->unit_slots, w->groups);
   if (!cache->stream) check_hip(hipStreamCreateWithFlags(&cache->stream, hipStreamNonBlocking), "hipStreamCreate");
   return w;
 }
@@ -347,7 +354,7 @@ void window_tail_copies(WindowPlan* w, hipStream_t st) {
 }
 void window_head_copies(WindowPlan* w, hipStream_t st) {
   check_hip(hipMemcpyAsync(w->d_pairs.p, w->h_pairs.p, w->n_pairs * sizeof(WinPair), hipMemcpyHostToDevice, st), "copy pairs");
-  check_hip(hipMemcpyAsync(w->d_units.p, w->h_units.p, (size_t)kPass1Groups * w->unit_slots * sizeof(int),
+  check_hip(hipMemcpyAsync(w->d_units.p, w->h_units.p, (size_t)w->groups * w->unit_slots * sizeof(int),
                            hipMemcpyHostToDevice, st), "copy units");
 }
 // the fields the store lacks: their list, the surrogate's channels (form 2), the fields

@@ -134,6 +134,35 @@ int hlbl_contractor_create(hlbl_contractor** out, hlbl_geom global, hlbl_geom lo
                            hlbl_coord proc_coords, hlbl_qed* qed);
 int hlbl_contractor_free(hlbl_contractor* c);
 
+/* Launch shapes of the contractor's kernels (no counterpart in the reference, whose
+ * launches are fixed at 32 threads per block, kernels/cuda_lattice.h:39-44). The
+ * defaults are those below; hlbl_amd/tune.py measures candidates on the device and
+ * records the fastest per lattice (hlbl_amd/data/tuning_gfx950.json), which the Python
+ * layer applies to every contractor it creates. Results do not depend on the shapes
+ * beyond the order of the FP64 sums across workgroups (pass 2, plain and window). */
+typedef struct {
+  int pass1_groups;          /* window pass 1: unit lists dealt to the workgroup slots in
+                                turn (units of a list share blocks in one XCD's L2);
+                                1..64, default 8 */
+  int field_waves;           /* window coefficient field: waves per workgroup (one site
+                                each), 4 or 8, default 8 */
+  int field_wg_per_cu;       /* its workgroups per CU: 1..2 (8 waves) or 1..3 (4 waves),
+                                default 2 */
+  int axes_wg_per_cu;        /* window axis rows: workgroups per CU, 1..16, default 4 */
+  int near_wg_per_cu;        /* window near-site FP64 field: workgroups per CU, 1..8,
+                                default 1 */
+  int pass2_wg_per_cu;       /* window pass 2: site ranges per CU (one resident), 1..4,
+                                default 1 */
+  int fp64_field_wg_per_cu;  /* plain path coefficient field: workgroups per CU, 1..16,
+                                default 2 */
+  int fp64_pass2_wg_per_cu;  /* plain path pass 2: workgroups per CU, 1..8, default 2 */
+} hlbl_tuning;
+/* out = the defaults */
+int hlbl_tuning_defaults(hlbl_tuning* out);
+/* validates and sets; later window calls capture their graphs again */
+int hlbl_contractor_set_tuning(hlbl_contractor* c, const hlbl_tuning* t);
+int hlbl_contractor_get_tuning(const hlbl_contractor* c, hlbl_tuning* out);
+
 /* Seam 1, Contractor::kernel_sum (src/contract.cpp:104-114): the connected kernel
  * sum of both flavours of the pair x = gsx, y = gsx + yv, plain FP64. Blocks are
  * borrowed device pointers. Synchronous on `stream`; out[2] is host memory. The sum

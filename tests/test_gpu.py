@@ -408,6 +408,37 @@ def test_window_field_store(qed):
     con.close()
 
 
+def test_tuning_shapes(qed):
+    """Launch shapes (hlbl_tuning) change only the order of the FP64 sums across
+    workgroups: window and plain pair sums under other shapes agree with the defaults'
+    to 1e-12; out-of-range shapes are refused."""
+    dims = (8, 8, 8, 8)
+    lat = H.Lattice(dims)
+    con = H.Contractor(lat, qed, tuning="default")
+    blocks = [dev_block(lat, s) for s in (41, 42, 43)]
+    pairs = [(blocks[0].data_ptr(), blocks[1].data_ptr(), (0, 0, 0, 0), (1, 1, 1, 1)),
+             (blocks[2].data_ptr(), blocks[1].data_ptr(), (2, 2, 2, 2), (-1, -1, -1, -1)),
+             (blocks[2].data_ptr(), blocks[0].data_ptr(), (2, 2, 2, 2), (2, -2, 2, -2))]
+    xunit = 0.3
+    base = np.array(con.window_kernel_sum(pairs, xunit, form=1))
+    base64 = np.array(con.kernel_sum(*pairs[0], xunit))
+    for t in ({"pass1_groups": 1, "pass2_wg_per_cu": 3}, {"pass1_groups": 32, "field_waves": 4, "field_wg_per_cu": 3},
+              {"axes_wg_per_cu": 1, "near_wg_per_cu": 8, "fp64_field_wg_per_cu": 7, "fp64_pass2_wg_per_cu": 1}):
+        con.set_tuning(**t)
+        got = np.array(con.window_kernel_sum(pairs, xunit, form=1))
+        assert con.fields_built() == 0
+        con.drop_fields()
+        cold = np.array(con.window_kernel_sum(pairs, xunit, form=1))
+        got64 = np.array(con.kernel_sum(*pairs[0], xunit))
+        assert np.all(np.abs(got - base) <= 1e-12 * np.abs(base)), t
+        assert np.all(np.abs(cold - base) <= 1e-12 * np.abs(base)), t
+        assert np.all(np.abs(got64 - base64) <= 1e-12 * np.abs(base64)), t
+    for bad in ({"field_waves": 6}, {"field_waves": 8, "field_wg_per_cu": 3}, {"pass1_groups": 0}):
+        with pytest.raises(H.HlblError):
+            con.set_tuning(**bad)
+    con.close()
+
+
 @pytest.mark.parametrize("dims", [(6, 6, 6, 6), (6, 6, 6, 12)])
 def test_window_form1_short_lines(qed, dims):
     """Form 1 where a (t, x, y) line is not a whole number of the plane sums' 8-site
