@@ -100,6 +100,53 @@ __device__ inline void dxu_vals(const R* __restrict__ sa, const R* __restrict__ 
   }
 }
 
+// dxu_vals with the spin sums paired: the phases of gamma_3 differ from those of gamma_0
+// by +-i spin by spin, as those of gamma_1 from gamma_2, so with the signed sums
+//   A+ = Xi0(1) - Xi0(2), A- = Xi0(0) - Xi0(3), B+ = -Xi1(1) - Xi1(3), B- = Xi1(0) + Xi1(2)
+// (each six colour-contracted products, the signs folded into the multiply-adds)
+//   dxu_0 = A+ + A-, dxu_3 = i (A+ - A-), dxu_2 = B+ + B-, dxu_1 = i (B+ - B-):
+// 96 multiply-adds and 8 adds, where dxu_vals spends 32 adds on the phases.
+template <class R, class A>
+HLBL_HD void dxu_vals_paired(const R* __restrict__ sa, const R* __restrict__ ur, A (&re)[4],
+                             A (&im)[4]) {
+  A ap_r = 0, ap_i = 0, am_r = 0, am_i = 0, bp_r = 0, bp_i = 0, bm_r = 0, bm_i = 0;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int s0 = s ^ 2, s1 = 3 - s;
+    // class 0 (Xi0, S spin s ^ 2): s = 1 -> A+ (+), 2 -> A+ (-), 0 -> A- (+), 3 -> A- (-)
+    // class 1 (Xi1, S spin 3 - s): s = 0, 2 -> B- (+), s = 1, 3 -> B+ (-)
+    const A g0 = (s == 2 || s == 3) ? A(-1) : A(1);
+    const A g1 = (s == 1 || s == 3) ? A(-1) : A(1);
+    A& c0r = (s == 1 || s == 2) ? ap_r : am_r;
+    A& c0i = (s == 1 || s == 2) ? ap_i : am_i;
+    A& c1r = (s == 1 || s == 3) ? bp_r : bm_r;
+    A& c1i = (s == 1 || s == 3) ? bp_i : bm_i;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+      const A vr = ur[2 * (3 * s + cc)], vi = ur[2 * (3 * s + cc) + 1];
+      const A t0r = g0 * A(sa[2 * (3 * s0 + cc)]), t0i = g0 * A(sa[2 * (3 * s0 + cc) + 1]);
+      const A t1r = g1 * A(sa[2 * (3 * s1 + cc)]), t1i = g1 * A(sa[2 * (3 * s1 + cc) + 1]);
+      // +- conj(t) v as four fused multiply-adds each (the signs are constants)
+      c0r = fma(t0r, vr, c0r);
+      c0r = fma(t0i, vi, c0r);
+      c0i = fma(t0r, vi, c0i);
+      c0i = fma(-t0i, vr, c0i);
+      c1r = fma(t1r, vr, c1r);
+      c1r = fma(t1i, vi, c1r);
+      c1i = fma(t1r, vi, c1i);
+      c1i = fma(-t1i, vr, c1i);
+    }
+  }
+  re[0] = ap_r + am_r;
+  im[0] = ap_i + am_i;
+  re[3] = am_i - ap_i;
+  im[3] = ap_r - am_r;
+  re[2] = bp_r + bm_r;
+  im[2] = bp_i + bm_i;
+  re[1] = bm_i - bp_i;
+  im[1] = bp_r - bm_r;
+}
+
 // Element (kidx = r*24 + j, col < 40) of flavour f of the pass-2 B operand, from
 // gdz[f][10][4][12][12] (complex): j = 2i: X_re[i][r], j = 2i+1: -X_im[i][r],
 // X = G[k][nu] (col = k*4+nu < 24, gdz rows 0..5: dzu) or H[s][nu]
@@ -123,6 +170,22 @@ __device__ inline void f16_split2(float x, float y, unsigned& hi, unsigned& lo) 
   __builtin_memcpy(&hi, &h, 4);
   __builtin_memcpy(&lo, &l, 4);
 }
+
+// (x, y) * sc (sc a power of two) = hi + lo in FP16, each packed as (x bits, y bits):
+// hi by one conversion of the scaled pair; lo = x sc - hi by a multiply-add that reads hi
+// as FP16 (a mixed-precision multiply-add), then one conversion. The same values as
+// f16_split2(x sc, y sc).
+__device__ inline void f16_split2_scaled(float x, float y, float sc, unsigned& hi, unsigned& lo) {
+  const f2v v = {x * sc, y * sc};
+  const h2v h = __builtin_convertvector(v, h2v);
+  const f2v r = {__builtin_fmaf(x, sc, -(float)h[0]), __builtin_fmaf(y, sc, -(float)h[1])};
+  const h2v l = __builtin_convertvector(r, h2v);
+  __builtin_memcpy(&hi, &h, 4);
+  __builtin_memcpy(&lo, &l, 4);
+}
+
+// 2^e as a float, e in [-126, 127]
+__device__ inline float pow2f(int e) { return __int_as_float((e + 127) << 23); }
 
 // Kahan-compensated accumulation
 struct Kahan {

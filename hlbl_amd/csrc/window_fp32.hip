@@ -469,22 +469,33 @@ __device__ inline int site_exp(unsigned ms, unsigned my) {
 // dxu_mu[r][a] of the four mu, scaled by 2^se of the site and split into FP16
 // (re, im) pairs into ah / al [(zi, mu)][kS2ARow] at k = r*24 + 2a; the site
 // exponents go to sse[zi] (from task (zi, 0, 0)).
+// The task's (zi, r, a) by exact 24-bit reciprocal multiplies (t < kS2Tasks = 1152:
+// t * 911 >> 17 = t / 144 and q * 171 >> 11 = q / 12 for q < 144), full-rate
+// multiplies in place of the quarter-rate 32-bit ones a division by a constant takes.
+static_assert(kS2Tasks <= 1152, "the reciprocal multiplies are exact below 1152 tasks");
 template <int N>
 __device__ inline void s2_dxu(int t0, int lane, const float* buf, const unsigned* smax, unsigned* ah,
                               unsigned* al, int* sse) {
 #pragma unroll 2
   for (int i = 0; i < N; ++i) {
-    const int t = t0 + 64 * i + lane;
-    const int zi = t / 144, r = (t / 12) % 12, a = t % 12;
+    const unsigned t = (unsigned)(t0 + 64 * i + lane);
+    const unsigned zi = __umul24(t, 911u) >> 17;
+    const unsigned q = t - __umul24(zi, 144u);
+    const unsigned r = __umul24(q, 171u) >> 11;
+    const unsigned a = q - __umul24(r, 12u);
     const int se = site_exp(smax[zi], smax[kS2Sites + zi]);
-    if (r == 0 && a == 0) sse[zi] = se;
+    if (q == 0) sse[zi] = se;
+    const float sc = pow2f(se);
     float re[4], im[4];
-    dxu_vals<float, float>(buf + a * kS2RowPad + zi * 24, buf + (12 + r) * kS2RowPad + zi * 24, re, im);
+    const unsigned zo = __umul24(zi, 24u);
+    dxu_vals_paired<float, float>(buf + __umul24(a, (unsigned)kS2RowPad) + zo,
+                                  buf + __umul24(12u + r, (unsigned)kS2RowPad) + zo, re, im);
+    const unsigned o0 = (__umul24(zi, 4u * kS2ARow) + __umul24(r, 24u) + 2u * a) >> 1;
 #pragma unroll
     for (int mu = 0; mu < 4; ++mu) {
       unsigned hw, lw;
-      f16_split2(ldexpf(re[mu], se), ldexpf(im[mu], se), hw, lw);
-      const int o = ((zi * 4 + mu) * kS2ARow + r * 24 + 2 * a) / 2;
+      f16_split2_scaled(re[mu], im[mu], sc, hw, lw);
+      const unsigned o = o0 + mu * (kS2ARow / 2);
       ah[o] = hw;
       al[o] = lw;
     }
