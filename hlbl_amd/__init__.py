@@ -113,6 +113,7 @@ _SIGS = {
                                  C.c_void_p, C.c_void_p]),
     "hlbl_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "hlbl_mlp_set_domain": (C.c_int, [C.c_void_p, C.c_double]),
+    "hlbl_mlp_set_inner_radius": (C.c_int, [C.c_void_p, C.c_double]),
     "hlbl_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "hlbl_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                    C.c_void_p]),
@@ -322,6 +323,11 @@ class QedKernel:
         """Train (and use) the surrogate on |x|, |y| <= radius (muon units); outside,
         the table's channels are used. Discards the trained state."""
         check(lib().hlbl_mlp_set_domain(self.handle, float(radius)))
+
+    def mlp_set_inner_radius(self, inner):
+        """Train (and use) the surrogate on |x|, |y| >= inner (default 0.125 muon units);
+        below, the table's channels are used. Discards the trained state."""
+        check(lib().hlbl_mlp_set_inner_radius(self.handle, float(inner)))
 
     def mlp_get_params(self):
         n = self.mlp_param_count()

@@ -674,7 +674,20 @@ int hlbl_mlp_set_domain(hlbl_qed* q, double radius) {
     check_ptr(q, "qed");
     if (!(radius > 0.0) || radius > qed::kRadius * (1.0 + 1e-12))
       throw Error("hlbl_mlp_set_domain: need 0 < radius <= the kernel radius");
+    if (!(radius > q->mlp_inner))
+      throw Error("hlbl_mlp_set_domain: the radius must exceed the inner radius");
     q->mlp_radius = radius;
+    q->mlp_ready = false;
+    ++q->mlp_gen;
+  });
+}
+
+int hlbl_mlp_set_inner_radius(hlbl_qed* q, double inner) {
+  return guarded([&] {
+    check_ptr(q, "qed");
+    if (!(inner >= 0.0) || !(inner < q->mlp_radius))
+      throw Error("hlbl_mlp_set_inner_radius: need 0 <= inner < the trained radius");
+    q->mlp_inner = inner;
     q->mlp_ready = false;
     ++q->mlp_gen;
   });

@@ -105,6 +105,9 @@ struct hlbl_qed {
   bool mlp_ready = false;
   uint64_t mlp_gen = 0;  // bumped whenever the network or its domain changes
   double mlp_radius = hlbl::qed::kRadius;  // trained domain: |x|, |y| <= mlp_radius (muon units)
+  // and |x|, |y| >= mlp_inner: below it (the rows L0(x, 0), L0(0, x) among them, where the
+  // weight functions bend like |y| log |y|) the table serves
+  double mlp_inner = 0.125;
   double* noalloc_scratch = nullptr;  // this device's fixed scratch array (qed_noalloc.hip)
 };
 

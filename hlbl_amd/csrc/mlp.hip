@@ -263,18 +263,19 @@ __host__ __device__ inline double mlp_uniform(uint64_t seed, uint64_t i, int k) 
   return (double)(hash64(seed * 0x9E3779B97F4A7C15ULL + i * 3 + k) >> 11) * (1.0 / 9007199254740992.0);
 }
 
-// |x|, |y| uniform in [0, radius], cos beta uniform in [-1, 1]; beta = acos(cos beta)
-void mlp_sample_inputs(uint64_t seed, size_t i, double radius, double& a, double& be, double& b) {
-  a = radius * mlp_uniform(seed, i, 0);
-  b = radius * mlp_uniform(seed, i, 1);
+// |x|, |y| uniform in [inner, radius], cos beta uniform in [-1, 1]; beta = acos(cos beta)
+void mlp_sample_inputs(uint64_t seed, size_t i, double inner, double radius, double& a, double& be,
+                       double& b) {
+  a = inner + (radius - inner) * mlp_uniform(seed, i, 0);
+  b = inner + (radius - inner) * mlp_uniform(seed, i, 1);
   be = std::acos(2.0 * mlp_uniform(seed, i, 2) - 1.0);
 }
 
 // Training set of n samples (host threads): the channels of the plain evaluation
 // (the table, interpolated as the plain kernel does) at |x|, |y| uniform in
-// [0, radius], cos beta uniform in [-1, 1]; x [n][3] normalised inputs (|x|/radius,
-// |y|/radius, beta/pi), t [n][42] channels.
-void mlp_build_set(uint64_t seed, int n, double radius, const std::vector<double>& table,
+// [inner, radius], cos beta uniform in [-1, 1]; x [n][3] normalised inputs
+// ((|x| - inner) / (radius - inner), the same for |y|, beta/pi), t [n][42] channels.
+void mlp_build_set(uint64_t seed, int n, double inner, double radius, const std::vector<double>& table,
                    std::vector<float>& x, std::vector<double>& t) {
   x.resize((size_t)n * 3);
   t.resize((size_t)n * kMlpOut);
@@ -287,11 +288,11 @@ void mlp_build_set(uint64_t seed, int n, double radius, const std::vector<double
       double ch[qed::kNCh];
       for (int i = th; i < n; i += nt) {
         double a, be, b;
-        mlp_sample_inputs(seed, (size_t)i, radius, a, be, b);
+        mlp_sample_inputs(seed, (size_t)i, inner, radius, a, be, b);
         const qed::Cell<double> cell = qed::locate(a, be, b);
         for (int k = 0; k < kMlpOut; ++k) ch[k] = qed::interp(table.data(), cell, k);
-        x[(size_t)i * 3 + 0] = (float)(a / radius);
-        x[(size_t)i * 3 + 1] = (float)(b / radius);
+        x[(size_t)i * 3 + 0] = (float)((a - inner) / (radius - inner));
+        x[(size_t)i * 3 + 1] = (float)((b - inner) / (radius - inner));
         x[(size_t)i * 3 + 2] = (float)(be / M_PI);
         for (int k = 0; k < kMlpOut; ++k) t[(size_t)i * kMlpOut + k] = ch[k];
       }
@@ -433,11 +434,12 @@ __global__ __launch_bounds__(kInfThreads, 2) void mlp_infer_rows(const unsigned 
 // Surrogate channels of a window (gsx = 0), chbuf[block][site][kNCh] =
 // mean + sd * net, for the blocks 0: (x, 0) and 1: (0, x), shared
 // by every yv, and 2 + 3u + j: (x, y), (y, x), (x, x - y) of the distinct yv u.
-// Arguments outside the trained domain (|x| or |y| above radius) take the
-// channels from the table instead.
+// Arguments outside the trained domain (|x| or |y| above radius or below inner) take
+// the channels from the table instead: with an inner radius, the rows (x, 0) and (0, x)
+// always do.
 __global__ __launch_bounds__(kInfThreads, 2) void mlp_surrogate_channels(
     const unsigned short* __restrict__ w16, const float* __restrict__ scale,
-    const float* __restrict__ table32, float radius, const hlbl_coord* __restrict__ yvs,
+    const float* __restrict__ table32, float inner, float radius, const hlbl_coord* __restrict__ yvs,
     int n_unique, hlbl_geom global, double xunit, float* __restrict__ chbuf) {
   __shared__ __attribute__((aligned(16))) unsigned short act[kInfThreads / 64][kInfWaveRows * kActStride];
   __shared__ float s_in[kInfThreads / 64][kInfWaveRows];  // 1: inside the trained domain
@@ -446,7 +448,7 @@ __global__ __launch_bounds__(kInfThreads, 2) void mlp_surrogate_channels(
   const size_t V = geom_volume(global);
   const size_t rows = (size_t)(2 + 3 * n_unique) * V;
   const hlbl_coord zero = {0, 0, 0, 0};
-  const float inv_r = 1.0f / radius;
+  const float inv_r = 1.0f / (radius - inner);
   {
     const size_t r0 = (size_t)blockIdx.x * kInfRows + (size_t)wave * kInfWaveRows;
     if (r0 >= rows) return;
@@ -462,7 +464,8 @@ __global__ __launch_bounds__(kInfThreads, 2) void mlp_surrogate_channels(
       float pu[4], pv[4];
       eval_args(e, g, pu, pv);
       qed::grid_coords(pu, pv, a, be, b);
-      in = (a > radius || b > radius) ? 0.f : 1.f;  // outside the trained domain: table
+      // outside the trained domain: table
+      in = (a > radius || b > radius || a < inner || b < inner) ? 0.f : 1.f;
     }
     s_in[wave][lane] = in;
     s_acb[wave][lane][0] = a;
@@ -470,7 +473,8 @@ __global__ __launch_bounds__(kInfThreads, 2) void mlp_surrogate_channels(
     s_acb[wave][lane][2] = b;
     const float* inw = s_in[wave];
     const float(*acbw)[3] = s_acb[wave];
-    mlp_wave64(w16, fminf(a * inv_r, 1.f), fminf(b * inv_r, 1.f), be * (float)M_1_PI, act[wave],
+    mlp_wave64(w16, fminf(fmaxf((a - inner) * inv_r, 0.f), 1.f), fminf(fmaxf((b - inner) * inv_r, 0.f), 1.f),
+               be * (float)M_1_PI, act[wave],
                [=](int r, int col, float v) {
       if (r0 + r >= rows) return;
       float ch;
@@ -490,29 +494,31 @@ void launch_surrogate_channels(hlbl_contractor* c, WindowPlan& w, hipStream_t st
   const size_t rows = (size_t)(2 + 3 * w.n_unique) * c->volume;
   const unsigned nwg = (unsigned)div_up(rows, kInfRows);
   hipLaunchKernelGGL(mlp_surrogate_channels, dim3(nwg), dim3(kInfThreads), 0, st, q->mlp_w_bf16.p,
-                     q->mlp_scale.p, q->table32.p, (float)q->mlp_radius, w.d_yvs.p, w.n_unique,
+                     q->mlp_scale.p, q->table32.p, (float)q->mlp_inner, (float)q->mlp_radius, w.d_yvs.p,
+                     w.n_unique,
                      c->global, w.xunit, w.chbuf.p);
   check_hip(hipGetLastError(), "mlp_surrogate_channels launch");
 }
 
 // surrogate channels at points (a, beta, b): inputs, network and scale; points
 // outside the trained domain take the table's channels
-__global__ void k_surrogate_inputs(const double* __restrict__ acb, int n, double radius,
+__global__ void k_surrogate_inputs(const double* __restrict__ acb, int n, double inner, double radius,
                                    float* __restrict__ x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  x[3 * i + 0] = (float)fmin(acb[3 * i + 0] / radius, 1.0);
-  x[3 * i + 1] = (float)fmin(acb[3 * i + 2] / radius, 1.0);
+  const double inv = 1.0 / (radius - inner);
+  x[3 * i + 0] = (float)fmin(fmax((acb[3 * i + 0] - inner) * inv, 0.0), 1.0);
+  x[3 * i + 1] = (float)fmin(fmax((acb[3 * i + 2] - inner) * inv, 0.0), 1.0);
   x[3 * i + 2] = (float)(acb[3 * i + 1] * M_1_PI);
 }
 __global__ void k_surrogate_outputs(const double* __restrict__ acb, const float* __restrict__ y,
                                     const float* __restrict__ scale, const double* __restrict__ table,
-                                    double radius, int n, double* __restrict__ out) {
+                                    double inner, double radius, int n, double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * kMlpOut) return;
   const int r = i / kMlpOut, c = i % kMlpOut;
   const double a = acb[3 * r], cb = acb[3 * r + 1], b = acb[3 * r + 2];
-  if (a > radius || b > radius) {
+  if (a > radius || b > radius || a < inner || b < inner) {
     out[i] = qed::interp(table, qed::locate(a, cb, b), c);
     return;
   }
@@ -525,10 +531,10 @@ void launch_surrogate_points(hlbl_qed* q, const double* acb, int n, double* out,
   check_hip(hipMallocAsync((void**)&x, (size_t)n * 3 * sizeof(float), st), "hipMallocAsync");
   check_hip(hipMallocAsync((void**)&y, (size_t)n * kMlpOut * sizeof(float), st), "hipMallocAsync");
   hipLaunchKernelGGL(k_surrogate_inputs, dim3((unsigned)div_up(n, 256)), dim3(256), 0, st, acb, n,
-                     q->mlp_radius, x);
+                     q->mlp_inner, q->mlp_radius, x);
   mlp_infer(q, x, n, y, st);
   hipLaunchKernelGGL(k_surrogate_outputs, dim3((unsigned)div_up((size_t)n * kMlpOut, 256)), dim3(256), 0,
-                     st, acb, y, q->mlp_scale.p, q->table64.p, q->mlp_radius, n, out);
+                     st, acb, y, q->mlp_scale.p, q->table64.p, q->mlp_inner, q->mlp_radius, n, out);
   hipError_t e = hipGetLastError();
   (void)hipFreeAsync(x, st);
   (void)hipFreeAsync(y, st);
@@ -583,7 +589,7 @@ void mlp_train(hlbl_qed* q, int n_samples, int epochs, int batch, float lr, uint
   if (!q->mlp_w.p) mlp_init_params(q, seed ^ 0x5eedULL);
   std::vector<float> x;
   std::vector<double> t;
-  mlp_build_set(seed, n_samples, q->mlp_radius, q->host_table, x, t);
+  mlp_build_set(seed, n_samples, q->mlp_inner, q->mlp_radius, q->host_table, x, t);
   // per-channel standardisation of the targets
   std::vector<float> sc(2 * kMlpOut);
   std::vector<float> tn((size_t)n_samples * kMlpOut);

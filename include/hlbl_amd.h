@@ -115,6 +115,13 @@ int hlbl_mlp_train(hlbl_qed* q, int n_samples, int epochs, int batch, float lr,
  * the kernel radius, 12 fm; default the kernel radius). Arguments outside it take
  * the table's channels. Changing it discards the trained state (train again). */
 int hlbl_mlp_set_domain(hlbl_qed* q, double radius);
+/* inner radius of the trained domain: |x|, |y| >= inner (muon units, default 0.125;
+ * 0 <= inner < radius). Below it the table's channels are used: the site-only
+ * evaluations L0(x, 0), L0(0, x) always, where the weight functions bend like
+ * |y| log |y| and a network of this size fits them worst. Network inputs are
+ * (|x| - inner) / (radius - inner), the same for |y|, and beta / pi. Changing it
+ * discards the trained state (train again). */
+int hlbl_mlp_set_inner_radius(hlbl_qed* q, double inner);
 /* parameters (FP32 master copy, packed layer by layer: W then b) to/from host */
 int hlbl_mlp_get_params(const hlbl_qed* q, float* host, int n);
 int hlbl_mlp_set_params(hlbl_qed* q, const float* host, int n);
