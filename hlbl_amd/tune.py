@@ -1,7 +1,9 @@
 """Launch-shape tuner for a contractor's kernels on this GPU.
 
 For one lattice it builds window 1 of the chain K=8, stride n, h=(1,1,1,1) on random
-blocks (RandomProvider, generated on the device): the workload the window form runs.
+blocks (RandomProvider, generated on the device): the workload the window form runs;
+or, with --workload source, the call of window 1's source with the most partners (the
+per-source calls a lattice uses whose windows do not fit in memory, as 32^3x64).
 Then, one shape at a time (coordinate descent from the defaults), it times every
 candidate and keeps the fastest:
   - pass1_groups, pass2_wg_per_cu: the window's pass-1 and pass-2 stage times (HIP
@@ -50,11 +52,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def window1(lat, stride, seed, torch):
-    """window 1 of the chain K=8, stride n, h=(1,1,1,1): descriptors and the blocks"""
+def window1(lat, stride, seed, torch, workload="window"):
+    """window 1 of the chain K=8, stride n, h=(1,1,1,1), or its source with the most
+    partners: descriptors and the blocks"""
     dims = lat.global_
     sch = H.Schedule((1, 1, 1, 1), stride, 8)
-    pairs = [(q, p) for w, p, parts in sch.sources() if w == 1 for q in parts]
+    if workload == "window":
+        pairs = [(q, p) for w, p, parts in sch.sources() if w == 1 for q in parts]
+    else:
+        _, p, parts = max((s for s in sch.sources() if s[0] == 1), key=lambda s: len(s[2]))
+        pairs = [(q, p) for q in parts]
     wrap = lambda v: tuple(a % L for a, L in zip(v, dims))
     blocks = {}
     for pos in sorted({q for q, p in pairs} | {p for q, p in pairs}):
@@ -116,6 +123,7 @@ def main():
     ap.add_argument("--dims", type=int, nargs=4, default=[48, 24, 24, 24])
     ap.add_argument("--stride", type=int, default=3)
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--workload", choices=["window", "source"], default="window")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--margin", type=float, default=0.01)
     ap.add_argument("--out", default=H.TUNING_PATH)
@@ -126,8 +134,8 @@ def main():
     lat = H.Lattice(tuple(args.dims))
     qed = H.QedKernel(0)
     con = H.Contractor(lat, qed, tuning="default")
-    descs, blocks = window1(lat, args.stride, args.seed, torch)
-    log(f"window 1: {len(descs)} pairs over {len(blocks)} blocks of {tuple(args.dims)}")
+    descs, blocks = window1(lat, args.stride, args.seed, torch, args.workload)
+    log(f"{args.workload} of window 1: {len(descs)} pairs over {len(blocks)} blocks of {tuple(args.dims)}")
 
     defaults = con.tuning()
     con.drop_fields()
@@ -157,10 +165,12 @@ def main():
     table.setdefault("device", "gfx950 (MI355X)")
     table.setdefault("lattices", {})
     table["lattices"]["x".join(map(str, args.dims))] = {
-        "workload": f"window 1 of chain K=8 n={args.stride} h=(1,1,1,1), {len(descs)} pairs, seed {args.seed}, "
-                    f"form 1; plain path: its first pair",
+        "workload": (f"window 1 of chain K=8 n={args.stride} h=(1,1,1,1)" if args.workload == "window" else
+                     f"the source of window 1 (chain K=8 n={args.stride} h=(1,1,1,1)) with the most partners") +
+                    f", {len(descs)} pairs, seed {args.seed}, form 1; plain path: its first pair",
         "command": "python -m hlbl_amd.tune --dims " + " ".join(map(str, args.dims)) +
-                   f" --stride {args.stride} --reps {args.reps} --margin {args.margin}",
+                   f" --stride {args.stride} --seed {args.seed} --workload {args.workload} --reps {args.reps}"
+                   f" --margin {args.margin}",
         "picks": picks, "defaults": defaults, "timings_ms": timings,
         "stage_ms_default": ms_default, "stage_ms_tuned": ms_tuned,
         "fields_cold_ms_default": ms_default_cold[2], "fields_cold_ms_tuned": ms_tuned_cold[2],
