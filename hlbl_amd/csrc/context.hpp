@@ -140,6 +140,8 @@ struct hlbl_contractor {
   hlbl::DevBuf<double> bmat;    // pass-2 B operands [2][288][48]
   hlbl::DevBuf<double> coef;    // pass-2 coefficient field [V][16][40]
   hlbl::DevBuf<double> kcst;    // the pair constants L0(0,y), L0(y,0) [2][384]
+  hlbl::DevBuf<double> ax64;    // FP64 site-only rows L0(x,0), L0(0,x) at relative sites [2][V][384]
+  double ax64_xunit = -1.0;     //   for this xunit (none: negative)
   hlbl::DevBuf<double> part2;   // pass-2 partials [nwg2][12]
   hlbl::DevBuf<double> pi2p2;   // the 2+2 pieces' Pi_{mu nu}(z) [V][16]
   hlbl::DevBuf<double> part234; // P2..P4 workgroup partials [workgroups][320]

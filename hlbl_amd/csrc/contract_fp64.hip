@@ -168,6 +168,60 @@ __global__ __launch_bounds__(kKfThreads) void kfield_fp64(
   }
 }
 
+// The site-only rows of the plain field, FP64, at every relative site (gsx = 0):
+// ax[0][V][384] = L0(x, 0), ax[1][V][384] = L0(0, x); one wave per site. They serve
+// every pair of the contractor's lattice and xunit (kept on the contractor).
+__global__ __launch_bounds__(kKfThreads) void kfield_axes64(const double* __restrict__ table, hlbl_geom global,
+                                                           double xunit, double* __restrict__ ax) {
+  __shared__ TermLds<double> tt;
+  __shared__ double w[kKfThreads / 64][kKwLen];
+  stage_terms(c_terms64, &tt);
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const size_t V = geom_volume(global);
+  const size_t nw = (size_t)gridDim.x * (kKfThreads / 64);
+  const hlbl_coord zero = {0, 0, 0, 0};
+  for (size_t site = (size_t)blockIdx.x * (kKfThreads / 64) + wave; site < V; site += nw) {
+    SiteGeom<double> g;
+    site_geom(site, global, global, zero, zero, zero, xunit, g);
+    kfield_wave_axis_rows<double, double, double>(table, tt, g, ChannelRows::none(), w[wave],
+                                                  ax + site * qed::kNL, ax + (V + site) * qed::kNL);
+  }
+}
+
+// kfield_fp64 on one process (local = global) with the site-only rows read from ax:
+// per site only L0(x, y), L0(y, x) and L0(x, x - y) are evaluated; L0(x, 0) and L0(0, x)
+// are the rows of the relative site x - gsx, L0(0, x - y) the (0, x) row of the
+// relative site x - gsx - yv.
+__global__ __launch_bounds__(kKfThreads) void kfield_fp64_axes(
+    const double* __restrict__ table, hlbl_geom global, hlbl_coord gsx, hlbl_coord yv, double xunit,
+    const double* __restrict__ cst, const double* __restrict__ ax, double* __restrict__ coef) {
+  __shared__ TermLds<double> tt;
+  __shared__ double w[kKfThreads / 64][kKwLen64A];
+  stage_terms(c_terms64, &tt);
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const size_t V = geom_volume(global);
+  const size_t nw = (size_t)gridDim.x * (kKfThreads / 64);
+  const hlbl_coord zero = {0, 0, 0, 0};
+  for (size_t site = (size_t)blockIdx.x * (kKfThreads / 64) + wave; site < V; site += nw) {
+    SiteGeom<double> g;
+    site_geom(site, global, global, zero, gsx, yv, xunit, g);
+    int crd[4], rel[4], dsh[4];
+    site_coords(site, global, crd);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int L = geom_dim(global, r);
+      rel[r] = mod_pos(crd[r] - coord_dim(gsx, r), L);
+      dsh[r] = mod_pos(rel[r] - coord_dim(yv, r), L);
+    }
+    const size_t rs = (((size_t)rel[0] * global.LX + rel[1]) * global.LY + rel[2]) * global.LZ + rel[3];
+    const size_t sd = (((size_t)dsh[0] * global.LX + dsh[1]) * global.LY + dsh[2]) * global.LZ + dsh[3];
+    kfield_wave_axes64<double, double>(table, tt, g, cst, ax + rs * qed::kNL, ax + (V + rs) * qed::kNL,
+                                       ax + (V + sd) * qed::kNL, w[wave], coef + site * kCoefField);
+  }
+}
+
 // 8 waves: wave = (flavour, quarter of the K range); each covers the three
 // 16-column blocks, so one A fragment feeds three MFMAs
 constexpr int kP2Threads = 512;
@@ -341,9 +395,25 @@ void launch_kfield(hlbl_contractor* c, hlbl_coord proc, hlbl_coord gsx, hlbl_coo
                      c->kcst.p);
   check_hip(hipGetLastError(), "kfield_consts64 launch");
   const int nwg = (int)std::min<size_t>(div_up(V, kKfThreads / 64), (size_t)c->tune.fp64_field_wg_per_cu * c->n_cu);
-  hipLaunchKernelGGL(kfield_fp64, dim3(nwg), dim3(kKfThreads), 0, st, c->qed->table64.p, global,
-                     local, proc, gsx, yv, xunit, c->kcst.p, c->coef.p);
-  check_hip(hipGetLastError(), "kfield_fp64 launch");
+  const bool whole = global.T == local.T && global.LX == local.LX && global.LY == local.LY && global.LZ == local.LZ;
+  if (!whole) {
+    hipLaunchKernelGGL(kfield_fp64, dim3(nwg), dim3(kKfThreads), 0, st, c->qed->table64.p, global,
+                       local, proc, gsx, yv, xunit, c->kcst.p, c->coef.p);
+    check_hip(hipGetLastError(), "kfield_fp64 launch");
+    return;
+  }
+  // one process holds the lattice: the site-only rows once per xunit, then three
+  // evaluations per site
+  if (c->ax64_xunit != xunit) {
+    c->ax64.ensure((size_t)2 * V * qed::kNL);
+    hipLaunchKernelGGL(kfield_axes64, dim3(nwg), dim3(kKfThreads), 0, st, c->qed->table64.p, global, xunit,
+                       c->ax64.p);
+    check_hip(hipGetLastError(), "kfield_axes64 launch");
+    c->ax64_xunit = xunit;
+  }
+  hipLaunchKernelGGL(kfield_fp64_axes, dim3(nwg), dim3(kKfThreads), 0, st, c->qed->table64.p, global, gsx, yv,
+                     xunit, c->kcst.p, c->ax64.p, c->coef.p);
+  check_hip(hipGetLastError(), "kfield_fp64_axes launch");
 }
 
 void launch_pass2(hlbl_contractor* c, const double* S, const double* Y, const double* bmat,

@@ -497,6 +497,63 @@ __device__ __forceinline__ void kfield_wave_axes(const TT* __restrict__ table, c
   wave_sync();
 }
 
+// FP64 kfield_wave with the site-only evaluations read from FP64 axis rows (x0 =
+// L0(x, 0) and ox = L0(0, x) of this site, od = L0(0, d) of the site d = x - y), as
+// kfield_wave_axes does in FP32: the same steps as kfield_wave in the same order, with
+// only (x, y), (y, x) and (x, d) evaluated. w holds kKwLen64A doubles.
+constexpr int kKwLen64A = 3 * kChS3 + qed::kNT + 3 * qed::kNL + kCoefField + 48;
+template <class TT, class O>
+__device__ __forceinline__ void kfield_wave_axes64(const TT* __restrict__ table, const TermLds<double>& tt,
+                                                   const SiteGeom<double>& g, const double* cst,
+                                                   const double* __restrict__ x0, const double* __restrict__ ox,
+                                                   const double* __restrict__ od, double* w,
+                                                   O* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  double* ch = w;  // [3][kChS3]: (x, y), (y, x), (x, d)
+  double* T = ch + 3 * kChS3;
+  double* L = T + qed::kNT;
+  double* C = L + 3 * qed::kNL;
+  double* D = C + kCoefField;
+  double* s0 = L;
+  double* s1 = L + qed::kNL;
+  double* s2 = L + 2 * qed::kNL;
+  if (lane < 16) kp_dmat(lane, g, D);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int t = lane + 64 * k;
+    if (t < 3 * qed::kNChUsed) {
+      const int j = t / qed::kNChUsed, c = t - j * qed::kNChUsed;
+      ch[j * kChS3 + c] = kp_channel(2 * j, c, g, table);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < qed::kNL / 64; ++k) {
+    s0[64 * k + lane] = x0[64 * k + lane];
+    s1[64 * k + lane] = ox[64 * k + lane];
+  }
+  wave_sync();
+  const double* Dx = D;
+  const double* Dy = D + 16;
+  const double* Dd = D + 32;
+  wave_l0_from(0, tt, g, ch + 0 * kChS3, T, s2);                 // (x, y)
+  wave_kernel(0, s2, cst, s0, Dx, Dy, g.in12, C);               // K1 = L(x, y)
+  wave_l0_from(2, tt, g, ch + 1 * kChS3, T, s2);                 // (y, x)
+  wave_kernel(1, s2, s1, cst + qed::kNL, Dy, Dx, g.in12, C);    // K2 = L(y, x)
+#pragma unroll
+  for (int k = 0; k < qed::kNL / 64; ++k) s1[64 * k + lane] = od[64 * k + lane];
+  wave_sync();
+  wave_l0_from(4, tt, g, ch + 2 * kChS3, T, s2);                 // (x, d)
+  wave_kernel(2, s2, s1, s0, Dx, Dd, g.in3, C);                 // K3 = L(x, d)
+  for (int t = lane; t < 16 * 16; t += 64) {
+    const int row = t >> 4, hc = t & 15;
+    C[row * kCoefUsed + 24 + hc] = coef_h(row, hc, g, s2);
+  }
+  wave_sync();
+#pragma unroll
+  for (int k = 0; k < kCoefField / 64; ++k) out[64 * k + lane] = (O)C[64 * k + lane];
+  wave_sync();
+}
+
 // The pair constants [L0(0,y) | L0(y,0)] into cst (768 elements), by one wave
 // (w: kKwLen elements of LDS).
 template <class R, class TT>
