@@ -94,6 +94,7 @@ _SIGS = {
     "hlbl_contractor_set_tuning": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hlbl_contractor_get_tuning": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hlbl_window_fields_built": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "hlbl_window_reserve_fields": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "hlbl_qed_init_file": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_char_p]),
     "hlbl_qed_exact": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
     "hlbl_qed_free": (C.c_int, [C.c_void_p]),
@@ -437,6 +438,12 @@ class Contractor:
         n = C.c_int()
         check(lib().hlbl_window_fields_built(self.handle, C.byref(n)))
         return n.value
+
+    def reserve_fields(self, n=0):
+        """At least n slots for kept coefficient fields; returns the slots the store has."""
+        s = C.c_int()
+        check(lib().hlbl_window_reserve_fields(self.handle, int(n), C.byref(s)))
+        return s.value
 
     def fell_back(self):
         """Whether the last form-2 window returned the plain kernel's results."""

@@ -556,6 +556,8 @@ void window_judge_result(const hlbl_contractor* c, double* err, int* n);
 void window_set_judge_bound(hlbl_contractor* c, double bound);
 void window_fallback_result(const hlbl_contractor* c, int* fell_back);
 void window_drop_fields(hlbl_contractor* c);
+void window_reserve_fields(hlbl_contractor* c, int n);
+int window_field_slots(const hlbl_contractor* c);
 int window_fields_built(const hlbl_contractor* c);
 
 
@@ -643,6 +645,15 @@ int hlbl_window_fields_built(const hlbl_contractor* c, int* n) {
     check_ptr(c, "contractor");
     check_ptr(n, "n");
     *n = window_fields_built(c);
+  });
+}
+
+int hlbl_window_reserve_fields(hlbl_contractor* c, int n, int* slots) {
+  return guarded([&] {
+    check_ptr(c, "contractor");
+    if (n < 0) throw Error("hlbl_window_reserve_fields: n < 0");
+    if (n > 0) window_reserve_fields(c, n);
+    if (slots) *slots = window_field_slots(c);
   });
 }
 

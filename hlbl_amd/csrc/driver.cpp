@@ -225,6 +225,7 @@ class StagingPool {
 
 void window_entry(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit, int form,
                   double* out, hipStream_t st);
+void window_reserve_fields(hlbl_contractor* c, int n);
 
 namespace {
 
@@ -313,6 +314,30 @@ void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xuni
   if (host_tier && batch != 0) {
     const int n_slots = staging_slots(s, batch);
     if (n_slots > 0) pool = std::make_unique<StagingPool>(len, n_slots);
+  }
+  // The window forms keep one coefficient field per distinct yv across calls; a chain
+  // has few (16 at 24^3x48 n=3 K=8 and 32^3x64 n=4 K=4), so the store gets a slot for
+  // each when the device memory left beside the chain's blocks allows: otherwise
+  // per-source calls (up to 8 yv each) evict fields that later calls build again.
+  if (form != 0 && batch != 0) {
+    std::set<Vec4> yvs;
+    for (const Step& st : s)
+      for (const Source& src : st.sources)
+        for (const Vec4& q : src.partners) yvs.insert(min_image(src.position, q, c->global));
+    size_t free_b = 0, total_b = 0;
+    check_hip(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    const size_t block_b = len * sizeof(double);
+    // the cache peak of the call mode: a window's sources are solved before its call
+    // (serial), a source's call follows its solve (interleaved)
+    const size_t peak = (size_t)simulate(s, batch == 2).peak_cache;
+    // the host tier's staging pool is already allocated; its solve target is not yet
+    const size_t blocks_b = (host_tier ? 1 : peak) * block_b;
+    const size_t margin = total_b / 8;  // window buffers, axis rows, allocator slack
+    const size_t field_b = c->volume * 640 * sizeof(float);
+    const size_t avail = free_b > blocks_b + margin ? free_b - blocks_b - margin : 0;
+    const size_t fit = avail / field_b;
+    const int want = (int)std::min<size_t>(yvs.size(), fit);
+    if (want > 0) window_reserve_fields(c, want);
   }
   auto pair_done = [&](const Vec4& a, const Vec4& b) {
     cache.pair_done(a, b);

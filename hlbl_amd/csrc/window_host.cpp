@@ -175,6 +175,24 @@ void window_drop_fields(hlbl_contractor* c) {
   if (c->window) c->window->fields.drop();
 }
 
+// at least n slots in the field store (growing it drops the kept fields and every
+// captured graph, which hold its address)
+void window_reserve_fields(hlbl_contractor* c, int n) {
+  WindowCache* cache = window_cache(c);
+  FieldStore& fs = cache->fields;
+  if (fs.cap >= n) return;
+  fs.coef.release();
+  fs.coef.ensure((size_t)n * c->volume * 640);
+  fs.cap = n;
+  fs.key.assign(n, FieldStore::Key{});
+  fs.valid.assign(n, 0);
+  fs.used.assign(n, 0);
+  fs.judge.assign(n, 0.0);
+  for (auto& pl : cache->plans) pl->drop_graph();
+}
+
+int window_field_slots(const hlbl_contractor* c) { return c->window ? c->window->fields.cap : 0; }
+
 int window_fields_built(const hlbl_contractor* c) { return c->window ? c->window->last_built : 0; }
 
 // launch shapes changed: every window graph is captured again
@@ -218,16 +236,7 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
   // the field store: at least one slot per distinct yv of the call (growing it drops
   // the cached fields and every captured graph, which hold its address)
   FieldStore& fs = cache->fields;
-  if (fs.cap < nu) {
-    fs.coef.release();
-    fs.coef.ensure((size_t)nu * V * 640);
-    fs.cap = nu;
-    fs.key.assign(nu, FieldStore::Key{});
-    fs.valid.assign(nu, 0);
-    fs.used.assign(nu, 0);
-    fs.judge.assign(nu, 0.0);
-    for (auto& pl : cache->plans) pl->drop_graph();
-  }
+  window_reserve_fields(c, nu);
   const uint64_t gen = form == 2 ? c->qed->mlp_gen : 0;
   std::vector<int> call_slot(nu, -1);
   std::vector<char> taken(fs.cap, 0);

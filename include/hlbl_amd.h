@@ -231,6 +231,11 @@ int hlbl_window_drop_fields(hlbl_contractor* c);
 /* n = the number of coefficient fields the last window call built (its distinct yv
  * that no kept field served). No counterpart in the reference. */
 int hlbl_window_fields_built(const hlbl_contractor* c, int* n);
+/* At least n slots in the store of kept fields (each V * 640 floats on the device;
+ * growing drops the kept fields); slots (may be NULL) = the slots it now has.
+ * hlbl_run_chain* reserve one per distinct yv of the chain when the device memory
+ * beside the chain's blocks allows. No counterpart in the reference. */
+int hlbl_window_reserve_fields(hlbl_contractor* c, int n, int* slots);
 /* Measurement hook (no counterpart in the reference): runs a window of form 1 or 2
  * once without the graph, with HIP events around each stage on the launch stream;
  * ms = {pass 1 incl. its reductions, surrogate channels (0 for form 1),

@@ -401,6 +401,10 @@ def test_window_field_store(qed):
     assert part[:3] == fresh[:3]
     con.window_kernel_sum(w2, xunit * 1.5, form=1)
     assert con.fields_built() == len(yv1)
+    # a larger store drops the kept fields: the next call builds all of its own
+    slots = con.reserve_fields()
+    assert con.reserve_fields(slots + 2) == slots + 2
+    assert con.window_kernel_sum(w2, xunit, form=1) == fresh and con.fields_built() == len(yv1)
     # the plain FP64 form of the same pairs is unaffected
     ref = [con.kernel_sum(*d, xunit) for d in w2[:2]]
     err = np.abs(np.array(fresh[:2]) - np.array(ref)) / np.abs(np.array(ref))
@@ -711,7 +715,8 @@ def test_config_32x64_two_chains_source_batches(qed):
                 errs.append(max(abs(a - b) / abs(b) for a, b in zip(r["kernel_sum"], ref)))
                 del X, Y
         log("32^3x64 chain", h, "contract s", st["seconds_contract"], "solve s", st["seconds_solve"])
-    log("32^3x64 form 1, sampled pairs: max rel err", max(errs), "median", np.median(errs))
+    log("32^3x64 form 1, sampled pairs: max rel err", max(errs), "median", np.median(errs),
+        "field slots", con.reserve_fields())
     assert len(errs) == 30 and max(errs) < 1e-3
     con.close()
 
