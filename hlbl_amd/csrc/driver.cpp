@@ -336,7 +336,9 @@ void run_chain(hlbl_contractor* c, const Schedule& s, uint64_t seed, double xuni
     const size_t field_b = c->volume * 640 * sizeof(float);
     const size_t avail = free_b > blocks_b + margin ? free_b - blocks_b - margin : 0;
     const size_t fit = avail / field_b;
-    const int want = (int)std::min<size_t>(yvs.size(), fit);
+    // the surrogate form keeps its own fields and, when its judge sends a window back to
+    // the plain kernel, form 1's too
+    const int want = (int)std::min<size_t>(yvs.size() * (form == 2 ? 2 : 1), fit);
     if (want > 0) window_reserve_fields(c, want);
   }
   auto pair_done = [&](const Vec4& a, const Vec4& b) {
