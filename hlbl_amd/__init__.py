@@ -430,7 +430,8 @@ class Contractor:
         check(lib().hlbl_window_set_judge_bound(self.handle, float(bound)))
 
     def drop_fields(self):
-        """Forget the kept coefficient fields (the next window call builds its own)."""
+        """Forget the kept coefficient fields and free their memory (the next window
+        call builds its own)."""
         check(lib().hlbl_window_drop_fields(self.handle))
 
     def fields_built(self):

@@ -135,7 +135,6 @@ struct FieldStore {
     const Key& a = key[s];
     return valid[s] && same(a.yv, k.yv) && a.xunit == k.xunit && a.form == k.form && a.gen == k.gen;
   }
-  void drop() { std::fill(valid.begin(), valid.end(), 0); }
 };
 
 struct WindowCache {
@@ -171,8 +170,18 @@ void window_set_judge_bound(hlbl_contractor* c, double bound) {
   window_cache(c)->judge_bound = bound;
 }
 
+// forget the kept fields and give their memory back (the captured graphs hold its
+// address: they are captured again)
 void window_drop_fields(hlbl_contractor* c) {
-  if (c->window) c->window->fields.drop();
+  if (!c->window) return;
+  FieldStore& fs = c->window->fields;
+  fs.coef.release();
+  fs.cap = 0;
+  fs.key.clear();
+  fs.valid.clear();
+  fs.used.clear();
+  fs.judge.clear();
+  for (auto& pl : c->window->plans) pl->drop_graph();
 }
 
 // at least n slots in the field store (growing it drops the kept fields and every

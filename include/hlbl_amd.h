@@ -225,8 +225,8 @@ int hlbl_window_judge(const hlbl_contractor* c, double* max_rel_err, int* n_samp
  * in the reference (the surrogate is this library's addition). */
 int hlbl_window_set_judge_bound(hlbl_contractor* c, double bound);
 int hlbl_window_fallback(const hlbl_contractor* c, int* fell_back);
-/* Forget the kept coefficient fields: the next window call builds all of its own. No
- * counterpart in the reference. */
+/* Forget the kept coefficient fields and free their device memory: the next window call
+ * builds all of its own. No counterpart in the reference. */
 int hlbl_window_drop_fields(hlbl_contractor* c);
 /* n = the number of coefficient fields the last window call built (its distinct yv
  * that no kept field served). No counterpart in the reference. */

@@ -697,10 +697,14 @@ def test_config_32x64_two_chains_source_batches(qed):
     con = H.Contractor(lat, qed)
     errs = []
     for h in ((1, 1, 1, 1), (-1, 1, -1, 1)):
+        torch.cuda.empty_cache()  # the previous chain's check blocks back to the device
         sch = H.Schedule(h, 4, 4)
         assert sch.simulate(True)["peak_cache"] == 16
         rows, st = H.run_chain(con, sch, seed, xunit, residency=0, form=1, batch=2)
         assert len(rows) == 275 and st["cache_peak"] == 16
+        slots = con.reserve_fields()
+        con.drop_fields()  # the kept fields' memory back before the FP64 checks
+        assert con.reserve_fields() == 0
         assert [(r["y"], r["x"]) for r in rows] == \
             [(tuple(np.mod(p, dims)), tuple(np.mod(q, dims))) for q, p in sch.pairs()]
         by_window = {}
@@ -716,7 +720,7 @@ def test_config_32x64_two_chains_source_batches(qed):
                 del X, Y
         log("32^3x64 chain", h, "contract s", st["seconds_contract"], "solve s", st["seconds_solve"])
     log("32^3x64 form 1, sampled pairs: max rel err", max(errs), "median", np.median(errs),
-        "field slots", con.reserve_fields())
+        "field slots during the chains", slots)
     assert len(errs) == 30 and max(errs) < 1e-3
     con.close()
 
