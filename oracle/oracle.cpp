@@ -32,7 +32,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <array>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -342,10 +345,38 @@ const GTerms& gterms() {
   return G;
 }
 
+// Where the T tensors' frame components come from: 0 the table (default); 1 the table,
+// recording each argument point (|x|, beta, |y|) met; 2 values given per point (the
+// kernel's integrals evaluated there), the table for a point without one. Points are
+// keyed by their three coordinates rounded to 1e-12.
+int g_channel_mode = 0;
+std::mutex g_channel_mu;
+std::map<std::array<long long, 3>, std::vector<double>> g_channel_points;
+long long g_channel_misses = 0;
+std::array<long long, 3> channel_key(double a, double be, double b) {
+  return {std::llround(a * 1e12), std::llround(be * 1e12), std::llround(b * 1e12)};
+}
+void channels(double a, double be, double b, double* ch) {
+  if (g_channel_mode == 2) {
+    const auto it = g_channel_points.find(channel_key(a, be, b));
+    if (it != g_channel_points.end()) {
+      std::memcpy(ch, it->second.data(), NCH * sizeof(double));
+      return;
+    }
+    std::lock_guard<std::mutex> l(g_channel_mu);
+    ++g_channel_misses;
+  }
+  table().interp(a, be, b, ch);
+  if (g_channel_mode == 1) {
+    std::lock_guard<std::mutex> l(g_channel_mu);
+    g_channel_points.emplace(channel_key(a, be, b), std::vector<double>{a, be, b});
+  }
+}
+
 void l0_eval(const double* x, const double* y, double* L0) {
   double a, be, b, ch[NCH];
   grid_coords(x, y, a, be, b);
-  table().interp(a, be, b, ch);
+  channels(a, be, b, ch);
   double T[3][64];
   t_tensors(x, y, ch, T);
   const double* Tf = &T[0][0];
@@ -680,6 +711,34 @@ int oracle_load_table(const char* path) {
 }
 // interpolated frame components (42) at (a = |x|, beta, b = |y|)
 void oracle_channels_interp(double a, double be, double b, double* ch) { table().interp(a, be, b, ch); }
+
+// channel source of the kernel evaluations: 0 table, 1 table recording the points met,
+// 2 the values set by oracle_channel_points_set (table where none); switching to 1
+// clears the recorded points
+void oracle_channel_mode(int mode) {
+  if (mode == 1) g_channel_points.clear();
+  g_channel_misses = 0;
+  g_channel_mode = mode;
+}
+// the recorded points: n = their count; with out, [n][3] = (|x|, beta, |y|)
+int oracle_channel_points(double* out) {
+  int i = 0;
+  for (const auto& kv : g_channel_points) {
+    if (out)
+      for (int j = 0; j < 3; ++j) out[3 * i + j] = kv.second[j];
+    ++i;
+  }
+  return i;
+}
+// values for n points: abb [n][3], ch [n][42]
+void oracle_channel_points_set(const double* abb, const double* ch, int n) {
+  g_channel_points.clear();
+  for (int i = 0; i < n; ++i)
+    g_channel_points[channel_key(abb[3 * i], abb[3 * i + 1], abb[3 * i + 2])] =
+        std::vector<double>(ch + (size_t)NCH * i, ch + (size_t)NCH * (i + 1));
+}
+// evaluations in mode 2 that found no value
+long long oracle_channel_misses(void) { return g_channel_misses; }
 // the table [33][33][49][42] as doubles
 void oracle_table(double* out) {
   const Table& t = table();
