@@ -42,7 +42,7 @@ class Pair(C.Structure):
 
 class Tuning(C.Structure):
     """Launch shapes of a contractor's kernels (hlbl_tuning of include/hlbl_amd.h)."""
-    _fields_ = [(n, C.c_int) for n in ("pass1_groups", "field_waves", "field_wg_per_cu", "axes_wg_per_cu",
+    _fields_ = [(n, C.c_int) for n in ("pass1_batch", "field_waves", "field_wg_per_cu", "axes_wg_per_cu",
                                         "near_wg_per_cu", "pass2_wg_per_cu", "fp64_field_wg_per_cu",
                                         "fp64_pass2_wg_per_cu")]
 

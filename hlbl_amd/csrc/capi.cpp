@@ -307,7 +307,7 @@ int hlbl_contractor_set_tuning(hlbl_contractor* c, const hlbl_tuning* t) {
     auto in = [](int v, int lo, int hi, const char* what) {
       if (v < lo || v > hi) throw Error(std::string("hlbl_contractor_set_tuning: ") + what + " out of range");
     };
-    in(t->pass1_groups, 1, 64, "pass1_groups");
+    in(t->pass1_batch, 1, 1 << 20, "pass1_batch");
     if (t->field_waves != 4 && t->field_waves != 8) throw Error("hlbl_contractor_set_tuning: field_waves must be 4 or 8");
     in(t->field_wg_per_cu, 1, t->field_waves == 8 ? 2 : 3, "field_wg_per_cu");
     in(t->axes_wg_per_cu, 1, 16, "axes_wg_per_cu");

@@ -121,7 +121,7 @@ void window_cache_free(WindowCache* w);
 }  // namespace hlbl
 
 namespace hlbl {
-inline hlbl_tuning default_tuning() { return hlbl_tuning{8, 8, 2, 4, 1, 1, 2, 2}; }
+inline hlbl_tuning default_tuning() { return hlbl_tuning{32, 8, 2, 4, 1, 1, 2, 2}; }
 }  // namespace hlbl
 
 struct hlbl_contractor {

@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kPlThreads, 2) void pass1_planes(
 // accumulation; about 21 significant bits per product); each chunk's sums are
 // unscaled exactly into FP32 running sums.
 //
-// One workgroup of 8 waves per (pair, flavour, plane), walking the plane in chunks
+// One workgroup of 12 waves per (pair, flavour, plane), walking the plane in chunks
 // of 8 sites of a line. K of a chunk: site 4 ks + kb (MFMA step ks = 0, 1; lane group
 // kb = l / 16) supplies the lane's 8 FP16 values, (Re, Im) of colours 0..2 and two
 // zeros. MFMA waves 0..3 = (g, ks) hold all nine 16 x 16 tiles of their half and
@@ -191,7 +191,14 @@ __global__ __launch_bounds__(kPlThreads, 2) void pass1_planes(
 // fragment order; the MFMA waves multiply. v_mfma_f32_16x16x32_f16: A[i][k] is element k % 8 of lane
 // i + 16 (k / 8), B[k][j] of lane j + 16 (k / 8); result r of lane l is
 // (4 (l / 16) + r, l % 16).
+//
+// Placement: the planes are dealt to the XCDs in turn, and an XCD runs every unit of
+// its plane, in the host's order (lists of units that share blocks, one after another).
+// Each block's plane is then fetched by one XCD only, and the units running together
+// on it (one per CU) read their shared blocks from its L2.
 namespace {
+
+constexpr unsigned kXcds = 8;  // MI355X: 8 XCDs of 32 CUs
 constexpr int kH1Mfma = 4;
 constexpr int kH1Threads = 64 * (kH1Mfma + 8);  // + 4 splitters + 4 loaders
 constexpr int kH1Sites = 8;                       // sites per chunk
@@ -262,20 +269,20 @@ __device__ inline unsigned h1_imrow(unsigned d) { return __builtin_amdgcn_alignb
 
 __global__ __launch_bounds__(kH1Threads, 1) void pass1_mfma(
     const WinPair* __restrict__ pairs, WinPair single, hlbl_geom global, hlbl_geom local,
-    hlbl_coord proc, int flav_mask, const int* __restrict__ units, int unit_slots, int groups,
+    hlbl_coord proc, int flav_mask, const int* __restrict__ order, int n_units, int n_planes,
     float* __restrict__ part) {
   __shared__ __attribute__((aligned(16))) float sF[2][kH1F];
   __shared__ __attribute__((aligned(16))) _Float16 sH[2][kH1H];
   __shared__ unsigned sMax[4][2];  // per chunk c % 4: largest |S|, |Y| (float bits)
-  int unit = blockIdx.x;
-  if (units) {
-    unit = units[(blockIdx.x % groups) * unit_slots + blockIdx.x / groups];
-    if (unit < 0) return;
-  }
+  // workgroup j of XCD k (workgroups are dealt to the XCDs in turn): plane
+  // kXcds (j / n_units) + k, unit order[j % n_units]
+  const unsigned xcd = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+  const int plane = kXcds * (int)(j / (unsigned)n_units) + (int)xcd;
+  if (plane >= n_planes) return;
+  const int unit = order ? order[j % (unsigned)n_units] : (int)(j % (unsigned)n_units);
   const int ip = unit >> 1, f = unit & 1;
   if (!((flav_mask >> f) & 1)) return;
   const WinPair wp = pairs ? pairs[ip] : single;
-  const int plane = blockIdx.y;
   const int LY = local.LY, LZ = local.LZ;
   const size_t V = geom_volume(local);
   const size_t pbase = (size_t)plane * LY * LZ;
@@ -506,7 +513,7 @@ __global__ __launch_bounds__(kH1Threads, 1) void pass1_mfma(
   }
   __syncthreads();
   if (ks) return;
-  float* out = part + (((size_t)ip * 2 + f) * gridDim.y + plane) * kPlPart;
+  float* out = part + (((size_t)ip * 2 + f) * n_planes + plane) * kPlPart;
   const int kb = lane >> 4, r16 = lane & 15;
 #pragma unroll
   for (int rt = 0; rt < 3; ++rt)
@@ -594,14 +601,16 @@ template <class T>
 void launch_pass1_planes(const WinPair* d_pairs, const WinPair& single, int n_pairs,
                          const hlbl_geom& global, const hlbl_geom& local, const hlbl_coord& proc,
                          int flav_mask, T* part, double* red1, double* xi, hipStream_t st,
-                         const int* d_units, int unit_slots, int groups) {
+                         const int* d_order) {
   if (n_pairs <= 0) return;
   const int np = n_planes(local), nc = n_chunks(local);
-  if constexpr (std::is_same<T, float>::value)
-    hipLaunchKernelGGL(pass1_mfma, dim3(d_units ? groups * unit_slots : 2 * n_pairs, np),
-                       dim3(kH1Threads), 0, st, d_pairs, single, global, local, proc, flav_mask,
-                       d_units, unit_slots, groups, part);
-  else
+  if constexpr (std::is_same<T, float>::value) {
+    // every XCD gets every unit of its planes: kXcds * ceil(np / kXcds) * units workgroups
+    const size_t nwg = (size_t)kXcds * div_up(np, kXcds) * 2 * n_pairs;
+    if (nwg > 0x7fffffffu) throw Error("pass 1: too many workgroups for one launch");
+    hipLaunchKernelGGL(pass1_mfma, dim3((unsigned)nwg), dim3(kH1Threads), 0, st, d_pairs, single, global,
+                       local, proc, flav_mask, d_order, 2 * n_pairs, np, part);
+  } else
     hipLaunchKernelGGL(pass1_planes<T>, dim3(2 * n_pairs, np), dim3(kPlThreads), 0, st, d_pairs,
                        single, global, local, proc, flav_mask, part);
   check_hip(hipGetLastError(), "pass1 plane sums launch");
@@ -615,9 +624,9 @@ void launch_pass1_planes(const WinPair* d_pairs, const WinPair& single, int n_pa
 
 template void launch_pass1_planes<double>(const WinPair*, const WinPair&, int, const hlbl_geom&,
                                           const hlbl_geom&, const hlbl_coord&, int, double*,
-                                          double*, double*, hipStream_t, const int*, int, int);
+                                          double*, double*, hipStream_t, const int*);
 template void launch_pass1_planes<float>(const WinPair*, const WinPair&, int, const hlbl_geom&,
                                          const hlbl_geom&, const hlbl_coord&, int, float*, double*,
-                                         double*, hipStream_t, const int*, int, int);
+                                         double*, hipStream_t, const int*);
 
 }  // namespace hlbl

@@ -40,9 +40,9 @@ struct WindowPlan {
   PinBuf<int> h_slots;
   std::vector<int> call_slots;  // the store slots of the call's distinct yv
   float* coef = nullptr;        // the field store [slot][V][640]
-  int groups = 0, unit_slots = 0;  // pass-1 unit lists: groups x unit_slots (pair * 2 + f, or -1)
-  DevBuf<int> d_units;
-  PinBuf<int> h_units;
+  int batch = 0;            // pass-1 units per list
+  DevBuf<int> d_order;      // pass-1 unit order (pair * 2 + f), lists of `batch` that share blocks
+  PinBuf<int> h_order;
   DevBuf<float> part1;     // pass-1 plane partials
   DevBuf<double> red1, xi, dz, gdz;
   DevBuf<unsigned short> bsplit;  // pass-2 B operands, FP16 hi/lo in MFMA fragment order

@@ -674,7 +674,7 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
   // pass 1
   if (stage == 0) {
     launch_pass1_planes<float>(w.d_pairs.p, WinPair{}, np, g, g, c->proc, 3, w.part1.p, w.red1.p,
-                               w.xi.p, st, w.d_units.p, w.unit_slots, w.groups);
+                               w.xi.p, st, w.d_order.p);
     launch_forms_batched(w.xi.p, w.dz.p, w.gdz.p, np, st);
     hipLaunchKernelGGL(bsplit_exp, dim3(1, np), dim3(128), 0, st, w.gdz.p, w.bexp.p);
     hipLaunchKernelGGL(form_bsplit, dim3((unsigned)div_up(kBsplitLen, 256), np), dim3(256), 0, st, w.gdz.p,

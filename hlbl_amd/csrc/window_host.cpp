@@ -37,8 +37,8 @@ constexpr int kJudgeSamples = 64;
 
 }  // namespace
 
-void pass1_group_units(const WinPair* pairs, int n_pairs, int* units, int unit_slots, int groups) {
-  const int nu = 2 * n_pairs, G = groups;
+void pass1_order_units(const WinPair* pairs, int n_pairs, int batch, int* order) {
+  const int nu = 2 * n_pairs, G = (nu + batch - 1) / batch, unit_slots = (nu + G - 1) / G;
   // the two (block, flavour) halves a unit reads: S[f] of x and Y[1-f] of y
   std::map<std::pair<const double*, int>, int> ids;
   auto hid = [&](const double* b, int f) {
@@ -95,16 +95,15 @@ void pass1_group_units(const WinPair* pairs, int n_pairs, int* units, int unit_s
     if (!improved) break;
   }
   std::vector<int> seen(nu, 0);
-  for (int k = 0; k < G; ++k) {
-    if ((int)grp[k].size() > unit_slots) throw Error("pass1_group_units: a list overflows its slots");
-    for (int j = 0; j < unit_slots; ++j) {
-      const int u = j < (int)grp[k].size() ? grp[k][j] : -1;
-      units[k * unit_slots + j] = u;
-      if (u >= 0) ++seen[u];
+  int j = 0;
+  for (int k = 0; k < G; ++k)
+    for (const int u : grp[k]) {
+      if (j == nu) throw Error("pass1_order_units: more units listed than there are");
+      order[j++] = u;
+      ++seen[u];
     }
-  }
   for (int u = 0; u < nu; ++u)
-    if (seen[u] != 1) throw Error("pass1_group_units: a unit is not listed exactly once");
+    if (seen[u] != 1) throw Error("pass1_order_units: a unit is not listed exactly once");
 }
 
 // Default gate of the surrogate form: a form-2 window whose judge (the largest
@@ -190,14 +189,14 @@ void window_reserve_fields(hlbl_contractor* c, int n) {
   WindowCache* cache = window_cache(c);
   FieldStore& fs = cache->fields;
   if (fs.cap >= n) return;
-  fs.coef.release();
+  // an empty store first, so that a failed allocation leaves no slots behind
+  window_drop_fields(c);
   fs.coef.ensure((size_t)n * c->volume * 640);
   fs.cap = n;
   fs.key.assign(n, FieldStore::Key{});
   fs.valid.assign(n, 0);
   fs.used.assign(n, 0);
   fs.judge.assign(n, 0.0);
-  for (auto& pl : cache->plans) pl->drop_graph();
 }
 
 int window_field_slots(const hlbl_contractor* c) { return c->window ? c->window->fields.cap : 0; }
@@ -294,10 +293,9 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
     w->coef = fs.coef.p;
     w->d_pairs.ensure(n);
     w->h_pairs.ensure(n);
-    w->groups = c->tune.pass1_groups;
-    w->unit_slots = (2 * n + w->groups - 1) / w->groups;
-    w->d_units.ensure((size_t)w->groups * w->unit_slots);
-    w->h_units.ensure((size_t)w->groups * w->unit_slots);
+    w->batch = c->tune.pass1_batch;
+    w->d_order.ensure((size_t)2 * n);
+    w->h_order.ensure((size_t)2 * n);
     w->part1.ensure(pass1_part_len(c->global, n));
     w->red1.ensure(pass1_red_len(c->global, n));
     w->xi.ensure((size_t)n * 2 * 8 * 5 * 144 * 2);
@@ -342,7 +340,7 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
                            mod_pos(pairs[i].gsx.y, g.LY), mod_pos(pairs[i].gsx.z, g.LZ)};
     w->h_pairs.p[i] = WinPair{pairs[i].d_x, pairs[i].d_y, gs, pairs[i].yv, call_slot[slot[i]], 0};
   }
-  pass1_group_units(w->h_pairs.p, n, w->h_units.p, w->unit_slots, w->groups);
+  pass1_order_units(w->h_pairs.p, n, w->batch, w->h_order.p);
   if (!cache->stream) check_hip(hipStreamCreateWithFlags(&cache->stream, hipStreamNonBlocking), "hipStreamCreate");
   return w;
 }
@@ -372,8 +370,8 @@ void window_tail_copies(WindowPlan* w, hipStream_t st) {
 }
 void window_head_copies(WindowPlan* w, hipStream_t st) {
   check_hip(hipMemcpyAsync(w->d_pairs.p, w->h_pairs.p, w->n_pairs * sizeof(WinPair), hipMemcpyHostToDevice, st), "copy pairs");
-  check_hip(hipMemcpyAsync(w->d_units.p, w->h_units.p, (size_t)w->groups * w->unit_slots * sizeof(int),
-                           hipMemcpyHostToDevice, st), "copy units");
+  check_hip(hipMemcpyAsync(w->d_order.p, w->h_order.p, (size_t)2 * w->n_pairs * sizeof(int),
+                           hipMemcpyHostToDevice, st), "copy unit order");
 }
 // the fields the store lacks: their list, the surrogate's channels (form 2), the fields
 // and the judge, launched directly on st (not part of the window's graph)

@@ -6,7 +6,7 @@ or, with --workload source, the call of window 1's source with the most partners
 per-source calls a lattice uses whose windows do not fit in memory, as 32^3x64).
 Then, one shape at a time (coordinate descent from the defaults), it times every
 candidate and keeps the fastest:
-  - pass1_groups, pass2_wg_per_cu: the window's pass-1 and pass-2 stage times (HIP
+  - pass1_batch, pass2_wg_per_cu: the window's pass-1 and pass-2 stage times (HIP
     events of hlbl_window_profile, fields kept);
   - field_waves x field_wg_per_cu, axes_wg_per_cu, near_wg_per_cu: the coefficient-field
     stage with every field built (the kept ones dropped before each run);
@@ -37,7 +37,7 @@ import hlbl_amd as H  # noqa: E402
 
 # candidates per knob, in the order they are tuned; a tuple value sets two fields
 CANDIDATES = [
-    ("pass1_groups", [1, 2, 4, 8, 16, 32], "pass1"),
+    ("pass1_batch", [8, 16, 24, 32, 48, 64, 1 << 20], "pass1"),
     ("pass2_wg_per_cu", [1, 2, 3, 4], "pass2"),
     (("field_waves", "field_wg_per_cu"), [(8, 1), (8, 2), (4, 1), (4, 2), (4, 3)], "fields"),
     ("axes_wg_per_cu", [1, 2, 4, 8, 16], "fields"),

@@ -148,9 +148,9 @@ int hlbl_contractor_free(hlbl_contractor* c);
  * layer applies to every contractor it creates. Results do not depend on the shapes
  * beyond the order of the FP64 sums across workgroups (pass 2, plain and window). */
 typedef struct {
-  int pass1_groups;          /* window pass 1: unit lists dealt to the workgroup slots in
-                                turn (units of a list share blocks in one XCD's L2);
-                                1..64, default 8 */
+  int pass1_batch;           /* window pass 1: the units of a plane run on one XCD in lists
+                                of at most this many that share blocks (they share them
+                                in its L2); 1..2^20, default 32 (the CUs of an XCD) */
   int field_waves;           /* window coefficient field: waves per workgroup (one site
                                 each), 4 or 8, default 8 */
   int field_wg_per_cu;       /* its workgroups per CU: 1..2 (8 waves) or 1..3 (4 waves),

@@ -426,7 +426,7 @@ def test_tuning_shapes(qed):
     xunit = 0.3
     base = np.array(con.window_kernel_sum(pairs, xunit, form=1))
     base64 = np.array(con.kernel_sum(*pairs[0], xunit))
-    for t in ({"pass1_groups": 1, "pass2_wg_per_cu": 3}, {"pass1_groups": 32, "field_waves": 4, "field_wg_per_cu": 3},
+    for t in ({"pass1_batch": 1, "pass2_wg_per_cu": 3}, {"pass1_batch": 1 << 20, "field_waves": 4, "field_wg_per_cu": 3},
               {"axes_wg_per_cu": 1, "near_wg_per_cu": 8, "fp64_field_wg_per_cu": 7, "fp64_pass2_wg_per_cu": 1}):
         con.set_tuning(**t)
         got = np.array(con.window_kernel_sum(pairs, xunit, form=1))
@@ -437,7 +437,7 @@ def test_tuning_shapes(qed):
         assert np.all(np.abs(got - base) <= 1e-12 * np.abs(base)), t
         assert np.all(np.abs(cold - base) <= 1e-12 * np.abs(base)), t
         assert np.all(np.abs(got64 - base64) <= 1e-12 * np.abs(base64)), t
-    for bad in ({"field_waves": 6}, {"field_waves": 8, "field_wg_per_cu": 3}, {"pass1_groups": 0}):
+    for bad in ({"field_waves": 6}, {"field_waves": 8, "field_wg_per_cu": 3}, {"pass1_batch": 0}):
         with pytest.raises(H.HlblError):
             con.set_tuning(**bad)
     con.close()
