@@ -361,7 +361,7 @@ struct S2Cursor {
 // double pair is d = j + 24 k (j = ptid % 24) of the key's 144: row d / 12,
 // doubles 2 (d % 12) of the site. S2Lane holds the lane's fixed byte offsets.
 struct S2Lane {
-  size_t row_off[kS2Fetch];  // (row * V * 24 + 2 (d % 12)) * 8
+  unsigned row_off[kS2Fetch];  // (row * V * 24 + 2 (d % 12)) / 2: 16-byte units (V < 2^32 / 144)
   int key, isy, site;
 };
 
@@ -374,7 +374,7 @@ __device__ inline S2Lane s2_lane(int ptid, size_t V) {
 #pragma unroll
   for (int k = 0; k < kS2Fetch; ++k) {
     const int d = j + kS2KeyLanes * k;
-    L.row_off[k] = ((size_t)(d / 12) * V * 24 + 2 * (d % 12)) * sizeof(double);
+    L.row_off[k] = (unsigned)((size_t)(d / 12) * V * 12 + (d % 12));
   }
   return L;
 }
@@ -391,7 +391,7 @@ __device__ inline void s2_fetch(const S2Lane& L, const WinPair* wp, int f, size_
   const int ff = L.isy ? f : 1 - f;
   const size_t base = (L.isy ? py : ps) + ((size_t)ff * 12 * V + z) * 24 * sizeof(double);
 #pragma unroll
-  for (int k = 0; k < kS2Fetch; ++k) r[k] = *reinterpret_cast<gd2v*>(base + L.row_off[k]);
+  for (int k = 0; k < kS2Fetch; ++k) r[k] = *reinterpret_cast<gd2v*>(base + (size_t)L.row_off[k] * 16);
 }
 
 // maximum over each group of 8 lanes (all lanes active), by DPP
@@ -465,40 +465,87 @@ __device__ inline int site_exp(unsigned ms, unsigned my) {
   return se < -120 ? -120 : (se > 120 ? 120 : se);
 }
 
-// dxu tasks t = t0 + 64 i + lane (i < N) of a staged group, task t = (site zi, r, a):
-// dxu_mu[r][a] of the four mu, scaled by 2^se of the site and split into FP16
-// (re, im) pairs into ah / al [(zi, mu)][kS2ARow] at k = r*24 + 2a; the site
-// exponents go to sse[zi] (from task (zi, 0, 0)).
-// The task's (zi, r, a) by exact 24-bit reciprocal multiplies (t < kS2Tasks = 1152:
-// t * 911 >> 17 = t / 144 and q * 171 >> 11 = q / 12 for q < 144), full-rate
-// multiplies in place of the quarter-rate 32-bit ones a division by a constant takes.
-static_assert(kS2Tasks <= 1152, "the reciprocal multiplies are exact below 1152 tasks");
+// dxu tasks of a staged group: task (site zi, r, a) is dxu_mu[r][a] of the four mu,
+// scaled by 2^se of the site and split into FP16 (re, im) pairs into ah / al
+// [(zi, mu)][kS2ARow] at k = r*24 + 2a; the site exponents go to sse[zi] (from the
+// task (zi, 0, 0)). Tasks are numbered t = 144 zi + 24 rp + 2 a + rb (r = 2 rp + rb),
+// so that the pair task P = (zi, rp, a), both r of the row pair rp with one read of
+// S row a, is the tasks 2P and 2P + 1.
+// The pair's (zi, rp, a) by exact 24-bit reciprocal multiplies (P < 576: P * 911 >> 16
+// = P / 72 and q * 171 >> 11 = q / 12 for q < 144), full-rate multiplies in place of
+// the quarter-rate 32-bit ones a division by a constant takes.
+static_assert(kS2Tasks == 1152, "the reciprocal multiplies are exact below 576 pair tasks");
+struct S2Task {
+  unsigned zi, rp, a;
+};
+__device__ inline S2Task s2_pair_task(unsigned P) {
+  const unsigned zi = __umul24(P, 911u) >> 16;
+  const unsigned q = P - __umul24(zi, 72u);
+  const unsigned rp = __umul24(q, 171u) >> 11;
+  return {zi, rp, q - __umul24(rp, 12u)};
+}
+
+// the four mu of task (zi, r, a) from dxu values, scaled and split into ah / al
+__device__ inline void s2_put(unsigned zi, unsigned r, unsigned a, float sc, const float (&re)[4],
+                              const float (&im)[4], unsigned* ah, unsigned* al) {
+  const unsigned o0 = (__umul24(zi, 4u * kS2ARow) + __umul24(r, 24u) + 2u * a) >> 1;
+#pragma unroll
+  for (int mu = 0; mu < 4; ++mu) {
+    unsigned hw, lw;
+    f16_split2_scaled(re[mu], im[mu], sc, hw, lw);
+    const unsigned o = o0 + mu * (kS2ARow / 2);
+    ah[o] = hw;
+    al[o] = lw;
+  }
+}
+
+// single tasks t = t0 + 64 i + lane (i < N)
 template <int N>
 __device__ inline void s2_dxu(int t0, int lane, const float* buf, const unsigned* smax, unsigned* ah,
                               unsigned* al, int* sse) {
 #pragma unroll 2
   for (int i = 0; i < N; ++i) {
     const unsigned t = (unsigned)(t0 + 64 * i + lane);
-    const unsigned zi = __umul24(t, 911u) >> 17;
-    const unsigned q = t - __umul24(zi, 144u);
-    const unsigned r = __umul24(q, 171u) >> 11;
-    const unsigned a = q - __umul24(r, 12u);
-    const int se = site_exp(smax[zi], smax[kS2Sites + zi]);
-    if (q == 0) sse[zi] = se;
-    const float sc = pow2f(se);
+    const S2Task k = s2_pair_task(t >> 1);
+    const unsigned r = 2u * k.rp + (t & 1u);
+    const int se = site_exp(smax[k.zi], smax[kS2Sites + k.zi]);
+    if (r == 0 && k.a == 0) sse[k.zi] = se;
     float re[4], im[4];
-    const unsigned zo = __umul24(zi, 24u);
-    dxu_vals_paired<float, float>(buf + __umul24(a, (unsigned)kS2RowPad) + zo,
+    const unsigned zo = __umul24(k.zi, 24u);
+    dxu_vals_paired<float, float>(buf + __umul24(k.a, (unsigned)kS2RowPad) + zo,
                                   buf + __umul24(12u + r, (unsigned)kS2RowPad) + zo, re, im);
-    const unsigned o0 = (__umul24(zi, 4u * kS2ARow) + __umul24(r, 24u) + 2u * a) >> 1;
+    s2_put(k.zi, r, k.a, pow2f(se), re, im, ah, al);
+  }
+}
+
+// pair tasks P = p0 + 64 i + lane (i < N): S row a read once for both r
+template <int N>
+__device__ inline void s2_dxu_pairs(int p0, int lane, const float* buf, const unsigned* smax, unsigned* ah,
+                                    unsigned* al, int* sse) {
+#pragma unroll 1
+  for (int i = 0; i < N; ++i) {
+    const S2Task k = s2_pair_task((unsigned)(p0 + 64 * i + lane));
+    const unsigned r = 2u * k.rp;
+    const int se = site_exp(smax[k.zi], smax[kS2Sites + k.zi]);
+    if (k.rp == 0 && k.a == 0) sse[k.zi] = se;
+    const unsigned zo = __umul24(k.zi, 24u);
+    float sa[24];
+    const float4* s4 = reinterpret_cast<const float4*>(buf + __umul24(k.a, (unsigned)kS2RowPad) + zo);
 #pragma unroll
-    for (int mu = 0; mu < 4; ++mu) {
-      unsigned hw, lw;
-      f16_split2_scaled(re[mu], im[mu], sc, hw, lw);
-      const unsigned o = o0 + mu * (kS2ARow / 2);
-      ah[o] = hw;
-      al[o] = lw;
+    for (int j = 0; j < 6; ++j) {
+      const float4 v = s4[j];
+      sa[4 * j] = v.x;
+      sa[4 * j + 1] = v.y;
+      sa[4 * j + 2] = v.z;
+      sa[4 * j + 3] = v.w;
     }
+    const float sc = pow2f(se);
+    const float* y0 = buf + __umul24(12u + r, (unsigned)kS2RowPad) + zo;
+    float re[4], im[4];
+    dxu_vals_paired<float, float>(sa, y0, re, im);
+    s2_put(k.zi, r, k.a, sc, re, im, ah, al);
+    dxu_vals_paired<float, float>(sa, y0 + kS2RowPad, re, im);
+    s2_put(k.zi, r + 1, k.a, sc, re, im, ah, al);
   }
 }
 
@@ -506,12 +553,15 @@ __device__ inline void s2_dxu(int t0, int lane, const float* buf, const unsigned
 // register shows the SIMD numbers as a permutation of wave % 4), and on a SIMD the
 // matrix and vector work of its waves add up (they hardly co-issue), so the dxu tasks
 // are spread to even out MFMA + vector time per SIMD: the SIMDs of waves = 0 and 1
-// (mod 4) carry three MFMA waves and producer 12 or 13 with 4 tasks per lane; those of
+// (mod 4) carry three MFMA waves and producer 12 or 13 with 4 tasks per lane, as two
+// rounds of pair tasks (a round of 64 pair tasks is 128 tasks; a pair reads S once for
+// two tasks, fewer LDS round trips on the SIMDs that set the step: pass 2 2.6% faster); those of
 // waves = 2 and 3 carry two MFMA waves and producers 10 + 14 or 11 + 15 with 3 + 2
-// tasks, and these four stage the coefficients of two sites each. Task ranges:
-// pw 2: [0, 256), pw 3: [256, 512), pw 0: [512, 704), pw 1: [704, 896),
-// pw 4: [896, 1024), pw 5: [1024, 1152).
-__device__ inline int s2_task0(int pw) { return pw == 2 ? 0 : pw == 3 ? 256 : pw == 0 ? 512 : pw == 1 ? 704 : pw == 4 ? 896 : 1024; }
+// single tasks, and these four stage the coefficients of two sites each. Ranges: pw 2:
+// pairs [0, 128) (tasks [0, 256)), pw 3: pairs [128, 256), pw 0: tasks [512, 704),
+// pw 1: [704, 896), pw 4: [896, 1024), pw 5: [1024, 1152). (Pairs on all six
+// producers measured the same: the SIMDs of waves = 2 and 3 do not set the step.)
+__device__ inline int s2_task0(int pw) { return pw == 2 ? 0 : pw == 3 ? 128 : pw == 0 ? 512 : pw == 1 ? 704 : pw == 4 ? 896 : 1024; }
 __device__ inline int s2_n_tasks(int pw) { return (pw == 2 || pw == 3) ? 4 : (pw == 0 || pw == 1) ? 3 : 2; }
 __device__ inline int s2_coef_site0(int pw) { return pw == 0 ? 0 : pw == 1 ? 2 : pw == 4 ? 4 : pw == 5 ? 6 : -1; }
 static_assert(2 * 256 + 2 * 192 + 2 * 128 == kS2Tasks, "the producer task ranges cover a group");
@@ -557,23 +607,24 @@ __global__ __launch_bounds__(kS2Threads) void pass2_window(
       unsigned* ah = reinterpret_cast<unsigned*>(sA[it & 1][0]);
       unsigned* al = reinterpret_cast<unsigned*>(sA[it & 1][1]);
       const int nt = s2_n_tasks(pw);
-      if (nt == 4) s2_dxu<4>(t0, lane, sF[it & 1], sMax[it % 3], ah, al, sSe[it & 1]);
+      if (nt == 4) s2_dxu_pairs<2>(t0, lane, sF[it & 1], sMax[it % 3], ah, al, sSe[it & 1]);
       else if (nt == 3) s2_dxu<3>(t0, lane, sF[it & 1], sMax[it % 3], ah, al, sSe[it & 1]);
       else s2_dxu<2>(t0, lane, sF[it & 1], sMax[it % 3], ah, al, sSe[it & 1]);
       nc.next(ng);
     };
-    // prologue: spinors of steps 0 and 1 staged, step 2 in flight; coefficients and
-    // dxu of step 0 built after the barrier
+    // prologue: spinors of steps 0 and 1 staged; coefficients and dxu of step 0 built
+    // after the barrier, then step 2's fetch issued (it flies through the loop's first
+    // barrier, as every later fetch does)
     fetch();
     s2_store(ptid, L, r, sF[0], sMax[0]);
     if (total > 1) {
       fetch();
       s2_store(ptid, L, r, sF[1], sMax[1]);
     }
+    barrier_all();
+    prepare(0);
     bool inflight = total > 2;
     if (inflight) fetch();
-    if (inflight) barrier_vm6(); else barrier_all();
-    prepare(0);
     int m3 = 0;  // it % 3
     for (int it = 0; it < total; ++it) {
       // waits for this step's coefficient loads but lets the later fetch fly
@@ -722,6 +773,8 @@ void window_stage(hlbl_contractor* c, WindowPlan& w, int stage, hipStream_t st) 
   }
   // pass 2
   if (stage == 2) {
+    // the producers address spinor rows in 16-byte units with 32-bit offsets
+    if (V * 144 >= ((size_t)1 << 32)) throw Error("window form: lattice too large for one block's offsets");
     const int nwg = c->tune.pass2_wg_per_cu * c->n_cu;
     const size_t spw = div_up(div_up(V, (size_t)nwg), kS2Sites) * kS2Sites;
     const int n = (int)div_up(V, spw);
