@@ -195,7 +195,7 @@ def main():
                          "lattice and print their own line instead of the kernel-sum line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
+        print(f"bench.py: measures one GPU only (got --gpus {args.gpus}); "
               "the multi-GPU path is a design (sites sharded, one allreduce per pass).", file=sys.stderr)
         sys.exit(2)
 
