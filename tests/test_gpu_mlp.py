@@ -225,3 +225,37 @@ def test_config4_surrogate_on_config2_window(qed):
     log(f"config 4 gated: fell back {con.fell_back()}, per-pair rel err max {e.max():.3e}")
     assert e.max() < 1e-3
     con.close()
+
+
+@pytest.mark.timeout(600)
+def test_config5_surrogate_chain_gated(qed):
+    """The at-scale configuration's named form: 32^3x64, chain K=4, n=4, h=(+1,+1,+1,+1),
+    RandomProvider seed 3, at most 16 blocks alive, one FP32 window call per new source
+    (run_chain batch 2) with the surrogate (trained over the lattice's reach) and the judge
+    gating every call, against the same chain in form 1. Every pair sum is within 1e-3 of
+    form 1's; a call whose judge exceeds the bound returns form 1's sums bit for bit (the
+    count of such rows is stated)."""
+    torch.cuda.empty_cache()
+    dims = (64, 32, 32, 32)
+    xunit = H.xunit(105.658, 0.1)
+    qed.mlp_set_domain(xunit * float(np.linalg.norm([d / 2 for d in dims])))
+    loss = qed.mlp_train(n_samples=1 << 20, epochs=8, batch=1024, lr=2e-3, seed=4)
+    log("config 5 surrogate training loss", loss[0], "->", loss[-1])
+    lat = H.Lattice(dims)
+    con = H.Contractor(lat, qed)
+    sch = H.Schedule((1, 1, 1, 1), 4, 4)
+    rows2, st2 = H.run_chain(con, sch, 3, xunit, residency=0, form=2, batch=2)
+    con.drop_fields()
+    torch.cuda.empty_cache()
+    rows1, st1 = H.run_chain(con, sch, 3, xunit, residency=0, form=1, batch=2)
+    assert len(rows2) == len(rows1) == 275 and st2["cache_peak"] == 16
+    assert [(r["x"], r["y"], r["yv"]) for r in rows2] == [(r["x"], r["y"], r["yv"]) for r in rows1]
+    k2 = np.array([r["kernel_sum"] for r in rows2])
+    k1 = np.array([r["kernel_sum"] for r in rows1])
+    e = np.abs(k2 - k1) / np.abs(k1)
+    same = int(np.sum(np.all(k2 == k1, axis=1)))
+    log(f"config 5 form 2 (gated) vs form 1: {same} of {len(rows2)} rows form 1's bit for bit, "
+        f"max rel diff {e.max():.3e}; contraction {st2['seconds_contract']:.2f} s against "
+        f"{st1['seconds_contract']:.2f} s")
+    assert e.max() <= 1e-3
+    con.close()
