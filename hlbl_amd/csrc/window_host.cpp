@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <array>
 #include <map>
+#include <tuple>
 #include <cmath>
 #include <memory>
 #include <vector>
@@ -136,9 +137,14 @@ struct FieldStore {
   }
 };
 
+// the surrogate's judge of a field, remembered after the field itself is evicted:
+// (yv, xunit, surrogate generation)
+using JudgeKey = std::tuple<int, int, int, int, double, uint64_t>;
+
 struct WindowCache {
   std::vector<std::unique_ptr<WindowPlan>> plans;
   FieldStore fields;
+  std::map<JudgeKey, double> judged;  // judges of the surrogate's fields built so far
   hipStream_t stream = nullptr;
   double last_judge = -1.0;
   int last_judge_n = 0;
@@ -279,16 +285,17 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
   const int nb = (int)build.size();
   cache->last_built = nb;
 
-  // One plan whose buffers only grow: a window of another size or form reuses them and
-  // its graph is captured again, so the footprint is that of the largest window so far
-  // and a run of differently sized batches does not reallocate.
+  // One plan whose buffers only grow: a window of another size reuses them and its graph
+  // is captured again, so the footprint is that of the largest window so far and a run
+  // of differently sized batches does not reallocate. The graph (pass 1 and pass 2) does
+  // not depend on the form or xunit, which only the field builds read.
   if (cache->plans.empty()) cache->plans.push_back(std::make_unique<WindowPlan>());
   WindowPlan* w = cache->plans.front().get();
-  if (!(w->n_pairs == n && w->form == form && w->xunit == xunit && w->coef == fs.coef.p && w->exec)) {
+  w->form = form;
+  w->xunit = xunit;
+  if (!(w->n_pairs == n && w->coef == fs.coef.p && w->exec)) {
     w->drop_graph();
     w->n_pairs = n;
-    w->form = form;
-    w->xunit = xunit;
     w->n_samples = kJudgeSamples;
     w->coef = fs.coef.p;
     w->d_pairs.ensure(n);
@@ -346,9 +353,9 @@ WindowPlan* window_prepare(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
 }
 
 namespace {
-// results; with the surrogate, the judge of the call's fields (built now or before)
-void window_finish(hlbl_contractor* c, WindowPlan* w, double* out) {
-  for (int i = 0; i < 2 * w->n_pairs; ++i) out[i] = w->h_result.p[i];
+// the fields the call built are valid, with their judges (surrogate form); the call's
+// judge is the largest of its fields' (built now or before)
+void window_record_fields(hlbl_contractor* c, WindowPlan* w) {
   FieldStore& fs = c->window->fields;
   for (int j = 0; j < w->n_unique; ++j) {
     const int s = w->h_slots.p[j];
@@ -357,6 +364,10 @@ void window_finish(hlbl_contractor* c, WindowPlan* w, double* out) {
       for (int k = 0; k < w->n_samples; ++k) e = std::max(e, w->h_judge.p[(size_t)j * w->n_samples + k]);
     fs.judge[s] = e;
     fs.valid[s] = 1;
+    if (w->form == 2) {
+      const hlbl_coord& y = w->h_yvs.p[j];
+      c->window->judged[JudgeKey{y.t, y.x, y.y, y.z, w->xunit, c->qed->mlp_gen}] = e;
+    }
   }
   if (w->form == 2) {
     double e = 0.0;
@@ -364,6 +375,9 @@ void window_finish(hlbl_contractor* c, WindowPlan* w, double* out) {
     c->window->last_judge = e;
     c->window->last_judge_n = (int)w->call_slots.size() * w->n_samples;
   }
+}
+void window_results(WindowPlan* w, double* out) {
+  for (int i = 0; i < 2 * w->n_pairs; ++i) out[i] = w->h_result.p[i];
 }
 void window_tail_copies(WindowPlan* w, hipStream_t st) {
   check_hip(hipMemcpyAsync(w->h_result.p, w->result.p, 2 * w->n_pairs * sizeof(double), hipMemcpyDeviceToHost, st), "copy result");
@@ -406,6 +420,34 @@ struct BuildGuard {
 
 void window_run(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit, int form,
                 double* out, hipStream_t user_stream) {
+  if (form == 2 && c->qed->mlp_ready) {
+    // a yv whose surrogate field has failed the judge before (kept or since evicted)
+    // fails it again: the call goes to the plain kernel without building any surrogate
+    // field
+    WindowCache* cache = window_cache(c);
+    double known = -1.0;
+    int n_known = 0;
+    std::vector<hlbl_coord> seen;
+    for (int i = 0; i < n; ++i) {
+      const hlbl_coord y = canonical_yv(c->global, pairs[i].yv);
+      bool dup = false;
+      for (const hlbl_coord& s : seen) dup = dup || same(s, y);
+      if (dup) continue;
+      seen.push_back(y);
+      const auto it = cache->judged.find(JudgeKey{y.t, y.x, y.y, y.z, xunit, c->qed->mlp_gen});
+      if (it != cache->judged.end()) {
+        known = std::max(known, it->second);
+        ++n_known;
+      }
+    }
+    if (known > cache->judge_bound) {
+      window_run(c, pairs, n, xunit, 1, out, user_stream);
+      cache->last_judge = known;
+      cache->last_judge_n = n_known * kJudgeSamples;
+      cache->last_fallback = 1;
+      return;
+    }
+  }
   WindowPlan* w = window_prepare(c, pairs, n, xunit, form);
   BuildGuard guard{c, w};
   hipStream_t st = c->window->stream;
@@ -431,12 +473,13 @@ void window_run(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double x
   if (user_stream) check_hip(hipStreamSynchronize(user_stream), "user stream");
   else check_hip(hipDeviceSynchronize(), "device");
   window_builds(c, w, st, nullptr);
-  check_hip(hipGraphLaunch(w->exec, st), "hipGraphLaunch");
-  check_hip(hipStreamSynchronize(st), "window");
-  guard.done = true;
-  window_finish(c, w, out);
   if (form == 2) {
-    // the judge as a gate: past the bound, the plain kernel's results instead
+    // the judge as a gate, before the window runs: past the bound (the judge of the
+    // fields built now, or recorded when the kept ones were built), the plain kernel's
+    // fields and results instead, and the surrogate's passes are not run at all
+    check_hip(hipStreamSynchronize(st), "window fields");
+    window_record_fields(c, w);
+    guard.done = true;
     WindowCache* cache = c->window;
     cache->last_fallback = cache->last_judge > cache->judge_bound ? 1 : 0;
     if (cache->last_fallback) {
@@ -446,8 +489,14 @@ void window_run(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double x
       cache->last_judge = judge;
       cache->last_judge_n = judge_n;
       cache->last_fallback = 1;
+      return;
     }
   }
+  check_hip(hipGraphLaunch(w->exec, st), "hipGraphLaunch");
+  check_hip(hipStreamSynchronize(st), "window");
+  guard.done = true;
+  if (form != 2) window_record_fields(c, w);
+  window_results(w, out);
 }
 
 void window_profile(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit, int form,
@@ -481,7 +530,8 @@ void window_profile(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, doub
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   guard.done = true;
-  window_finish(c, w, out);
+  window_record_fields(c, w);
+  window_results(w, out);
 }
 
 }  // namespace hlbl
