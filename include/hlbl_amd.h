@@ -219,10 +219,12 @@ int hlbl_window_kernel_sum(hlbl_contractor* c, const hlbl_pair_desc* pairs, int 
 /* the surrogate's judge for the last form-2 window: max relative deviation of the
  * surrogate kernel from the plain one over the sampled sites */
 int hlbl_window_judge(const hlbl_contractor* c, double* max_rel_err, int* n_samples);
-/* The judge gates form 2: a window whose judge exceeds `bound` (default 1e-3) is
- * recomputed with the plain kernel (form 1), and those results are returned;
- * hlbl_window_fallback tells whether the last form-2 window did so. No counterpart
- * in the reference (the surrogate is this library's addition). */
+/* The judge gates form 2: a window whose judge exceeds `bound` (default 1e-3) runs
+ * with the plain kernel (form 1) instead, decided before the window's passes run (the
+ * judge of each surrogate field is taken when the field is built and remembered per
+ * yv, xunit and network), and those results are returned; hlbl_window_fallback tells
+ * whether the last form-2 window did so. No counterpart in the reference (the
+ * surrogate is this library's addition). */
 int hlbl_window_set_judge_bound(hlbl_contractor* c, double bound);
 int hlbl_window_fallback(const hlbl_contractor* c, int* fell_back);
 /* Forget the kept coefficient fields and free their device memory: the next window call
