@@ -53,6 +53,12 @@ void check_ptr(const void* p, const char* what) {
   if (!p) throw Error(std::string(what) + " is null");
 }
 
+// m_mu a in muon units: the kernels scale site coordinates by it (and size the
+// near-origin FP64 region by its inverse)
+void check_xunit(double xunit) {
+  if (!(xunit > 0) || !std::isfinite(xunit)) throw Error("xunit must be positive and finite");
+}
+
 int device_cu_count(int device) {
   int n = 0;
   check_hip(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device),
@@ -337,6 +343,7 @@ int hlbl_kernel_sum(hlbl_contractor* c, const double* d_fwd_src, const double* d
     check_ptr(d_fwd_src, "d_fwd_src");
     check_ptr(d_fwd_y, "d_fwd_y");
     check_ptr(out, "out");
+    check_xunit(xunit);
     hipStream_t st = as_stream(stream);
     launch_pass1(c, d_fwd_src, d_fwd_y, c->proc, gsx, c->global, c->local, 3, st);
     c->gdz.ensure(2 * 40 * 144 * 2);
@@ -447,6 +454,7 @@ int hlbl_4pt_contraction(hlbl_contractor* c, double* d_kernel_sum, const double*
     check_ptr(fwd_y, "fwd_y");
     if (iflavor != 0 && iflavor != 1) throw Error("iflavor must be 0 or 1");
     check_layout(global, local, proc_coords);
+    check_xunit(xunit.a);
     hipStream_t st = as_stream(stream);
     c->gdz.ensure(2 * 40 * 144 * 2);
     c->bmat.ensure(2 * 288 * 48);
@@ -578,6 +586,7 @@ void pack_row(const PairRow& p, int chain, double* r) {
 void window_entry(hlbl_contractor* c, const hlbl_pair_desc* pairs, int n, double xunit, int form,
                   double* out, hipStream_t st) {
   if (form < 0 || form > 2) throw Error("form must be 0, 1 or 2");
+  check_xunit(xunit);
   for (int i = 0; i < n; ++i)
     if (!pairs[i].d_x || !pairs[i].d_y) throw Error("a pair has a null block");
   if (n == 0) return;
@@ -771,6 +780,7 @@ int hlbl_run_chain_2p2(hlbl_contractor* c, const hlbl_schedule* s, uint64_t seed
     if (chain < 0) throw Error("chain must be non-negative");
     if (batch < 0 || batch > 2) throw Error("batch must be 0 (pairs), 1 (windows) or 2 (sources)");
     if (n_plan < 0) throw Error("n_plan < 0");
+    check_xunit(xunit);
     if (n_plan > 0) check_ptr(plan, "plan");
     if (iflavor != 0 && iflavor != 1) throw Error("iflavor must be 0 or 1");
     std::map<Vec4, Row2p2> plan2;

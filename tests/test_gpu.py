@@ -437,6 +437,12 @@ def test_tuning_shapes(qed):
         assert np.all(np.abs(got - base) <= 1e-12 * np.abs(base)), t
         assert np.all(np.abs(cold - base) <= 1e-12 * np.abs(base)), t
         assert np.all(np.abs(got64 - base64) <= 1e-12 * np.abs(base64)), t
+    # a non-positive or non-finite xunit is refused by the kernel-sum and window entries
+    for bad in (0.0, -0.3, float("nan"), float("inf")):
+        with pytest.raises(H.HlblError):
+            con.window_kernel_sum(pairs, bad, form=1)
+        with pytest.raises(H.HlblError):
+            con.kernel_sum(*pairs[0], bad)
     for bad in ({"field_waves": 6}, {"field_waves": 8, "field_wg_per_cu": 3}, {"pass1_batch": 0}):
         with pytest.raises(H.HlblError):
             con.set_tuning(**bad)
