@@ -205,7 +205,7 @@ constexpr int kH1Sites = 8;                       // sites per chunk
 constexpr int kH1Units = 2 * 12 * kH1Sites * 12;  // 16-byte units of a chunk (S and Y rows)
 constexpr int kH1Fetch = kH1Units / 256;          // per producer lane
 static_assert(kH1Fetch * 256 == kH1Units, "the producers cover a chunk");
-constexpr int kH1Row = kH1Sites * 24 + 8;         // FP32 staging floats per row a (padded)
+constexpr int kH1Row = kH1Sites * 24 + 4;         // FP32 staging floats per row a (padded)
 constexpr int kH1F = 2 * 12 * kH1Row;             // [isy][a][site][24]
 constexpr int kH1Frag = 64 * 8;                   // FP16 of one fragment (a ds_read_b128 per lane)
 constexpr int kH1H = 2 * 2 * 2 * 3 * 2 * kH1Frag;  // [A, B][g][ks][tile][hi, lo][lane][8]
