@@ -241,10 +241,10 @@ __global__ __launch_bounds__(64) void judge_coef(
 // Sixteen waves, one barrier per group:
 //  - waves 0..9 run the 27 MFMAs of their block for group g and dot the result
 //    with the coefficients (the sum is linear, so the K halves dot separately);
+//    waves 0..7 then stage group g+1's coefficients (direct loads into LDS);
 //  - waves 10..15 (producers) store group g+2's spinors (fetched into registers
-//    during the previous step) as FP32 with their per-site maxima, stage group
-//    g+1's coefficients (direct loads into LDS), build dxu of group g+1 (scaled,
-//    split hi/lo), fetch group g+3.
+//    during the previous step) as FP32 with their per-site maxima, build dxu of
+//    group g+1 (scaled, split hi/lo), fetch group g+3.
 // Every MFMA wave reads the whole A tile of a group from LDS; at 8 sites per
 // group that is half the LDS traffic per site of a 16-row tile.
 constexpr int kS2Mfma = 10;                // MFMA waves: 5 column blocks of 32 x 2 halves of K
@@ -422,7 +422,7 @@ __device__ inline void s2_store(int ptid, const S2Lane& L, const d2v (&r)[kS2Fet
   if ((ptid & 7) == 0) atomicMax(smax + L.key, (unsigned)mg);
 }
 
-// A producer stages the 16 x 40 coefficients of site z0 + p (relative to gsx) into
+// An MFMA wave stages the 16 x 40 coefficients of site z0 + p (relative to gsx) into
 // dst + p * kS2CoefSite with direct loads (lane l: 16 bytes at 16 l). p, z0 and the
 // pair are wave-uniform, so the site arithmetic runs on the scalar unit (32-bit:
 // the volume is below 2^31).
@@ -557,13 +557,12 @@ __device__ inline void s2_dxu_pairs(int p0, int lane, const float* buf, const un
 // rounds of pair tasks (a round of 64 pair tasks is 128 tasks; a pair reads S once for
 // two tasks, fewer LDS round trips on the SIMDs that set the step: pass 2 2.6% faster); those of
 // waves = 2 and 3 carry two MFMA waves and producers 10 + 14 or 11 + 15 with 3 + 2
-// single tasks, and these four stage the coefficients of two sites each. Ranges: pw 2:
+// single tasks. Ranges: pw 2:
 // pairs [0, 128) (tasks [0, 256)), pw 3: pairs [128, 256), pw 0: tasks [512, 704),
 // pw 1: [704, 896), pw 4: [896, 1024), pw 5: [1024, 1152). (Pairs on all six
 // producers measured the same: the SIMDs of waves = 2 and 3 do not set the step.)
 __device__ inline int s2_task0(int pw) { return pw == 2 ? 0 : pw == 3 ? 128 : pw == 0 ? 512 : pw == 1 ? 704 : pw == 4 ? 896 : 1024; }
 __device__ inline int s2_n_tasks(int pw) { return (pw == 2 || pw == 3) ? 4 : (pw == 0 || pw == 1) ? 3 : 2; }
-__device__ inline int s2_coef_site0(int pw) { return pw == 0 ? 0 : pw == 1 ? 2 : pw == 4 ? 4 : pw == 5 ? 6 : -1; }
 static_assert(2 * 256 + 2 * 192 + 2 * 128 == kS2Tasks, "the producer task ranges cover a group");
 
 __global__ __launch_bounds__(kS2Threads) void pass2_window(
@@ -590,20 +589,14 @@ __global__ __launch_bounds__(kS2Threads) void pass2_window(
   if (producer) {
     const int pw = __builtin_amdgcn_readfirstlane(wave - kS2Mfma), ptid = tid - 64 * kS2Mfma;
     const S2Lane L = s2_lane(ptid, V);
-    const int t0 = s2_task0(pw), cs0 = s2_coef_site0(pw);
+    const int t0 = s2_task0(pw);
     S2Cursor fc, nc;  // the step being fetched, the step being prepared
     d2v r[kS2Fetch];
     auto fetch = [&]() {
       s2_fetch(L, pairs + (fc.u >> 1), fc.u & 1, V, begin + (size_t)fc.g * kS2Sites, r);
       fc.next(ng);
     };
-    auto prepare = [&](int it) {  // coefficients and dxu of step it (= nc)
-      if (cs0 >= 0) {
-        const WinPair& wp = pairs[nc.u >> 1];
-        const size_t z0 = begin + (size_t)nc.g * kS2Sites;
-        s2_coef(cs0, wp, global, V, z0, end, coef, sCoef[it & 1]);
-        s2_coef(cs0 + 1, wp, global, V, z0, end, coef, sCoef[it & 1]);
-      }
+    auto prepare = [&](int it) {  // dxu of step it (= nc)
       unsigned* ah = reinterpret_cast<unsigned*>(sA[it & 1][0]);
       unsigned* al = reinterpret_cast<unsigned*>(sA[it & 1][1]);
       const int nt = s2_n_tasks(pw);
@@ -612,9 +605,9 @@ __global__ __launch_bounds__(kS2Threads) void pass2_window(
       else s2_dxu<2>(t0, lane, sF[it & 1], sMax[it % 3], ah, al, sSe[it & 1]);
       nc.next(ng);
     };
-    // prologue: spinors of steps 0 and 1 staged; coefficients and dxu of step 0 built
-    // after the barrier, then step 2's fetch issued (it flies through the loop's first
-    // barrier, as every later fetch does)
+    // prologue: spinors of steps 0 and 1 staged; the dxu of step 0 built after the
+    // barrier, then step 2's fetch issued (it flies through the loop's first barrier,
+    // as every later fetch does)
     fetch();
     s2_store(ptid, L, r, sF[0], sMax[0]);
     if (total > 1) {
@@ -627,7 +620,7 @@ __global__ __launch_bounds__(kS2Threads) void pass2_window(
     if (inflight) fetch();
     int m3 = 0;  // it % 3
     for (int it = 0; it < total; ++it) {
-      // waits for this step's coefficient loads but lets the later fetch fly
+      // lets the fetch (the last 6 loads) fly
       if (inflight) barrier_vm6(); else barrier_all();
       // maxima of group it were read by its dxu (last step): clear them for group it+3
       if (ptid < kS2Keys) sMax[m3][ptid] = 0u;
@@ -650,6 +643,16 @@ __global__ __launch_bounds__(kS2Threads) void pass2_window(
   int te = 0;
   Kahan acc;
   S2Cursor cur;
+  // coefficients: MFMA wave w < 8 stages site w of a step with direct loads into LDS,
+  // step 0's before the prologue barrier and step it + 1's after step it's products
+  // (they land while the wave waits at the barrier, whose vmcnt(0) covers them; the
+  // producers then issue no direct loads, which the compiler would await before their
+  // LDS writes)
+  auto coefs = [&](const S2Cursor& c, int it) {
+    if (wave < kS2Sites)
+      s2_coef(wave, pairs[c.u >> 1], global, V, begin + (size_t)c.g * kS2Sites, end, coef, sCoef[it & 1]);
+  };
+  coefs(cur, 0);
   barrier_all();  // the producers' prologue
   for (int it = 0; it < total; ++it, cur.next(ng)) {
     const int u = cur.u;
@@ -701,6 +704,11 @@ __global__ __launch_bounds__(kS2Threads) void pass2_window(
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
       if (lane == 0) part[((size_t)u * nwg + blockIdx.x) * kS2Mfma + wave] = s;
+    }
+    if (it + 1 < total) {
+      S2Cursor nx = cur;
+      nx.next(ng);
+      coefs(nx, it + 1);
     }
   }
 }
